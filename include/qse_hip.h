@@ -1,0 +1,254 @@
+/*
+ * qse_hip.h — C ABI of the MI355X (gfx950) state-vector library `libqse_hip.so`.
+ *
+ * This is the thin, exception-free boundary that the C++ class
+ * `CircuitUnitaryOperationHIP` (include/quantumsimevo/CircuitUnitaryOperationHIP.h)
+ * calls. Signatures use only plain C types: no torch, no HIP types.
+ *
+ * Conventions (identical to the reference simulator):
+ *  - qubit q is bit q of the amplitude index (qubit 0 = least significant bit),
+ *    QSETests/QSETests.cpp:131, StateVector.cpp:262-265;
+ *  - a state of n qubits holds 2^n amplitudes, stored contiguously and
+ *    interleaved (re, im): FP64 pairs (QSE_C128, 16 B/amplitude) or FP32 pairs
+ *    (QSE_C64, 8 B/amplitude). Host buffers crossing this ABI are always FP64
+ *    (re, im) pairs, whatever the device precision;
+ *  - gate matrices follow QubitUnitaryOperation.cpp:10-71 and
+ *    CircuitUnitaryOperation.cpp:10-338.
+ *
+ * Every function returns a status code (QSE_OK == 0). qse_status_string() gives a
+ * message for it and qse_last_error() a detailed message for the last failure on
+ * the calling thread. The reference aborts on device errors
+ * (CircuitUnitaryOperationGPU.cu:13-21); this ABI reports them instead so callers
+ * (C++, Python) can raise.
+ *
+ * Stream semantics: a context owns one HIP stream. Gate and readout calls are
+ * stream-ordered; calls that return host data (download, sample, measure,
+ * expectation, gradients) synchronise the context stream before returning.
+ * A context is not thread-safe: one host thread drives it at a time, as the
+ * reference backend (CircuitUnitaryOperationGPU.h:39-47).
+ */
+#ifndef QSE_HIP_H
+#define QSE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QSE_ABI_VERSION 1
+
+/* ---- status codes ---- */
+enum qse_status {
+    QSE_OK = 0,
+    QSE_ERR_INVALID_ARG = 1,   /* bad qubit index, null pointer, bad size */
+    QSE_ERR_NO_DEVICE = 2,     /* no HIP device visible */
+    QSE_ERR_OUT_OF_MEMORY = 3, /* device allocation failed */
+    QSE_ERR_HIP = 4,           /* a HIP runtime call failed */
+    QSE_ERR_UNSUPPORTED = 5,   /* gate kind / precision / size not supported */
+    QSE_ERR_LIMIT = 6          /* request exceeds a documented limit */
+};
+
+/* ---- precisions ---- */
+enum qse_precision {
+    QSE_C128 = 0, /* complex<double> storage and arithmetic */
+    QSE_C64 = 1   /* complex<float> storage and arithmetic; FP64 reductions */
+};
+
+/* ---- gate kinds: the numbering of GateType, Structurs.h:10 ---- */
+enum qse_gate_kind {
+    QSE_GATE_IDENTITY = 0,
+    QSE_GATE_X = 1,
+    QSE_GATE_Y = 2,
+    QSE_GATE_Z = 3,
+    QSE_GATE_H = 4,
+    QSE_GATE_PHASE = 5,
+    QSE_GATE_RX = 6,
+    QSE_GATE_RY = 7,
+    QSE_GATE_RZ = 8,
+    QSE_GATE_CNOT = 9,
+    QSE_GATE_CPHASE = 10,
+    QSE_GATE_MCPHASE = 11,
+    QSE_GATE_SWAP = 12,
+    /* GateType::MEASURE (13) and RESET_QUBIT (14) have no dispatch in the
+     * reference (Circuit.cpp:110-111) and are rejected here. */
+    QSE_GATE_MATRIX = 32 /* generic 2x2, ICircuitUnitaryOperation::applyGate */
+};
+
+/*
+ * One gate of a gate list; the C counterpart of GateCommand (Structurs.h:12-20).
+ *  target   : target qubit (SWAP: first qubit, Circuit.cpp:59-62)
+ *  control  : CNOT/CPHASE control, SWAP second qubit; -1 when unused
+ *  controls : MCPHASE control mask, bit q = qubit q (GateCommand::controls_list)
+ *  angle    : theta, or theta + scale * params[param] when param >= 0
+ *             (differentiable RX, RY, RZ, PHASE, CPHASE)
+ *  m        : MATRIX kind only: row-major 2x2, (re, im) interleaved (Gate2x2)
+ */
+typedef struct qse_op {
+    int32_t kind;
+    int32_t target;
+    int32_t control;
+    int32_t param;
+    uint64_t controls;
+    double theta;
+    double scale;
+    double m[8];
+} qse_op; /* 104 bytes */
+
+typedef struct qse_ctx qse_ctx;
+typedef struct qse_state qse_state;
+
+/* ---- library / device ---- */
+int qse_abi_version(void);
+const char* qse_status_string(int status);
+const char* qse_last_error(void);
+/* Number of HIP devices (0 when none). Replaces the cudaGetDeviceCount probe of
+ * CircuitUnitaryOperationFactory.cpp:17-24. */
+int qse_device_count(int* count);
+/* Free / total device memory of `device` (CircuitUnitaryOperationFactory.cpp:41-63). */
+int qse_device_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
+/* Device name, compute units and HBM size (HardwareCheck.cpp:35-54). */
+int qse_device_info(int device, char* name, size_t name_len, int* compute_units, size_t* total_bytes);
+
+/* ---- context: one device, one stream (CircuitUnitaryOperationGPU.cu:140-156) ---- */
+int qse_ctx_create(int device, qse_ctx** out);
+int qse_ctx_destroy(qse_ctx* ctx);
+int qse_ctx_sync(qse_ctx* ctx);
+/* Fusion shape used by qse_run(): tile_bits = qubits per LDS tile (k),
+ * low_bits = contiguous low qubits always in the tile (coalescing run of
+ * 2^low_bits amplitudes). 0 selects the tuned default for the precision. */
+int qse_ctx_set_fusion(qse_ctx* ctx, int precision, int tile_bits, int low_bits);
+int qse_ctx_get_fusion(qse_ctx* ctx, int precision, int* tile_bits, int* low_bits);
+/* Kernel timing: when enabled, every kernel launched by the context records HIP
+ * events; qse_ctx_timing() returns the count and summed milliseconds per kernel
+ * family. Families: 0 pair sweep, 1 fused tile pass, 2 readout, 3 adjoint. */
+int qse_ctx_enable_timing(qse_ctx* ctx, int enable);
+int qse_ctx_timing(qse_ctx* ctx, int family, int64_t* launches, double* total_ms, double* bytes);
+int qse_ctx_reset_timing(qse_ctx* ctx);
+
+/* ---- state storage (device mirror of StateVector, StateVector.h:17-47) ---- */
+int qse_state_create(qse_ctx* ctx, int nqubits, int precision, qse_state** out);
+int qse_state_destroy(qse_ctx* ctx, qse_state* st);
+int qse_state_info(const qse_state* st, int* nqubits, int* precision, void** device_ptr, size_t* bytes);
+/* Host <-> device copies of the whole state; host layout is FP64 (re, im) pairs.
+ * Upload replaces ensureOnDevice's H2D copy (CircuitUnitaryOperationGPU.cu:158-180),
+ * download replaces flush's D2H copy (:182-188). Download synchronises. */
+int qse_state_upload(qse_ctx* ctx, qse_state* st, const double* host_amps);
+int qse_state_download(qse_ctx* ctx, qse_state* st, double* host_amps);
+/* Partial download of amplitudes [first, first+count). */
+int qse_state_download_range(qse_ctx* ctx, qse_state* st, uint64_t first, uint64_t count, double* host_amps);
+/* |index> (StateVector::reset with index 0, StateVector.cpp:34-46). */
+int qse_state_set_basis(qse_ctx* ctx, qse_state* st, uint64_t index);
+int qse_state_copy(qse_ctx* ctx, qse_state* dst, const qse_state* src);
+/* Seeded random normalised state, generated on the device (synthetic input of
+ * the large-state benchmarks; no reference counterpart). Amplitude i is
+ * r*(cos 2 pi u2, sin 2 pi u2), r = sqrt(-2 ln u1) (Box-Muller), with
+ * h1 = splitmix64(seed ^ (2i+1) * 0x9E3779B97F4A7C15), h2 = splitmix64(h1),
+ * u1 = ((h1 >> 11) + 1) * 2^-53, u2 = (h2 >> 11) * 2^-53; then the state is
+ * divided by its FP64 norm. */
+int qse_state_set_random(qse_ctx* ctx, qse_state* st, uint64_t seed);
+
+/* ---- single gates: one coalesced strided sweep per call ----
+ * CircuitUnitaryOperation.cpp line ranges each replaces are cited in the
+ * implementation; the kernels are independent of the reference's CUDA file. */
+int qse_apply_matrix(qse_ctx* ctx, qse_state* st, int q, const double m[8]); /* applyGate */
+int qse_apply_hadamard(qse_ctx* ctx, qse_state* st, int q);
+int qse_apply_x(qse_ctx* ctx, qse_state* st, int q);
+int qse_apply_y(qse_ctx* ctx, qse_state* st, int q);
+int qse_apply_z(qse_ctx* ctx, qse_state* st, int q);
+int qse_apply_phase(qse_ctx* ctx, qse_state* st, int q, double theta);
+int qse_apply_rx(qse_ctx* ctx, qse_state* st, int q, double theta);
+int qse_apply_ry(qse_ctx* ctx, qse_state* st, int q, double theta);
+int qse_apply_rz(qse_ctx* ctx, qse_state* st, int q, double theta);
+int qse_apply_cnot(qse_ctx* ctx, qse_state* st, int control, int target);
+int qse_apply_cphase(qse_ctx* ctx, qse_state* st, int control, int target, double theta);
+int qse_apply_mcphase(qse_ctx* ctx, qse_state* st, const int* controls, int ncontrols, int target, double theta);
+int qse_apply_swap(qse_ctx* ctx, qse_state* st, int q0, int q1);
+/* One Pauli frame: psi'[i] = (-1)^popcount(i & zmask) * psi[i ^ xmask]; equal to
+ * Noise::applyPauliNoise (Noise.cpp:34-44) with xmask/zmask the odd flip counts. */
+int qse_pauli_frame(qse_ctx* ctx, qse_state* st, uint64_t xmask, uint64_t zmask);
+
+/* ---- fused execution of a gate list (Circuit::execute, Circuit.cpp:64-120) ----
+ * The list is planned into LDS-tiled passes (gates on tile qubits and every
+ * diagonal gate fused into one HBM read+write of the state). params may be NULL
+ * when no op has param >= 0. */
+int qse_run(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params);
+/* Same list, one single-gate sweep per op (no fusion): the drop-in per-call
+ * path of CircuitUnitaryOperationHIP and the unfused baseline. */
+int qse_run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params);
+/* Number of HBM passes qse_run() would make for this list (planner report). */
+int qse_plan_passes(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops, int nops, int* passes);
+
+/* ---- readout (StateVector.cpp:77-156), driven by host-supplied uniforms ----
+ * u are canonical uniforms in [0,1) (std::generate_canonical<double,53>); the
+ * target of shot s is u[s] * total, total = sum |a_i|^2, and the outcome is the
+ * first index whose running probability sum is >= target (std::lower_bound over
+ * the sequential cumulative array). Outcomes equal the sequential reference on
+ * identical amplitudes: draws whose target lies inside the proven rounding band
+ * of the device cumulative sum are re-resolved by an exact sequential sum. */
+int qse_sample(qse_ctx* ctx, qse_state* st, const double* u, int64_t nshots, uint64_t* out);
+/* Collapsing measurement: outcome as StateVector::measure (first running sum >=
+ * u*total, else 0); the state becomes |outcome>. */
+int qse_measure(qse_ctx* ctx, qse_state* st, double u, uint64_t* outcome);
+/* sum_i |a_i|^2 in FP64. */
+int qse_norm2(qse_ctx* ctx, qse_state* st, double* out);
+/* Probability of qubit q being 1 (StateVector::measureQubit, StateVector.cpp:158-185). */
+int qse_prob_one(qse_ctx* ctx, qse_state* st, int q, double* out);
+/* Mid-circuit single-qubit collapse with a host uniform u: outcome = (u < P(1));
+ * renormalises (StateVector.cpp:158-185). With reset != 0 the qubit is then
+ * returned to |0> (StateVector::resetQubit, StateVector.cpp:187-221). */
+int qse_collapse_qubit(qse_ctx* ctx, qse_state* st, int q, double u, int reset, int* outcome);
+
+/* ---- expectation values of diagonal observables (FP64 accumulation) ----
+ * sum_i |a_i|^2 f(i) with f(i) = sum_k w_k * (-1)^popcount(i & masks[k])
+ * (a weighted sum of Z-strings). */
+int qse_expect_zstrings(qse_ctx* ctx, qse_state* st, const uint64_t* masks, const double* weights, int nterms, double* out);
+
+/* ---- adjoint-method gradients (no counterpart in the reference) ----
+ * For `batch` parameter sets params[b*nparams .. +nparams), evolve |0...0> by
+ * ops, evaluate E_b = <psi|O|psi> with O the Z-string observable above, and
+ * dE_b/dparams by one reverse sweep that un-applies each gate on psi and on
+ * lambda = O psi and accumulates 2 Re<lambda|dU/dtheta|psi>. ckpt_every > 0
+ * re-loads a forward checkpoint every ckpt_every differentiable gates to bound
+ * rounding drift in deep circuits (0 = no checkpoints). energies: batch values;
+ * grads: batch*nparams values. */
+int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision,
+                       const qse_op* ops, int nops,
+                       const double* params, int nparams, int batch,
+                       const uint64_t* obs_masks, const double* obs_weights, int nterms,
+                       int ckpt_every,
+                       double* energies, double* grads);
+/* Plain forward expectation for a batch of parameter sets (used by parameter
+ * shift and finite differences). */
+int qse_batch_expect(qse_ctx* ctx, int nqubits, int precision,
+                     const qse_op* ops, int nops,
+                     const double* params, int nparams, int batch,
+                     const uint64_t* obs_masks, const double* obs_weights, int nterms,
+                     double* energies);
+
+/* ---- batched circuits of uneven sizes in one persistent launch ----
+ * ncirc circuits; circuit c has nq[c] qubits (<= 16 in this entry point: the
+ * whole state lives in one workgroup's LDS) and the ops ops[op_off[c] ..
+ * op_off[c+1]). Each circuit starts from |0...0>, runs its ops, and returns
+ * E_c = <Z-string observable> (same masks/weights for every circuit, masks
+ * restricted to the circuit's qubits) and, when grads != NULL, dE_c/dparams
+ * for params[par_off[c] .. par_off[c+1]). */
+int qse_batch_run_small(qse_ctx* ctx, int precision, int ncirc,
+                        const int* nq, const int64_t* op_off, const qse_op* ops,
+                        const int64_t* par_off, const double* params,
+                        const uint64_t* obs_masks, const double* obs_weights, int nterms,
+                        double* energies, double* grads);
+
+/* ---- batched Pauli-noise trajectories (Noise.cpp semantics) ----
+ * For ntraj frames (xmask[t], zmask[t]) applied to the state, draw
+ * nshots_per_traj outcomes per trajectory from u[t*nshots + s], with the same
+ * lower_bound rule as qse_sample over the framed amplitudes
+ * psi_t[i] = psi[i ^ xmask[t]]; out[t*nshots + s]. The state is not modified. */
+int qse_sample_frames(qse_ctx* ctx, qse_state* st, const uint64_t* xmask, const uint64_t* zmask,
+                      int ntraj, const double* u, int64_t nshots_per_traj, uint64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QSE_HIP_H */

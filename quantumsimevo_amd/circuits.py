@@ -1,0 +1,234 @@
+"""Gate lists (qse_op arrays) and the synthetic workloads of the benchmark configs.
+
+`GateList` mirrors the reference's Circuit builder methods (Circuit.cpp:9-62)
+and produces the `qse_op` array the C ABI consumes. The circuit families are
+the reference's own definitions:
+  QFT / IQFT        QSEplayground/QFT.cpp:9-37
+  QAOA Max-Cut      QSEplayground/QAOA.cpp:121-138 (H layer; per layer and edge
+                    CNOT, RZ(4 gamma), CNOT; then RX(2 beta) on every qubit),
+                    with float64 angles instead of the reference's float
+and the seeded generators below define the synthetic inputs (random layered
+circuits, random normalised states, random 3-regular graphs, uneven ansatz
+batches). Generators use numpy's PCG64 (numpy.random.default_rng(seed)), so the
+same seed yields the same gate list on every host.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .capi import (CNOT, CPHASE, H, IDENTITY, MATRIX, MCPHASE, OP_DTYPE, PHASE, RX, RY, RZ, SWAP, X, Y, Z)
+
+PI = 3.14159265358979323846  # QFT.cpp:7
+
+
+class GateList:
+    """Builder for a qse_op list with the reference Circuit's argument order."""
+
+    def __init__(self, num_qubits: int):
+        self.n = num_qubits
+        self._ops: list[tuple] = []
+
+    def _add(self, kind, target, control=-1, theta=0.0, param=-1, scale=0.0, controls=0, m=None):
+        self._ops.append((kind, target, control, param, controls, theta, scale,
+                          tuple(m) if m is not None else (0.0,) * 8))
+        return self
+
+    # single-qubit gates (Circuit.cpp:9-39)
+    def hadamard(self, q): return self._add(H, q)
+    def pauli_x(self, q): return self._add(X, q)
+    def pauli_y(self, q): return self._add(Y, q)
+    def pauli_z(self, q): return self._add(Z, q)
+    def phase(self, theta, q, param=-1, scale=0.0): return self._add(PHASE, q, theta=theta, param=param, scale=scale)
+    def rotate_x(self, theta, q, param=-1, scale=0.0): return self._add(RX, q, theta=theta, param=param, scale=scale)
+    def rotate_y(self, theta, q, param=-1, scale=0.0): return self._add(RY, q, theta=theta, param=param, scale=scale)
+    def rotate_z(self, theta, q, param=-1, scale=0.0): return self._add(RZ, q, theta=theta, param=param, scale=scale)
+
+    # multi-qubit gates (Circuit.cpp:41-62)
+    def cnot(self, control, target): return self._add(CNOT, target, control)
+
+    def cphase(self, theta, control, target, param=-1, scale=0.0):
+        return self._add(CPHASE, target, control, theta=theta, param=param, scale=scale)
+
+    def mcphase(self, theta, controls, target):
+        mask = 0
+        for c in controls:
+            mask |= 1 << int(c)
+        return self._add(MCPHASE, target, -1, theta=theta, controls=mask)
+
+    def swap(self, q0, q1): return self._add(SWAP, q0, q1)
+
+    def gate(self, m, q):
+        """Generic 2x2 (ICircuitUnitaryOperation::applyGate); m row-major complex."""
+        mm = np.asarray(m, dtype=np.complex128).reshape(4)
+        flat = []
+        for z in mm:
+            flat += [float(z.real), float(z.imag)]
+        return self._add(MATRIX, q, m=flat)
+
+    def identity(self, q=0): return self._add(IDENTITY, q)
+
+    def extend(self, other: "GateList"):
+        self._ops.extend(other._ops)
+        return self
+
+    def __len__(self):
+        return len(self._ops)
+
+    def ops(self) -> np.ndarray:
+        a = np.zeros(len(self._ops), dtype=OP_DTYPE)
+        for i, (k, t, c, p, cm, th, sc, m) in enumerate(self._ops):
+            a[i] = (k, t, c, p, cm, th, sc, m)
+        return a
+
+
+# ---- circuit families ----
+
+def qft(n: int, qubits=None) -> GateList:
+    """QFT.cpp:9-22: H(q_i), CP(pi/2^(j-i), q_j -> q_i) for j > i, then swaps."""
+    qs = list(range(n)) if qubits is None else list(qubits)
+    g = GateList(n)
+    m = len(qs)
+    for i in range(m):
+        g.hadamard(qs[i])
+        for j in range(i + 1, m):
+            g.cphase(PI / math.pow(2.0, j - i), qs[j], qs[i])
+    for i in range(m // 2):
+        g.swap(qs[i], qs[m - 1 - i])
+    return g
+
+
+def iqft(n: int, qubits=None) -> GateList:
+    """QFT.cpp:24-37."""
+    qs = list(range(n)) if qubits is None else list(qubits)
+    g = GateList(n)
+    m = len(qs)
+    for i in range(m // 2):
+        g.swap(qs[i], qs[m - 1 - i])
+    for i in range(m - 1, -1, -1):
+        for j in range(m - 1, i, -1):
+            g.cphase(-PI / math.pow(2.0, j - i), qs[j], qs[i])
+        g.hadamard(qs[i])
+    return g
+
+
+def random_layered(n: int, layers: int, seed: int) -> GateList:
+    """Random layered circuit: per layer one of {H, RX, RY, RZ, phase} per qubit
+    (uniform choice, angle uniform in [0, 2pi), always drawn), then a CNOT
+    brickwork on (q, q+1) starting at q = 0 on even layers and q = 1 on odd ones."""
+    rng = np.random.default_rng(seed)
+    g = GateList(n)
+    for layer in range(layers):
+        for q in range(n):
+            kind = int(rng.integers(0, 5))
+            theta = float(rng.random() * 2.0 * math.pi)
+            if kind == 0:
+                g.hadamard(q)
+            elif kind == 1:
+                g.rotate_x(theta, q)
+            elif kind == 2:
+                g.rotate_y(theta, q)
+            elif kind == 3:
+                g.rotate_z(theta, q)
+            else:
+                g.phase(theta, q)
+        for q in range(layer % 2, n - 1, 2):
+            g.cnot(q, q + 1)
+    return g
+
+
+def random_state(n: int, seed: int) -> np.ndarray:
+    """Complex Gaussian amplitudes, normalised in FP64."""
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    a /= np.sqrt(np.sum(np.abs(a) ** 2))
+    return a
+
+
+def random_regular_graph(n: int, degree: int, seed: int) -> list[tuple[int, int]]:
+    """Random d-regular simple graph by the configuration model with rejection."""
+    rng = np.random.default_rng(seed)
+    while True:
+        stubs = np.repeat(np.arange(n), degree)
+        rng.shuffle(stubs)
+        pairs = stubs.reshape(-1, 2)
+        edges = set()
+        ok = True
+        for a, b in pairs:
+            a, b = int(a), int(b)
+            if a == b or (min(a, b), max(a, b)) in edges:
+                ok = False
+                break
+            edges.add((min(a, b), max(a, b)))
+        if ok:
+            return sorted(edges)
+
+
+def qaoa_maxcut(n: int, edges, p: int) -> GateList:
+    """QAOA.cpp:121-138 with parameters: gamma_l = params[l], beta_l = params[p + l].
+    RZ angle = 4 * gamma (scale 4), RX angle = 2 * beta (scale 2)."""
+    g = GateList(n)
+    for q in range(n):
+        g.hadamard(q)
+    for layer in range(p):
+        for (a, b) in edges:
+            g.cnot(a, b)
+            g.rotate_z(0.0, b, param=layer, scale=4.0)
+            g.cnot(a, b)
+        for q in range(n):
+            g.rotate_x(0.0, q, param=p + layer, scale=2.0)
+    return g
+
+
+def maxcut_observable(edges):
+    """<C> = sum_edges (1 - Z_a Z_b)/2 as Z-strings: masks and weights
+    (constant term as mask 0)."""
+    masks = [0] + [(1 << a) | (1 << b) for (a, b) in edges]
+    weights = [0.5 * len(edges)] + [-0.5] * len(edges)
+    return np.array(masks, dtype=np.uint64), np.array(weights, dtype=np.float64)
+
+
+def zsum_observable(n: int):
+    """sum_i Z_i."""
+    return np.array([1 << i for i in range(n)], dtype=np.uint64), np.ones(n)
+
+
+def ansatz(n: int, depth: int) -> GateList:
+    """Layers of RY(theta) RZ(phi) per qubit then a linear CNOT ladder; the
+    parameters are numbered in gate order (2 per qubit per layer)."""
+    g = GateList(n)
+    k = 0
+    for _ in range(depth):
+        for q in range(n):
+            g.rotate_y(0.0, q, param=k, scale=1.0)
+            g.rotate_z(0.0, q, param=k + 1, scale=1.0)
+            k += 2
+        for q in range(n - 1):
+            g.cnot(q, q + 1)
+    return g
+
+
+def amp_updates(ops: np.ndarray, n: int) -> float:
+    """Algorithmic amplitude updates sum_g T(g): N for H, X, Y, RX, RY, RZ,
+    generic 2x2; N/2 for Z, phase, CNOT, SWAP; N/4 for CPhase; N/2^m for
+    MCPhase with m distinct pinned qubits (0 for identity, self-swap, c == t CNOT)."""
+    N = float(1 << n)
+    tot = 0.0
+    for o in ops:
+        k = int(o["kind"])
+        if k in (H, X, Y, RX, RY, RZ, MATRIX):
+            tot += N
+        elif k in (Z, PHASE):
+            tot += N / 2
+        elif k == CNOT:
+            tot += 0.0 if o["control"] == o["target"] else N / 2
+        elif k == SWAP:
+            tot += 0.0 if o["control"] == o["target"] else N / 2
+        elif k == CPHASE:
+            m = 1 if o["control"] == o["target"] else 2
+            tot += N / (1 << m)
+        elif k == MCPHASE:
+            mask = int(o["controls"]) | (1 << int(o["target"]))
+            tot += N / (1 << bin(mask).count("1"))
+    return tot

@@ -1,0 +1,184 @@
+// Circuit.cpp — command list and replay (reference Circuit.cpp:5-143) on the
+// HIP backend.
+#include "quantumsimevo/Circuit.h"
+
+#include <iostream>
+#include <stdexcept>
+
+#include "quantumsimevo/CircuitUnitaryOperationFactory.h"
+#include "quantumsimevo/CircuitUnitaryOperationHIP.h"
+
+Circuit::Circuit(size_t N, bool simulatePauliNoise, int precision)
+    : stateVector(N), simulateNoise(simulatePauliNoise), pauliNoise(N, 42, 1) {
+    circuitOp = CircuitUnitaryOperationFactory::create(N, precision);
+    std::random_device rd;
+    readoutGen.seed(rd());
+}
+
+static void check_q(const StateVector& sv, size_t q) {
+    if (q >= sv.qubits()) throw std::out_of_range("qubit index " + std::to_string(q) + " out of range");
+}
+
+// Circuit.cpp:9-62: one GateCommand per call (argument order as the reference).
+void Circuit::hadamard(size_t q) {
+    check_q(stateVector, q);
+    commandList.push_back({GateType::H, q, 0, 0.0, "Hadamard on qubit " + std::to_string(q)});
+}
+void Circuit::pauliX(size_t q) {
+    check_q(stateVector, q);
+    commandList.push_back({GateType::X, q, 0, 0.0, "PauliX on qubit " + std::to_string(q)});
+}
+void Circuit::pauliZ(size_t q) {
+    check_q(stateVector, q);
+    commandList.push_back({GateType::Z, q, 0, 0.0, "PauliZ on qubit " + std::to_string(q)});
+}
+void Circuit::pauliY(size_t q) {
+    check_q(stateVector, q);
+    commandList.push_back({GateType::Y, q, 0, 0.0, "PauliY on qubit " + std::to_string(q)});
+}
+void Circuit::phase(double theta, size_t q) {
+    check_q(stateVector, q);
+    commandList.push_back({GateType::PHASE, q, 0, theta, "Phase shift on qubit " + std::to_string(q)});
+}
+void Circuit::rotateX(double theta, size_t q) {
+    check_q(stateVector, q);
+    commandList.push_back({GateType::RX, q, 0, theta, "RX on qubit " + std::to_string(q)});
+}
+void Circuit::rotateY(double theta, size_t q) {
+    check_q(stateVector, q);
+    commandList.push_back({GateType::RY, q, 0, theta, "RY on qubit " + std::to_string(q)});
+}
+void Circuit::rotateZ(double theta, size_t q) {
+    check_q(stateVector, q);
+    commandList.push_back({GateType::RZ, q, 0, theta, "RZ on qubit " + std::to_string(q)});
+}
+void Circuit::cnot(size_t control, size_t target) {
+    check_q(stateVector, control);
+    check_q(stateVector, target);
+    commandList.push_back({GateType::CNOT, target, control, 0.0,
+                           "CNOT (control: " + std::to_string(control) + ", target: " + std::to_string(target) + ")"});
+}
+void Circuit::cphase(double theta, size_t control, size_t target) {
+    check_q(stateVector, control);
+    check_q(stateVector, target);
+    commandList.push_back({GateType::CPHASE, target, control, theta,
+                           "CPhase (theta: " + std::to_string(theta) + ", control: " + std::to_string(control) +
+                               ", target: " + std::to_string(target) + ")"});
+}
+void Circuit::mcphase(double theta, const std::vector<size_t>& controls, size_t target) {
+    check_q(stateVector, target);
+    std::string desc = "MCPhase (theta: " + std::to_string(theta) + ", target: " + std::to_string(target) + ", controls:";
+    for (size_t c : controls) {
+        check_q(stateVector, c);
+        desc += " " + std::to_string(c);
+    }
+    desc += ")";
+    GateCommand cmd{GateType::MCPHASE, target, 0, theta, desc, 0, controls};
+    commandList.push_back(cmd);
+}
+void Circuit::swap(size_t q0, size_t q1) {
+    check_q(stateVector, q0);
+    check_q(stateVector, q1);
+    commandList.push_back({GateType::SWAP, q0, q1, 0.0,
+                           "SWAP (q0: " + std::to_string(q0) + ", q1: " + std::to_string(q1) + ")"});
+}
+
+void Circuit::gate(const Gate2x2& g, size_t q) {
+    check_q(stateVector, q);
+    GateCommand cmd{GateType::IDENTITY, q, 0, 0.0, "Gate2x2 on qubit " + std::to_string(q)};
+    // GateType has no generic-matrix member; the matrix rides in a side table.
+    cmd.resultIdx = matrices.size();
+    matrices.push_back(g);
+    cmd.controls_list = {SIZE_MAX};
+    commandList.push_back(cmd);
+}
+
+// Circuit.cpp:64-120: replay from the current state, noise draws before each gate,
+// noise frame after the last one.
+void Circuit::execute(bool print_steps) {
+    if (print_steps) std::cout << "Initial state: |0...0>\n";
+    circuitOp->setPrintSteps(print_steps);
+    for (const auto& cmd : commandList) {
+        if (print_steps) std::cout << "Executing: " << cmd.description << std::endl;
+        if (simulateNoise) pauliNoise.accumulateError(cmd.type, cmd.target);
+        switch (cmd.type) {
+            case GateType::H: circuitOp->applyHadamard(stateVector, cmd.target); break;
+            case GateType::X: circuitOp->applyPauliX(stateVector, cmd.target); break;
+            case GateType::RX: circuitOp->applyRotateX(stateVector, cmd.target, cmd.theta); break;
+            case GateType::RY: circuitOp->applyRotateY(stateVector, cmd.target, cmd.theta); break;
+            case GateType::RZ: circuitOp->applyRotateZ(stateVector, cmd.target, cmd.theta); break;
+            case GateType::CNOT: circuitOp->applyCNOT(stateVector, cmd.control, cmd.target); break;
+            case GateType::CPHASE: circuitOp->applyCPhase(stateVector, cmd.control, cmd.target, cmd.theta); break;
+            case GateType::MCPHASE:
+                circuitOp->applyMCPhase(stateVector, cmd.controls_list, cmd.target, cmd.theta);
+                break;
+            case GateType::SWAP: circuitOp->applySwap(stateVector, cmd.target, cmd.control); break;
+            case GateType::Y: circuitOp->applyPauliY(stateVector, cmd.target); break;
+            case GateType::PHASE: circuitOp->applyPhase(stateVector, cmd.target, cmd.theta); break;
+            case GateType::Z: circuitOp->applyPauliZ(stateVector, cmd.target); break;
+            case GateType::IDENTITY:
+                if (cmd.controls_list.size() == 1 && cmd.controls_list[0] == SIZE_MAX)
+                    circuitOp->applyGate(stateVector, matrices[cmd.resultIdx], cmd.target);
+                break;
+            default: std::cerr << "Unsupported gate type: " << static_cast<int>(cmd.type) << std::endl;
+        }
+    }
+    if (simulateNoise) pauliNoise.applyPauliNoise(stateVector, *circuitOp);
+    if (print_steps)
+        std::cout << "Total of occured errors: \n BitFlips:" << pauliNoise.getTotalBitFlips()
+                  << " times\n PhaseFlips:" << pauliNoise.getTotalPhaseFlips() << " times.\n";
+}
+
+CircuitUnitaryOperationHIP& Circuit::backend() { return static_cast<CircuitUnitaryOperationHIP&>(*circuitOp); }
+
+// Circuit.cpp:122-125 with the collapse on the device: the device copy is the
+// one later gates act on, so it is the one that collapses.
+size_t Circuit::measure() {
+    std::uniform_real_distribution<double> dist(0.0, 1.0);
+    return backend().measure(stateVector, dist(readoutGen));
+}
+
+// Circuit.cpp:140-143, sampled on the device.
+std::vector<size_t> Circuit::sample(int numShots) {
+    if (numShots < 0) throw std::invalid_argument("sample: negative shot count");
+    std::uniform_real_distribution<double> dist(0.0, 1.0);
+    std::vector<double> u((size_t)numShots);
+    for (auto& x : u) x = dist(readoutGen);
+    return backend().sample(stateVector, u);
+}
+
+// Circuit.cpp:127-133.
+void Circuit::reset() {
+    stateVector.reset();
+    circuitOp->markHostDirty();
+    if (simulateNoise) pauliNoise.resetStats();
+    commandList.clear();
+    matrices.clear();
+}
+
+void Circuit::printState() {
+    circuitOp->flush();
+    stateVector.printState();
+}
+
+void Circuit::setSeed(uint64_t seed) { readoutGen.seed((std::mt19937::result_type)seed); }
+
+std::vector<Complex> Circuit::state() { return backend().readState(stateVector); }
+
+void Circuit::setState(const std::vector<Complex>& amps) { backend().setState(stateVector, amps); }
+
+double Circuit::expectationZ(const std::vector<uint64_t>& masks, const std::vector<double>& weights) {
+    return backend().expectZStrings(stateVector, masks, weights);
+}
+
+int Circuit::measureQubit(size_t q) {
+    check_q(stateVector, q);
+    std::uniform_real_distribution<double> dist(0.0, 1.0);
+    return backend().collapseQubit(stateVector, q, dist(readoutGen), false);
+}
+
+void Circuit::resetQubit(size_t q) {
+    check_q(stateVector, q);
+    std::uniform_real_distribution<double> dist(0.0, 1.0);
+    backend().collapseQubit(stateVector, q, dist(readoutGen), true);
+}

@@ -1,0 +1,98 @@
+// pymodule.cpp — pybind11 module `_quantumsimevo`: the reference's Python API
+// (QSEPythonModule/QuantumSimEvoPy.cpp:8-41: Circuit with snake_case gate
+// methods, execute/measure/sample/reset/print_state) on the HIP backend, plus
+// state I/O, expectation values and seeded readout.
+#include <pybind11/complex.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "quantumsimevo/Circuit.h"
+#include "quantumsimevo/CircuitUnitaryOperationHIP.h"
+#include "quantumsimevo/HardwareCheck.h"
+
+namespace py = pybind11;
+
+static int parse_precision(const py::object& p) {
+    if (py::isinstance<py::str>(p)) {
+        const std::string s = p.cast<std::string>();
+        if (s == "c128" || s == "complex128") return QSE_C128;
+        if (s == "c64" || s == "complex64") return QSE_C64;
+        throw std::invalid_argument("precision must be 'c128' or 'c64'");
+    }
+    return p.cast<int>();
+}
+
+PYBIND11_MODULE(_quantumsimevo, m) {
+    m.doc() = "QuantumSimEvo circuits on the MI355X (gfx950) HIP backend";
+
+    py::class_<Circuit>(m, "Circuit")
+        .def(py::init([](size_t n, bool noise, py::object precision) {
+                 return new Circuit(n, noise, parse_precision(precision));
+             }),
+             py::arg("num_qubits"), py::arg("simulate_pauli_noise") = false, py::arg("precision") = "c128")
+        // single-qubit gates (QuantumSimEvoPy.cpp:16-24)
+        .def("hadamard", &Circuit::hadamard, py::arg("q"))
+        .def("pauli_x", &Circuit::pauliX, py::arg("q"))
+        .def("pauli_y", &Circuit::pauliY, py::arg("q"))
+        .def("pauli_z", &Circuit::pauliZ, py::arg("q"))
+        .def("phase", &Circuit::phase, py::arg("theta"), py::arg("q"))
+        .def("rotate_x", &Circuit::rotateX, py::arg("theta"), py::arg("q"))
+        .def("rotate_y", &Circuit::rotateY, py::arg("theta"), py::arg("q"))
+        .def("rotate_z", &Circuit::rotateZ, py::arg("theta"), py::arg("q"))
+        // multi-qubit gates (QuantumSimEvoPy.cpp:26-32)
+        .def("cnot", &Circuit::cnot, py::arg("control"), py::arg("target"))
+        .def("cphase", &Circuit::cphase, py::arg("theta"), py::arg("control"), py::arg("target"))
+        .def("mcphase", &Circuit::mcphase, py::arg("theta"), py::arg("controls"), py::arg("target"))
+        .def("swap", &Circuit::swap, py::arg("q0"), py::arg("q1"))
+        // execution & measurement (QuantumSimEvoPy.cpp:34-38)
+        .def("execute", &Circuit::execute, py::arg("print_steps") = false)
+        .def("measure", &Circuit::measure)
+        .def("sample", &Circuit::sample, py::arg("num_shots"))
+        .def("reset", &Circuit::reset)
+        .def("print_state", &Circuit::printState)
+        // extensions
+        .def("gate", [](Circuit& c, py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> mat,
+                        size_t q) {
+                 if (mat.size() != 4) throw std::invalid_argument("gate matrix must have 4 entries");
+                 auto r = mat.unchecked();
+                 const std::complex<double>* p = mat.data();
+                 (void)r;
+                 c.gate(Gate2x2(p[0], p[1], p[2], p[3]), q);
+             },
+             py::arg("matrix"), py::arg("q"))
+        .def("set_seed", &Circuit::setSeed, py::arg("seed"))
+        .def("state", [](Circuit& c) {
+            std::vector<Complex> s = c.state();
+            py::array_t<std::complex<double>> out(s.size());
+            std::copy(s.begin(), s.end(), out.mutable_data());
+            return out;
+        })
+        .def("set_state", [](Circuit& c, py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> a) {
+                 std::vector<Complex> v(a.data(), a.data() + a.size());
+                 c.setState(v);
+             },
+             py::arg("amplitudes"))
+        .def("expectation_z", &Circuit::expectationZ, py::arg("masks"), py::arg("weights"))
+        .def("measure_qubit", &Circuit::measureQubit, py::arg("q"))
+        .def("reset_qubit", &Circuit::resetQubit, py::arg("q"))
+        .def_property_readonly("num_qubits", &Circuit::numQubits)
+        .def_property_readonly("num_commands", [](const Circuit& c) { return c.commands().size(); })
+        .def("noise_stats", [](Circuit& c) {
+            return py::dict(py::arg("bit_flips") = c.noise().getTotalBitFlips(),
+                            py::arg("phase_flips") = c.noise().getTotalPhaseFlips(),
+                            py::arg("x_mask") = c.noise().bitFlipMask(),
+                            py::arg("z_mask") = c.noise().phaseFlipMask());
+        })
+        .def("set_fusion", [](Circuit& c, bool on) { c.backend().setFusion(on); }, py::arg("fused"));
+
+    m.def("device_report", [](int device) {
+              const DeviceReport r = HardwareCheck::report(device);
+              return py::dict(py::arg("available") = r.available, py::arg("name") = r.name,
+                              py::arg("compute_units") = r.computeUnits, py::arg("total_bytes") = r.totalBytes,
+                              py::arg("free_bytes") = r.freeBytes, py::arg("max_qubits_c128") = r.maxQubitsC128,
+                              py::arg("max_qubits_c64") = r.maxQubitsC64, py::arg("host_threads") = r.hostThreads);
+          },
+          py::arg("device") = 0);
+    m.def("print_report", &HardwareCheck::printReport, py::arg("device") = 0);
+}

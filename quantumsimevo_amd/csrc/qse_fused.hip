@@ -1,0 +1,387 @@
+// qse_fused.hip — the LDS-tiled fused pass for gfx950 and its host driver.
+//
+// One launch = one pass = one HBM read and one HBM write of every amplitude
+// (algorithmic traffic 2*b*N bytes, b = 16 for complex128, 8 for complex64).
+// Workgroup w processes tiles t = w, w + grid, ...: a tile is the 2^K
+// amplitudes obtained by fixing the n-K external qubits of t and varying the
+// K tile qubits (program format and planning rules: qse_fused.hpp,
+// qse_plan.cpp). Per tile:
+//   1. per-tile external factors (products of monomial phases whose
+//      conditions are on external qubits), one slot per thread, into LDS;
+//   2. load: thread tid reads tile positions i*NT + tid (i < 16) — the low
+//      tile qubits are the state's low qubits, so a wavefront reads
+//      contiguous runs — into the LDS tile (bank-swizzled);
+//   3. stages: gather 16 amplitudes per thread over the stage's 4 register
+//      bits, apply the stage's ops in registers, scatter back;
+//   4. store with the pass's final tile-bit labelling (in-tile SWAPs).
+// LDS is XOR-swizzled in 16-byte units (position ^ fold of bits 4.. into bits
+// 0..3) so that both the contiguous load/store and the strided stage gathers
+// spread a wavefront's accesses over the banks.
+#include <cstring>
+#include <vector>
+
+#include "qse_device.hpp"
+#include "qse_fused.hpp"
+#include "qse_internal.hpp"
+
+namespace qse {
+
+void fusion_shape(const qse_ctx* ctx, int n, int prec, int* k_out, int* L_out);
+int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P, bool relabel);
+int run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params);
+
+__device__ __forceinline__ uint32_t swz(uint32_t li) { return li ^ (((li >> 4) ^ (li >> 8) ^ (li >> 12)) & 15u); }
+
+template <class R, int P>
+__device__ __forceinline__ void dense_p(cx<R>* a, const FOp& op) {
+    const uint32_t rctl = op.rctl;
+    const cx<R> m00 = {(R)op.m[0], (R)op.m[1]}, m01 = {(R)op.m[2], (R)op.m[3]};
+    const cx<R> m10 = {(R)op.m[4], (R)op.m[5]}, m11 = {(R)op.m[6], (R)op.m[7]};
+    const uint8_t type = op.type;
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+        const int j0 = ((jj >> P) << (P + 1)) | (jj & ((1 << P) - 1));
+        const int j1 = j0 | (1 << P);
+        if (((uint32_t)j0 & rctl) != rctl) continue;
+        const cx<R> v0 = a[j0], v1 = a[j1];
+        if (type == FOP_H) {
+            const R h = (R)0.7071067811865475244;
+            a[j0] = {h * (v0.re + v1.re), h * (v0.im + v1.im)};
+            a[j1] = {h * (v0.re - v1.re), h * (v0.im - v1.im)};
+        } else if (type == FOP_XSWAP) {
+            a[j0] = v1;
+            a[j1] = v0;
+        } else {
+            a[j0] = cadd(cmul(m00, v0), cmul(m01, v1));
+            a[j1] = cadd(cmul(m10, v0), cmul(m11, v1));
+        }
+    }
+}
+
+template <class R>
+__device__ __forceinline__ void apply_op(const FOp& op, cx<R>* a, uint32_t toff, uint64_t base, const FProgram& prog,
+                                         const cx<double>* slotv) {
+    if (op.type == FOP_MONO) {
+        double gr = 1.0, gi = 0.0;
+        for (uint32_t t = op.term_begin; t < op.term_end; ++t) {
+            const FTerm& T = prog.terms[t];
+            if ((toff & T.tmask) != T.tmask) continue;
+            double dr, di;
+            if (T.slot >= 0) {
+                dr = slotv[T.slot].re;
+                di = slotv[T.slot].im;
+            } else {
+                if ((base & T.emask) != T.emask) continue;
+                dr = T.d[0];
+                di = T.d[1];
+            }
+            const double nr = gr * dr - gi * di;
+            gi = gr * di + gi * dr;
+            gr = nr;
+        }
+        const cx<R> g = {(R)gr, (R)gi};
+        const uint32_t rctl = op.rctl;
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (((uint32_t)j & rctl) == rctl) a[j] = cmul(g, a[j]);
+        return;
+    }
+    if ((toff & op.tctl) != op.tctl || (base & op.ectl) != op.ectl) return;
+    switch (op.p) {
+        case 0: dense_p<R, 0>(a, op); break;
+        case 1: dense_p<R, 1>(a, op); break;
+        case 2: dense_p<R, 2>(a, op); break;
+        default: dense_p<R, 3>(a, op); break;
+    }
+}
+
+template <class R, int K>
+__global__ __launch_bounds__(1 << (K - 4)) void k_fused(const cx<R>* src, cx<R>* dst, FProgram prog, int pass_idx,
+                                                        uint64_t ntiles) {
+    constexpr int NT = 1 << (K - 4);
+    constexpr int TB = K - 4;
+    __shared__ cx<R> tile[1 << K];
+    __shared__ cx<double> slotv[FUSE_MAX_SLOTS];
+    const FPass& P = prog.passes[pass_idx];
+    const int tid = threadIdx.x;
+    uint64_t gt = 0, ot = 0;
+    uint32_t st_t = 0;
+#pragma unroll
+    for (int b = 0; b < TB; ++b)
+        if ((tid >> b) & 1) {
+            gt |= 1ULL << P.tq[b];
+            ot |= 1ULL << P.so_q[b];
+            st_t |= 1u << P.so_l[b];
+        }
+    uint64_t gw[4], ow[4];
+    uint32_t sw[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        gw[c] = 1ULL << P.tq[TB + c];
+        ow[c] = 1ULL << P.so_q[TB + c];
+        sw[c] = 1u << P.so_l[TB + c];
+    }
+    const int next = P.next, nstages = P.nstages, nslots = P.nslots;
+    for (uint64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        uint64_t base = 0, obase = 0;
+        {
+            uint64_t t = ti;
+            for (int e = 0; e < next; ++e) {
+                base |= (t & 1ULL) << P.epos[e];
+                obase |= (t & 1ULL) << P.eout[e];
+                t >>= 1;
+            }
+        }
+        for (int sl = tid; sl < nslots; sl += NT) {
+            const FSlot S = prog.slots[P.slot_begin + sl];
+            double gr = 1.0, gi = 0.0;
+            for (uint32_t e = S.begin; e < S.end; ++e) {
+                const FExtTerm& E = prog.eterms[e];
+                if ((base & E.emask) != E.emask) continue;
+                const double nr = gr * E.d[0] - gi * E.d[1];
+                gi = gr * E.d[1] + gi * E.d[0];
+                gr = nr;
+            }
+            slotv[sl] = {gr, gi};
+        }
+        // load
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            uint64_t g = base | gt;
+            if (i & 1) g |= gw[0];
+            if (i & 2) g |= gw[1];
+            if (i & 4) g |= gw[2];
+            if (i & 8) g |= gw[3];
+            tile[swz((uint32_t)(i * NT + tid))] = src[g];
+        }
+        __syncthreads();
+        for (int sg = 0; sg < nstages; ++sg) {
+            const FStage& S = prog.stages[P.stage_begin + sg];
+            uint32_t toff = 0;
+#pragma unroll
+            for (int t = 0; t < TB; ++t)
+                if ((tid >> t) & 1) toff |= 1u << S.tmap[t];
+            const uint32_t r0 = 1u << S.rmap[0], r1 = 1u << S.rmap[1], r2 = 1u << S.rmap[2], r3 = 1u << S.rmap[3];
+            uint32_t li[16];
+            cx<R> a[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                li[j] = swz(toff | ((j & 1) ? r0 : 0u) | ((j & 2) ? r1 : 0u) | ((j & 4) ? r2 : 0u) |
+                            ((j & 8) ? r3 : 0u));
+                a[j] = tile[li[j]];
+            }
+            for (uint32_t oi = S.op_begin; oi < S.op_end; ++oi) apply_op<R>(prog.ops[oi], a, toff, base, prog, slotv);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) tile[li[j]] = a[j];
+            __syncthreads();
+        }
+        // store
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            uint64_t g = obase | ot;
+            uint32_t ls = st_t;
+            if (i & 1) { g |= ow[0]; ls |= sw[0]; }
+            if (i & 2) { g |= ow[1]; ls |= sw[1]; }
+            if (i & 4) { g |= ow[2]; ls |= sw[2]; }
+            if (i & 8) { g |= ow[3]; ls |= sw[3]; }
+            dst[g] = tile[swz(ls)];
+        }
+        __syncthreads();
+    }
+}
+
+
+namespace {
+
+size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct Serialized {
+    std::vector<char> bytes;
+    size_t off[6];
+};
+
+void serialize(const HostPlan& P, Serialized& S) {
+    const size_t sz[6] = {sizeof(FPass) * P.passes.size(), sizeof(FStage) * P.stages.size(),
+                          sizeof(FOp) * P.ops.size(), sizeof(FTerm) * P.terms.size(),
+                          sizeof(FSlot) * P.slots.size(), sizeof(FExtTerm) * P.eterms.size()};
+    const void* src[6] = {P.passes.data(), P.stages.data(), P.ops.data(),
+                          P.terms.data(), P.slots.data(), P.eterms.data()};
+    size_t tot = 0;
+    for (int i = 0; i < 6; ++i) {
+        S.off[i] = tot;
+        tot += al256(std::max<size_t>(sz[i], 1));
+    }
+    S.bytes.assign(tot, 0);
+    for (int i = 0; i < 6; ++i)
+        if (sz[i]) std::memcpy(S.bytes.data() + S.off[i], src[i], sz[i]);
+}
+
+}  // namespace
+
+// Device copy of a serialized program, reused when the same program runs again.
+int program_on_device(qse_ctx* ctx, const Serialized& S, char** dev) {
+    for (auto& e : ctx->prog_cache) {
+        if (e.bytes.size() == S.bytes.size() && std::memcmp(e.bytes.data(), S.bytes.data(), S.bytes.size()) == 0) {
+            e.stamp = ++ctx->prog_clock;
+            *dev = (char*)e.dev;
+            return QSE_OK;
+        }
+    }
+    if (ctx->prog_cache.size() >= 32) {
+        size_t oldest = 0;
+        for (size_t i = 1; i < ctx->prog_cache.size(); ++i)
+            if (ctx->prog_cache[i].stamp < ctx->prog_cache[oldest].stamp) oldest = i;
+        QSE_HIP(hipStreamSynchronize(ctx->stream));
+        QSE_HIP(hipFree(ctx->prog_cache[oldest].dev));
+        ctx->prog_cache.erase(ctx->prog_cache.begin() + oldest);
+    }
+    ProgEntry e;
+    e.bytes = S.bytes;
+    QSE_HIP(hipMalloc(&e.dev, S.bytes.size()));
+    // pageable source: the copy is staged before this call returns
+    QSE_HIP(hipMemcpyAsync(e.dev, S.bytes.data(), S.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
+    e.stamp = ++ctx->prog_clock;
+    ctx->prog_cache.push_back(std::move(e));
+    *dev = (char*)ctx->prog_cache.back().dev;
+    return QSE_OK;
+}
+
+template <class R, int K>
+static void launch_k(qse_ctx* ctx, const cx<R>* src, cx<R>* dst, const FProgram& prog, int pass, uint64_t ntiles) {
+    const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)ctx->num_cus * 16);
+    k_fused<R, K><<<grid, 1 << (K - 4), 0, ctx->stream>>>(src, dst, prog, pass, ntiles);
+}
+
+template <class R>
+static int launch_pass(qse_ctx* ctx, qse_state* st, int K, const FProgram& prog, int pass, bool oop) {
+    const uint64_t ntiles = 1ULL << (st->n - K);
+    const cx<R>* s = (const cx<R>*)st->d;
+    cx<R>* d = oop ? (cx<R>*)st->shadow : (cx<R>*)st->d;
+    switch (K) {
+        case 8: launch_k<R, 8>(ctx, s, d, prog, pass, ntiles); break;
+        case 9: launch_k<R, 9>(ctx, s, d, prog, pass, ntiles); break;
+        case 10: launch_k<R, 10>(ctx, s, d, prog, pass, ntiles); break;
+        case 11: launch_k<R, 11>(ctx, s, d, prog, pass, ntiles); break;
+        case 12: launch_k<R, 12>(ctx, s, d, prog, pass, ntiles); break;
+        case 13: launch_k<R, 13>(ctx, s, d, prog, pass, ntiles); break;
+        default: return set_error(QSE_ERR_UNSUPPORTED, "tile size outside [8, 13]");
+    }
+    QSE_HIP(hipGetLastError());
+    return QSE_OK;
+}
+
+// Smallest state that goes through the fused path (below: one sweep per gate).
+constexpr int FUSE_MIN_QUBITS = 10;
+
+// The relabelling plan (SWAPs free, last pass out of place) when it saves
+// passes, else the in-place plan.
+int choose_plan(const qse_ctx* ctx, int n, int k, int L, const qse_op* ops, int nops, const double* params,
+                HostPlan& P) {
+    (void)ctx;
+    HostPlan A, B;
+    int rc = build_plan(n, k, L, ops, nops, params, A, true);
+    if (rc) return rc;
+    if (A.passes.empty() || !A.passes.back().out_of_place) {
+        P = std::move(A);
+        return QSE_OK;
+    }
+    rc = build_plan(n, k, L, ops, nops, params, B, false);
+    if (rc) return rc;
+    P = (A.passes.size() < B.passes.size()) ? std::move(A) : std::move(B);
+    return QSE_OK;
+}
+
+int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int* passes_out,
+              bool dry_run) {
+    if (st->n < FUSE_MIN_QUBITS) {
+        if (passes_out) *passes_out = nops;
+        return dry_run ? QSE_OK : run_sweeps(ctx, st, ops, nops, params);
+    }
+    int k, L;
+    fusion_shape(ctx, st->n, st->prec, &k, &L);
+    HostPlan P;
+    int rc = choose_plan(ctx, st->n, k, L, ops, nops, params, P);
+    if (rc) return rc;
+    if (passes_out) *passes_out = (int)P.passes.size();
+    if (dry_run) return QSE_OK;
+    const bool needs_shadow = !P.passes.empty() && P.passes.back().out_of_place;
+    if (needs_shadow && !st->shadow) {
+        if (hipMalloc(&st->shadow, st->bytes) != hipSuccess) {
+            // no room for a second buffer: realise SWAPs in place instead
+            (void)hipGetLastError();
+            st->shadow = nullptr;
+            P = HostPlan();
+            rc = build_plan(st->n, k, L, ops, nops, params, P, false);
+            if (rc) return rc;
+        }
+    }
+    Serialized S;
+    serialize(P, S);
+    char* dev = nullptr;
+    rc = program_on_device(ctx, S, &dev);
+    if (rc) return rc;
+    FProgram prog;
+    prog.passes = (const FPass*)(dev + S.off[0]);
+    prog.stages = (const FStage*)(dev + S.off[1]);
+    prog.ops = (const FOp*)(dev + S.off[2]);
+    prog.terms = (const FTerm*)(dev + S.off[3]);
+    prog.slots = (const FSlot*)(dev + S.off[4]);
+    prog.eterms = (const FExtTerm*)(dev + S.off[5]);
+    const double bytes = 2.0 * (double)amp_bytes(st->prec) * (double)(1ULL << st->n);
+    for (int p = 0; p < (int)P.passes.size(); ++p) {
+        TimedLaunch t;
+        launch_begin(ctx, FAM_FUSED, &t);
+        const bool oop = P.passes[p].out_of_place != 0;
+        rc = (st->prec == QSE_C128) ? launch_pass<double>(ctx, st, k, prog, p, oop)
+                                    : launch_pass<float>(ctx, st, k, prog, p, oop);
+        if (rc) return rc;
+        launch_end(ctx, &t, bytes);
+        if (oop) std::swap(st->d, st->shadow);
+    }
+    return QSE_OK;
+}
+
+}  // namespace qse
+
+using namespace qse;
+
+extern "C" int qse_plan_passes(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops, int nops, int* passes) {
+    if (!passes || nqubits < 1 || nqubits > 40 || (precision != QSE_C128 && precision != QSE_C64) || nops < 0 ||
+        (nops > 0 && !ops))
+        return set_error(QSE_ERR_INVALID_ARG, "bad argument");
+    int rc = QSE_OK;
+    // parameters do not change the plan's shape: plan with a zero parameter vector
+    int maxp = -1;
+    for (int i = 0; i < nops; ++i) maxp = std::max(maxp, (int)ops[i].param);
+    std::vector<double> zeros(maxp + 1, 0.0);
+    if (nqubits < FUSE_MIN_QUBITS) {
+        *passes = nops;
+        return QSE_OK;
+    }
+    int k, L;
+    fusion_shape(ctx, nqubits, precision, &k, &L);
+    HostPlan P;
+    rc = choose_plan(ctx, nqubits, k, L, ops, nops, zeros.data(), P);
+    if (rc) return rc;
+    *passes = (int)P.passes.size();
+    return QSE_OK;
+}
+
+extern "C" int qse_ctx_set_fusion(qse_ctx* ctx, int precision, int tile_bits, int low_bits) {
+    if (!ctx || precision < 0 || precision > 1) return set_error(QSE_ERR_INVALID_ARG, "bad argument");
+    if (tile_bits != 0 && (tile_bits < FUSE_MIN_K || tile_bits > FUSE_MAX_K))
+        return set_error(QSE_ERR_INVALID_ARG, "tile_bits must be 0 or in [8, 13]");
+    if (low_bits < 0 || low_bits > 9) return set_error(QSE_ERR_INVALID_ARG, "low_bits must be in [0, 9]");
+    ctx->tile_bits[precision] = tile_bits;
+    ctx->low_bits[precision] = low_bits;
+    return QSE_OK;
+}
+
+extern "C" int qse_ctx_get_fusion(qse_ctx* ctx, int precision, int* tile_bits, int* low_bits) {
+    if (!ctx || precision < 0 || precision > 1) return set_error(QSE_ERR_INVALID_ARG, "bad argument");
+    int k, L;
+    fusion_shape(ctx, 34, precision, &k, &L);
+    if (tile_bits) *tile_bits = k;
+    if (low_bits) *low_bits = L;
+    return QSE_OK;
+}

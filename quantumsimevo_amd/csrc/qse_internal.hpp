@@ -1,0 +1,117 @@
+// qse_internal.hpp — host-side internals of libqse_hip.so: context/state
+// structs, kernel launchers and error plumbing. Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/qse_hip.h"
+
+namespace qse {
+
+// Kernel families for the timing report (qse_ctx_timing).
+enum Family { FAM_SWEEP = 0, FAM_FUSED = 1, FAM_READOUT = 2, FAM_ADJOINT = 3, FAM_COUNT = 4 };
+
+struct TimedLaunch {
+    int family;
+    hipEvent_t a, b;
+    double bytes;
+};
+
+struct Scratch {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+// A fused-pass program resident on the device, keyed by its serialized bytes.
+struct ProgEntry {
+    std::vector<char> bytes;
+    void* dev = nullptr;
+    uint64_t stamp = 0;
+};
+
+}  // namespace qse
+
+struct qse_ctx {
+    int device = 0;
+    int num_cus = 256;
+    size_t lds_per_block = 65536;
+    hipStream_t stream = nullptr;
+    // Device scratch, grown on demand (partials, flags, cumulative arrays).
+    qse::Scratch scratch[8];
+    // Device copies of recently run fused programs.
+    std::vector<qse::ProgEntry> prog_cache;
+    uint64_t prog_clock = 0;
+    // Pinned host staging for small readbacks.
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+    // Fusion shape per precision (tile qubits k, contiguous low qubits L).
+    int tile_bits[2] = {0, 0};
+    int low_bits[2] = {0, 0};
+    // Timing.
+    bool timing = false;
+    std::vector<qse::TimedLaunch> pending;
+    std::vector<hipEvent_t> event_pool;
+    int64_t launches[qse::FAM_COUNT] = {0, 0, 0, 0};
+    double total_ms[qse::FAM_COUNT] = {0, 0, 0, 0};
+    double total_bytes[qse::FAM_COUNT] = {0, 0, 0, 0};
+};
+
+struct qse_state {
+    int n = 0;
+    int prec = QSE_C128;
+    void* d = nullptr;
+    // Second buffer of the same size, allocated on first need: a fused pass
+    // that permutes qubits writes here, then the two pointers are exchanged.
+    void* shadow = nullptr;
+    size_t bytes = 0;
+    bool owned = true;
+};
+
+namespace qse {
+
+// Error plumbing: set_error records a message for qse_last_error().
+int set_error(int status, const std::string& msg);
+int hip_error(hipError_t e, const char* what);
+#define QSE_HIP(call)                                            \
+    do {                                                         \
+        hipError_t _e = (call);                                  \
+        if (_e != hipSuccess) return ::qse::hip_error(_e, #call); \
+    } while (0)
+
+// Scratch buffer `slot` of at least `bytes` bytes (grows; stream-ordered free).
+int scratch(qse_ctx* ctx, int slot, size_t bytes, void** out);
+// Pinned host buffer of at least `bytes` bytes.
+int pinned(qse_ctx* ctx, size_t bytes, void** out);
+
+// Timing hooks around a launch; `bytes` = algorithmic bytes of that launch.
+void launch_begin(qse_ctx* ctx, int family, TimedLaunch* t);
+void launch_end(qse_ctx* ctx, TimedLaunch* t, double bytes);
+int fold_timing(qse_ctx* ctx);
+
+// Grid for a grid-stride sweep over `work` items with `block` threads.
+unsigned grid_for(const qse_ctx* ctx, uint64_t work, int block, int waves_per_cu = 32);
+
+inline size_t amp_bytes(int prec) { return prec == QSE_C128 ? 16 : 8; }
+
+// ---- per-gate sweeps (qse_gates.hip) ----
+// Pair kernel: for k < nk, base = deposit(k); (i0, i1) = (base|off0, base|off1).
+enum PairMode {
+    PAIR_DENSE = 0,  // (v0, v1) <- M (v0, v1)
+    PAIR_SWAP = 1,   // exchange v0, v1
+    PAIR_DIAG = 2,   // v0 *= d0, v1 *= d1
+    PAIR_DIAG1 = 3,  // v1 *= d1 only (off0 unused)
+    PAIR_HADAMARD = 4 // s*(v0+v1), s*(v0-v1) — the reference's rounding
+};
+int launch_pairs(qse_ctx* ctx, qse_state* st, int mode, const uint64_t* fixed0, int nfixed0, uint64_t ones,
+                 uint64_t off0, uint64_t off1, const double m[8], double algo_bytes);
+int launch_frame(qse_ctx* ctx, qse_state* st, uint64_t xmask, uint64_t zmask);
+int launch_set_basis(qse_ctx* ctx, qse_state* st, uint64_t index);
+int launch_convert_upload(qse_ctx* ctx, qse_state* st, const double* dev_c128, uint64_t count, uint64_t first);
+int launch_convert_download(qse_ctx* ctx, const qse_state* st, double* dev_c128, uint64_t count, uint64_t first);
+int launch_scale(qse_ctx* ctx, qse_state* st, double s);
+int launch_random(qse_ctx* ctx, qse_state* st, uint64_t seed);
+
+}  // namespace qse

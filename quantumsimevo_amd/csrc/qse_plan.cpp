@@ -1,0 +1,431 @@
+// qse_plan.cpp — host fusion planner: turns a gate list (qse_op, the C form of
+// the reference's GateCommand list replayed by Circuit::execute,
+// Circuit.cpp:64-120) into LDS-tiled passes for the gfx950 fused kernel.
+//
+// Phase 1 (passes): scan the gates in order, growing the current pass's set of
+// tile qubits T (|T| <= k, always holding the low qubits 0..L-1). A gate that
+// is non-diagonal on a qubit outside T, when T is full, closes the pass.
+// Diagonal gates (Z, phase, RZ, cphase, mcphase) never close a pass. A SWAP of
+// two tile qubits is a relabelling of tile bits (no data movement until the
+// pass's store, which writes each tile bit to its final qubit).
+// Phase 2 (stages): inside a pass, consecutive non-diagonal gates share a
+// stage while their targets fit in 4 register bits. Diagonal gates are kept
+// pending as monomials (d applied where all bits of a mask are 1) and flushed
+// right before the next non-diagonal gate on one of their qubits, or at the
+// end of the pass; flushed monomials are merged by register mask and by
+// thread/external condition, and conditions on external qubits are folded into
+// per-tile factors computed once per tile.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "qse_fused.hpp"
+#include "qse_internal.hpp"
+
+namespace qse {
+
+double op_angle(const qse_op& op, const double* params);
+
+using cd = std::complex<double>;
+
+namespace {
+
+enum ItemType { IT_DENSE, IT_MONO, IT_SWAP };
+
+struct Item {
+    ItemType t;
+    int q = -1, q2 = -1;   // dense target / swap qubits
+    uint8_t dtype = FOP_DENSE;
+    uint64_t cmask = 0;    // dense: control qubits
+    cd m[4];
+    uint64_t qmask = 0;    // mono: qubits that must be 1
+    cd d;
+};
+
+struct Mono {
+    uint64_t lmask;  // tile local bits
+    uint64_t emask;  // external qubits
+    cd d;
+};
+
+struct SOp {
+    bool mono;
+    // dense
+    uint8_t dtype;
+    int lt;
+    uint64_t lctl, ectl;
+    cd m[4];
+    // mono
+    std::vector<Mono> monos;
+};
+
+struct SStage {
+    std::vector<int> R;  // register local bits, position order
+    std::vector<SOp> ops;
+};
+
+void to_items(const qse_op* ops, int nops, const double* params, std::vector<Item>& out) {
+    const double s = 0.7071067811865475244;
+    for (int k = 0; k < nops; ++k) {
+        const qse_op& o = ops[k];
+        const double th = op_angle(o, params);
+        Item it;
+        auto dense = [&](uint8_t dt, cd a, cd b, cd c, cd d) {
+            it.t = IT_DENSE;
+            it.q = o.target;
+            it.dtype = dt;
+            it.m[0] = a; it.m[1] = b; it.m[2] = c; it.m[3] = d;
+            out.push_back(it);
+        };
+        auto mono = [&](uint64_t mask, cd d) {
+            Item mi;
+            mi.t = IT_MONO;
+            mi.qmask = mask;
+            mi.d = d;
+            out.push_back(mi);
+        };
+        const uint64_t tb = 1ULL << o.target;
+        switch (o.kind) {
+            case QSE_GATE_IDENTITY: break;
+            case QSE_GATE_H: dense(FOP_H, s, s, s, -s); break;
+            case QSE_GATE_X: dense(FOP_XSWAP, 0, 1, 1, 0); break;
+            case QSE_GATE_Y: dense(FOP_DENSE, 0, cd(0, -1), cd(0, 1), 0); break;
+            case QSE_GATE_RX: {
+                const double c = std::cos(th / 2), sn = std::sin(th / 2);
+                dense(FOP_DENSE, c, cd(0, -sn), cd(0, -sn), c);
+                break;
+            }
+            case QSE_GATE_RY: {
+                const double c = std::cos(th / 2), sn = std::sin(th / 2);
+                dense(FOP_DENSE, c, -sn, sn, c);
+                break;
+            }
+            case QSE_GATE_MATRIX:
+                dense(FOP_DENSE, cd(o.m[0], o.m[1]), cd(o.m[2], o.m[3]), cd(o.m[4], o.m[5]), cd(o.m[6], o.m[7]));
+                break;
+            case QSE_GATE_CNOT:
+                if (o.control == o.target) break;  // no-op in the reference loop
+                it.cmask = 1ULL << o.control;
+                dense(FOP_XSWAP, 0, 1, 1, 0);
+                break;
+            case QSE_GATE_Z: mono(tb, -1.0); break;
+            case QSE_GATE_PHASE: mono(tb, cd(std::cos(th), std::sin(th))); break;
+            case QSE_GATE_RZ:
+                mono(0, cd(std::cos(th / 2), -std::sin(th / 2)));
+                mono(tb, cd(std::cos(th), std::sin(th)));
+                break;
+            case QSE_GATE_CPHASE: mono(tb | (1ULL << o.control), cd(std::cos(th), std::sin(th))); break;
+            case QSE_GATE_MCPHASE: mono(tb | o.controls, cd(std::cos(th), std::sin(th))); break;
+            case QSE_GATE_SWAP:
+                if (o.target == o.control) break;
+                it.t = IT_SWAP;
+                it.q = o.target;
+                it.q2 = o.control;
+                out.push_back(it);
+                break;
+            default: break;  // validated earlier
+        }
+    }
+}
+
+struct PassItems {
+    uint64_t T = 0;
+    std::vector<Item> items;
+};
+
+int popc(uint64_t x) { return __builtin_popcountll(x); }
+
+}  // namespace
+
+
+// Choose (k, L) for an n-qubit state of precision prec given the context
+// settings (0 = default).
+void fusion_shape(const qse_ctx* ctx, int n, int prec, int* k_out, int* L_out) {
+    int k = ctx ? ctx->tile_bits[prec] : 0;
+    int L = ctx ? ctx->low_bits[prec] : 0;
+    if (k <= 0) k = (prec == QSE_C128) ? 12 : 13;
+    if (L <= 0) L = (prec == QSE_C128) ? 4 : 5;
+    k = std::min(k, FUSE_MAX_K);
+    k = std::min(k, n - 2);
+    k = std::max(k, FUSE_MIN_K);
+    L = std::min(L, k - FUSE_REG);
+    L = std::max(L, 0);
+    *k_out = k;
+    *L_out = L;
+}
+
+// relabel: SWAPs become a logical->physical qubit map (no data movement); the
+// last pass then stores out of place with the inverse map so the state ends in
+// the canonical layout. Otherwise SWAPs are in-tile relabellings (in place).
+int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P, bool relabel) {
+    std::vector<Item> items;
+    to_items(ops, nops, params, items);
+    const uint64_t lowT = (L > 0) ? ((1ULL << L) - 1) : 0;
+    int phys[64];
+    for (int q = 0; q < 64; ++q) phys[q] = q;
+    auto map_mask = [&](uint64_t m) {
+        uint64_t r = 0;
+        for (int q = 0; q < n; ++q)
+            if ((m >> q) & 1) r |= 1ULL << phys[q];
+        return r;
+    };
+    // ---- phase 1: passes ----
+    std::vector<PassItems> passes;
+    passes.push_back({lowT, {}});
+    for (Item it : items) {
+        if (relabel) {
+            if (it.t == IT_SWAP) {
+                std::swap(phys[it.q], phys[it.q2]);
+                continue;
+            }
+            if (it.t == IT_DENSE) {
+                it.q = phys[it.q];
+                it.cmask = map_mask(it.cmask);
+            } else {
+                it.qmask = map_mask(it.qmask);
+            }
+        }
+        PassItems* cur = &passes.back();
+        if (it.t == IT_DENSE || it.t == IT_SWAP) {
+            uint64_t need = 1ULL << it.q;
+            if (it.t == IT_SWAP) need |= 1ULL << it.q2;
+            const uint64_t miss = need & ~cur->T;
+            if (popc(cur->T | miss) > k) {
+                passes.push_back({lowT | need, {}});
+                cur = &passes.back();
+            } else {
+                cur->T |= miss;
+            }
+        }
+        cur->items.push_back(it);
+    }
+    bool permuted = false;
+    for (int q = 0; q < n; ++q) permuted |= (phys[q] != q);
+    if (relabel && permuted) {
+        // the destination's low qubits 0..L-1 come from phys[0..L-1]: they must be
+        // tile qubits of the last pass for coalesced writes
+        uint64_t need = 0;
+        for (int q = 0; q < L; ++q) need |= 1ULL << phys[q];
+        PassItems* cur = &passes.back();
+        if (popc(cur->T | need) > k) passes.push_back({lowT | need, {}});
+        else cur->T |= need;
+    }
+    if (passes.size() > 1 && passes.back().items.empty() && !(relabel && permuted)) passes.pop_back();
+    // ---- phase 2: per pass, fill T to k qubits and build stages ----
+    P.k = k;
+    for (PassItems& pi : passes) {
+        for (int q = 0; q < n && popc(pi.T) < k; ++q) pi.T |= 1ULL << q;
+        FPass fp{};
+        fp.k = k;
+        fp.next = n - k;
+        std::vector<int> tq, epos;
+        for (int q = 0; q < n; ++q) ((pi.T >> q) & 1 ? tq : epos).push_back(q);
+        std::vector<int> loc(64, -1);
+        for (int b = 0; b < k; ++b) {
+            fp.tq[b] = (uint8_t)tq[b];
+            loc[tq[b]] = b;
+        }
+        for (size_t e = 0; e < epos.size(); ++e) fp.epos[e] = (uint8_t)epos[e];
+
+        std::vector<SStage> stages(1);
+        std::vector<Mono> pending;
+        auto flush = [&](SStage& st, int lt_or_all) {
+            std::vector<Mono> take, keep;
+            for (auto& m : pending) {
+                if (lt_or_all < 0 || ((m.lmask >> lt_or_all) & 1)) take.push_back(m);
+                else keep.push_back(m);
+            }
+            pending.swap(keep);
+            if (take.empty()) return;
+            SOp op;
+            op.mono = true;
+            op.monos = std::move(take);
+            st.ops.push_back(std::move(op));
+        };
+        for (const Item& it : pi.items) {
+            if (it.t == IT_MONO) {
+                uint64_t lm = 0;
+                for (int q = 0; q < n; ++q)
+                    if (((it.qmask >> q) & 1) && loc[q] >= 0) lm |= 1ULL << loc[q];
+                pending.push_back({lm, it.qmask & ~pi.T, it.d});
+            } else if (it.t == IT_SWAP) {
+                std::swap(loc[it.q], loc[it.q2]);
+            } else {
+                const int lt = loc[it.q];
+                SStage* st = &stages.back();
+                if (std::find(st->R.begin(), st->R.end(), lt) == st->R.end()) {
+                    if ((int)st->R.size() == FUSE_REG) {
+                        stages.emplace_back();
+                        st = &stages.back();
+                    }
+                    st->R.push_back(lt);
+                }
+                flush(*st, lt);
+                SOp op;
+                op.mono = false;
+                op.dtype = it.dtype;
+                op.lt = lt;
+                op.lctl = 0;
+                op.ectl = 0;
+                for (int q = 0; q < n; ++q)
+                    if ((it.cmask >> q) & 1) {
+                        if (loc[q] >= 0) op.lctl |= 1ULL << loc[q];
+                        else op.ectl |= 1ULL << q;
+                    }
+                for (int j = 0; j < 4; ++j) op.m[j] = it.m[j];
+                st->ops.push_back(op);
+            }
+        }
+        flush(stages.back(), -1);
+        // store maps: destination qubit of each source qubit
+        int dest[64];
+        for (int q = 0; q < n; ++q) dest[q] = q;
+        const bool oop = relabel && permuted && (&pi == &passes.back());
+        if (oop)
+            for (int q = 0; q < n; ++q) dest[phys[q]] = q;
+        fp.out_of_place = oop ? 1 : 0;
+        {
+            // store position c: tile qubits ordered by destination qubit
+            std::vector<std::pair<int, int>> byd;  // (dest qubit, local bit)
+            for (int b = 0; b < k; ++b) byd.push_back({dest[tq[b]], loc[tq[b]]});
+            std::sort(byd.begin(), byd.end());
+            for (int c = 0; c < k; ++c) {
+                fp.so_q[c] = (uint8_t)byd[c].first;
+                fp.so_l[c] = (uint8_t)byd[c].second;
+            }
+            for (size_t e = 0; e < epos.size(); ++e) fp.eout[e] = (uint8_t)dest[epos[e]];
+        }
+
+        // ---- emit stages ----
+        fp.stage_begin = (uint32_t)P.stages.size();
+        fp.slot_begin = (uint32_t)P.slots.size();
+        int nslots = 0;
+        for (SStage& st : stages) {
+            // complete the register set with unused local bits
+            for (int l = 0; l < k && (int)st.R.size() < FUSE_REG; ++l)
+                if (std::find(st.R.begin(), st.R.end(), l) == st.R.end()) st.R.push_back(l);
+            int pos[64];
+            for (int l = 0; l < 64; ++l) pos[l] = -1;
+            uint64_t Rmask = 0;
+            for (int p = 0; p < FUSE_REG; ++p) {
+                pos[st.R[p]] = p;
+                Rmask |= 1ULL << st.R[p];
+            }
+            // thread bits: first up to 4 with distinct (bit mod 4) for conflict-free LDS banks
+            std::vector<int> rest, tm;
+            for (int l = 0; l < k; ++l)
+                if (pos[l] < 0) rest.push_back(l);
+            int used_res = 0;
+            for (int l : rest)
+                if ((int)tm.size() < 4 && !((used_res >> (l & 3)) & 1)) {
+                    tm.push_back(l);
+                    used_res |= 1 << (l & 3);
+                }
+            for (int l : rest)
+                if (std::find(tm.begin(), tm.end(), l) == tm.end()) tm.push_back(l);
+            FStage fs{};
+            for (int p = 0; p < FUSE_REG; ++p) fs.rmap[p] = (uint8_t)st.R[p];
+            for (size_t t = 0; t < tm.size(); ++t) fs.tmap[t] = (uint8_t)tm[t];
+            fs.op_begin = (uint32_t)P.ops.size();
+            for (SOp& so : st.ops) {
+                if (!so.mono) {
+                    FOp f{};
+                    f.type = so.dtype;
+                    f.p = (uint8_t)pos[so.lt];
+                    uint32_t rctl = 0;
+                    for (int l = 0; l < k; ++l)
+                        if (((so.lctl >> l) & 1) && pos[l] >= 0) rctl |= 1u << pos[l];
+                    f.rctl = (uint8_t)rctl;
+                    f.tctl = (uint32_t)(so.lctl & ~Rmask);
+                    f.ectl = so.ectl;
+                    for (int j = 0; j < 4; ++j) {
+                        f.m[2 * j] = so.m[j].real();
+                        f.m[2 * j + 1] = so.m[j].imag();
+                    }
+                    P.ops.push_back(f);
+                    continue;
+                }
+                // group monomials by register mask
+                std::map<uint32_t, std::vector<const Mono*>> byR;
+                for (const Mono& m : so.monos) {
+                    uint32_t mr = 0;
+                    for (int l = 0; l < k; ++l)
+                        if (((m.lmask >> l) & 1) && pos[l] >= 0) mr |= 1u << pos[l];
+                    byR[mr].push_back(&m);
+                }
+                for (auto& kv : byR) {
+                    FOp f{};
+                    f.type = FOP_MONO;
+                    f.rctl = (uint8_t)kv.first;
+                    f.term_begin = (uint32_t)P.terms.size();
+                    std::map<uint32_t, cd> cst;                         // tmask -> product
+                    std::map<uint32_t, std::vector<const Mono*>> ext;  // tmask -> external monomials
+                    for (const Mono* m : kv.second) {
+                        const uint32_t tmask = (uint32_t)(m->lmask & ~Rmask);
+                        if (m->emask == 0) {
+                            auto itc = cst.find(tmask);
+                            if (itc == cst.end()) cst[tmask] = m->d;
+                            else itc->second *= m->d;
+                        } else {
+                            ext[tmask].push_back(m);
+                        }
+                    }
+                    for (auto& c : cst) {
+                        FTerm t{};
+                        t.tmask = c.first;
+                        t.slot = -1;
+                        t.emask = 0;
+                        t.d[0] = c.second.real();
+                        t.d[1] = c.second.imag();
+                        P.terms.push_back(t);
+                    }
+                    for (auto& e : ext) {
+                        if (nslots < FUSE_MAX_SLOTS) {
+                            FSlot sl;
+                            sl.begin = (uint32_t)P.eterms.size();
+                            for (const Mono* m : e.second) {
+                                FExtTerm et;
+                                et.emask = m->emask;
+                                et.d[0] = m->d.real();
+                                et.d[1] = m->d.imag();
+                                P.eterms.push_back(et);
+                            }
+                            sl.end = (uint32_t)P.eterms.size();
+                            P.slots.push_back(sl);
+                            FTerm t{};
+                            t.tmask = e.first;
+                            t.slot = nslots++;
+                            t.emask = 0;
+                            t.d[0] = 1.0;
+                            t.d[1] = 0.0;
+                            P.terms.push_back(t);
+                        } else {
+                            for (const Mono* m : e.second) {
+                                FTerm t{};
+                                t.tmask = e.first;
+                                t.slot = -1;
+                                t.emask = m->emask;
+                                t.d[0] = m->d.real();
+                                t.d[1] = m->d.imag();
+                                P.terms.push_back(t);
+                            }
+                        }
+                    }
+                    f.term_end = (uint32_t)P.terms.size();
+                    P.ops.push_back(f);
+                }
+            }
+            fs.op_end = (uint32_t)P.ops.size();
+            P.stages.push_back(fs);
+        }
+        fp.nstages = (int)stages.size();
+        fp.nslots = nslots;
+        P.passes.push_back(fp);
+    }
+    return QSE_OK;
+}
+
+}  // namespace qse

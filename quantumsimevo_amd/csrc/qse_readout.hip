@@ -1,0 +1,429 @@
+// qse_readout.hip — device readout for gfx950: norms, sampling, measurement,
+// single-qubit collapse and diagonal (Z-string) expectation values.
+//
+// Reference semantics (StateVector.cpp:77-156): the cumulative array of |a_i|^2
+// is summed sequentially, total = cum[N-1], and a shot with canonical uniform u
+// returns lower_bound(cum, u*total). A parallel device CDF rounds differently
+// from the sequential one, so sampling here is two-stage:
+//   1. a blocked FP64 CDF (block sums of B amplitudes, a scan over blocks, then
+//      a sequential walk inside the chosen block) resolves every shot and
+//      checks it against a proven rounding band: the outcome is certain when
+//      the walk's values at j-1 and j sit outside the band around the target,
+//      taking into account that both the device and the sequential sums carry
+//      at most (index + depth) unit roundoffs of relative error;
+//   2. shots inside the band are re-resolved by one workgroup that reproduces
+//      the sequential sum exactly (same order, same rounding) over the whole
+//      state: first the sequential total, then one walk that hands out the
+//      flagged (sorted) targets as the running sum crosses them.
+// Outcomes therefore equal the sequential reference on identical amplitudes.
+// Pauli frames (noise trajectories) are sampled the same way over the framed
+// order psi[i ^ x] without materialising the framed state.
+#include "qse_device.hpp"
+#include "qse_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace qse {
+
+static constexpr int RB_THREADS = 256;
+static constexpr double UR = 2.220446049250313e-16;  // 2^-52: twice the unit roundoff
+
+// Block sums of framed probabilities: block b of trajectory t covers framed
+// indices [b*B, (b+1)*B), i.e. amplitudes psi[i ^ x_t].
+template <class R>
+__global__ __launch_bounds__(RB_THREADS) void k_block_sums(const cx<R>* __restrict__ s, uint64_t B, uint64_t nb,
+                                                           const uint64_t* __restrict__ xmasks,
+                                                           double* __restrict__ bsum) {
+    const uint64_t b = blockIdx.x;
+    const int t = blockIdx.y;
+    const uint64_t x = xmasks ? xmasks[t] : 0;
+    double acc = 0.0;
+    for (uint64_t i = threadIdx.x; i < B; i += RB_THREADS) acc += prob(s[(b * B + i) ^ x]);
+    acc = wave_sum(acc);
+    __shared__ double part[RB_THREADS / 64];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = 0.0;
+        for (int w = 0; w < RB_THREADS / 64; ++w) v += part[w];
+        bsum[(uint64_t)t * nb + b] = v;
+    }
+}
+
+// Inclusive-start prefix: pre[t][0] = 0, pre[t][b+1] = pre[t][b] + bsum[t][b].
+// One workgroup of 1024 threads per trajectory; thread chunks summed
+// sequentially, chunk totals scanned in LDS.
+__global__ __launch_bounds__(1024) void k_scan_blocks(const double* __restrict__ bsum, uint64_t nb,
+                                                      double* __restrict__ pre) {
+    const int t = blockIdx.x;
+    const double* in = bsum + (uint64_t)t * nb;
+    double* out = pre + (uint64_t)t * (nb + 1);
+    const uint64_t per = (nb + 1023) / 1024;
+    const uint64_t lo = std::min<uint64_t>(nb, threadIdx.x * per), hi = std::min<uint64_t>(nb, lo + per);
+    double tot = 0.0;
+    for (uint64_t i = lo; i < hi; ++i) tot += in[i];
+    __shared__ double sh[1024];
+    sh[threadIdx.x] = tot;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const double v = (threadIdx.x >= (unsigned)o) ? sh[threadIdx.x - o] : 0.0;
+        __syncthreads();
+        sh[threadIdx.x] += v;
+        __syncthreads();
+    }
+    // exclusive start of this chunk
+    double run = (threadIdx.x == 0) ? 0.0 : sh[threadIdx.x - 1];
+    if (threadIdx.x == 0) out[0] = 0.0;
+    for (uint64_t i = lo; i < hi; ++i) {
+        run += in[i];
+        out[i + 1] = run;
+    }
+}
+
+// One thread per shot: coarse binary search over block prefixes, then a
+// sequential walk in the block, then the rounding-band certainty check.
+template <class R>
+__global__ __launch_bounds__(RB_THREADS) void k_shots(const cx<R>* __restrict__ s, uint64_t N, uint64_t B, uint64_t nb,
+                                                      const uint64_t* __restrict__ xmasks,
+                                                      const double* __restrict__ pre, const double* __restrict__ u,
+                                                      int64_t nshots, int ntraj, uint64_t* __restrict__ out,
+                                                      uint8_t* __restrict__ flag) {
+    const uint64_t total_shots = (uint64_t)nshots * ntraj;
+    for (uint64_t g = gtid(); g < total_shots; g += gstride()) {
+        const int t = (int)(g / (uint64_t)nshots);
+        const uint64_t x = xmasks ? xmasks[t] : 0;
+        const double* P = pre + (uint64_t)t * (nb + 1);
+        const double T = P[nb];
+        const double target = u[g] * T;
+        // largest b with P[b] < target (P[0] = 0), clamped to [0, nb-1]
+        uint64_t lo = 0, hi = nb;  // invariant: P[lo] < target or lo == 0
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (P[mid] < target) lo = mid;
+            else hi = mid;
+        }
+        const uint64_t b = lo;
+        double prev = P[b], cur = prev;
+        uint64_t j = b * B;
+        bool found = false;
+        for (uint64_t i = 0; i < B; ++i) {
+            cur = prev + prob(s[(b * B + i) ^ x]);
+            if (cur >= target) {
+                j = b * B + i;
+                found = true;
+                break;
+            }
+            prev = cur;
+        }
+        const double depth = (double)(B + nb + 64);
+        const double delta = ((double)N + depth) * UR;
+        const double t_lo = target * (1.0 - delta - 4.0 * UR);
+        const double t_hi = target * (1.0 + delta + 4.0 * UR);
+        bool certain = found;
+        if (certain) {
+            const double e_j = ((double)j + depth) * UR * cur;
+            certain = (cur - e_j >= t_hi);
+            if (certain && j > 0) {
+                const double e_p = ((double)j + depth) * UR * prev;
+                certain = (prev + e_p < t_lo);
+            }
+        }
+        out[g] = found ? j : (b * B + B - 1);
+        flag[g] = certain ? 0 : 1;
+    }
+}
+
+// Exact sequential resolution for flagged shots. One workgroup per listed
+// trajectory: pass 1 reproduces the sequential total, pass 2 walks the
+// sequential running sum and hands out the (ascending) targets.
+//   traj[w]: trajectory of work item w; [first[w], first[w+1]) its flagged
+//   shots in `ulist` (ascending u) with output slots `slot`.
+template <class R>
+__global__ __launch_bounds__(RB_THREADS) void k_exact_resolve(const cx<R>* __restrict__ s, uint64_t N,
+                                                              const uint64_t* __restrict__ xmasks,
+                                                              const int* __restrict__ traj,
+                                                              const int64_t* __restrict__ first,
+                                                              const double* __restrict__ ulist,
+                                                              const int64_t* __restrict__ slot,
+                                                              uint64_t* __restrict__ out) {
+    constexpr int CH = 4096;
+    __shared__ double p[CH];
+    __shared__ double total_sh;
+    __shared__ int64_t ksh;
+    const int w = blockIdx.x;
+    const uint64_t x = xmasks ? xmasks[traj[w]] : 0;
+    const int64_t f0 = first[w], f1 = first[w + 1];
+    // pass 1: sequential total
+    double run = 0.0;
+    for (uint64_t base = 0; base < N; base += CH) {
+        const uint64_t len = std::min<uint64_t>(CH, N - base);
+        __syncthreads();
+        for (uint64_t i = threadIdx.x; i < len; i += RB_THREADS) p[i] = prob(s[(base + i) ^ x]);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (uint64_t i = 0; i < len; ++i) run += p[i];
+    }
+    if (threadIdx.x == 0) total_sh = run;
+    __syncthreads();
+    const double T = total_sh;
+    // pass 2: walk and hand out targets in ascending order
+    int64_t k = f0;
+    double c = 0.0;
+    for (uint64_t base = 0; base < N && k < f1; base += CH) {
+        const uint64_t len = std::min<uint64_t>(CH, N - base);
+        __syncthreads();
+        for (uint64_t i = threadIdx.x; i < len; i += RB_THREADS) p[i] = prob(s[(base + i) ^ x]);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (uint64_t i = 0; i < len && k < f1; ++i) {
+                c += p[i];
+                while (k < f1 && c >= ulist[k] * T) {
+                    out[slot[k]] = base + i;
+                    ++k;
+                }
+            }
+        }
+        // broadcast k so every thread leaves the loop together
+        if (threadIdx.x == 0) ksh = k;
+        __syncthreads();
+        k = ksh;
+    }
+    if (threadIdx.x == 0)
+        for (; k < f1; ++k) out[slot[k]] = N - 1;  // unreachable: u*T <= T
+}
+
+template <class R>
+static int sample_frames_t(qse_ctx* ctx, qse_state* st, const uint64_t* xmask_h, int ntraj, const double* u_h,
+                           int64_t nshots, uint64_t* out_h) {
+    const uint64_t N = 1ULL << st->n;
+    const uint64_t B = std::min<uint64_t>(1024, N);
+    const uint64_t nb = N / B;
+    const uint64_t total = (uint64_t)nshots * ntraj;
+    // scratch layout (slot 0): xmasks | bsum | pre | u | out | flags
+    const size_t sz_x = sizeof(uint64_t) * (size_t)ntraj;
+    const size_t sz_b = sizeof(double) * (size_t)(nb * ntraj);
+    const size_t sz_p = sizeof(double) * (size_t)((nb + 1) * ntraj);
+    const size_t sz_u = sizeof(double) * (size_t)total;
+    const size_t sz_o = sizeof(uint64_t) * (size_t)total;
+    const size_t sz_f = (size_t)total;
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t need = al(sz_x) + al(sz_b) + al(sz_p) + al(sz_u) + al(sz_o) + al(sz_f);
+    void* base;
+    int rc = scratch(ctx, 0, need, &base);
+    if (rc) return rc;
+    char* p = (char*)base;
+    uint64_t* d_x = (uint64_t*)p; p += al(sz_x);
+    double* d_b = (double*)p; p += al(sz_b);
+    double* d_p = (double*)p; p += al(sz_p);
+    double* d_u = (double*)p; p += al(sz_u);
+    uint64_t* d_o = (uint64_t*)p; p += al(sz_o);
+    uint8_t* d_f = (uint8_t*)p;
+    const bool framed = (xmask_h != nullptr);
+    if (framed) QSE_HIP(hipMemcpyAsync(d_x, xmask_h, sz_x, hipMemcpyHostToDevice, ctx->stream));
+    QSE_HIP(hipMemcpyAsync(d_u, u_h, sz_u, hipMemcpyHostToDevice, ctx->stream));
+    const cx<R>* s = (const cx<R>*)st->d;
+    TimedLaunch tl;
+    launch_begin(ctx, FAM_READOUT, &tl);
+    k_block_sums<R><<<dim3((unsigned)nb, (unsigned)ntraj), RB_THREADS, 0, ctx->stream>>>(s, B, nb, framed ? d_x : nullptr,
+                                                                                          d_b);
+    QSE_HIP(hipGetLastError());
+    launch_end(ctx, &tl, (double)amp_bytes(st->prec) * N * ntraj);
+    k_scan_blocks<<<ntraj, 1024, 0, ctx->stream>>>(d_b, nb, d_p);
+    QSE_HIP(hipGetLastError());
+    if (total > 0) {
+        k_shots<R><<<grid_for(ctx, total, RB_THREADS, 8), RB_THREADS, 0, ctx->stream>>>(
+            s, N, B, nb, framed ? d_x : nullptr, d_p, d_u, nshots, ntraj, d_o, d_f);
+        QSE_HIP(hipGetLastError());
+    }
+    std::vector<uint8_t> flags(total);
+    if (total > 0) {
+        QSE_HIP(hipMemcpyAsync(out_h, d_o, sz_o, hipMemcpyDeviceToHost, ctx->stream));
+        QSE_HIP(hipMemcpyAsync(flags.data(), d_f, sz_f, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    QSE_HIP(hipStreamSynchronize(ctx->stream));
+    // Group flagged shots by trajectory, ascending u.
+    std::vector<std::vector<int64_t>> per(ntraj);
+    int64_t nflag = 0;
+    for (uint64_t g = 0; g < total; ++g)
+        if (flags[g]) {
+            per[g / (uint64_t)nshots].push_back((int64_t)g);
+            ++nflag;
+        }
+    if (nflag == 0) return QSE_OK;
+    std::vector<int> traj;
+    std::vector<int64_t> first{0}, slot;
+    std::vector<double> ulist;
+    for (int t = 0; t < ntraj; ++t) {
+        if (per[t].empty()) continue;
+        std::stable_sort(per[t].begin(), per[t].end(), [&](int64_t a, int64_t b) { return u_h[a] < u_h[b]; });
+        traj.push_back(t);
+        for (int64_t g : per[t]) {
+            slot.push_back(g);
+            ulist.push_back(u_h[g]);
+        }
+        first.push_back((int64_t)slot.size());
+    }
+    const int nw = (int)traj.size();
+    const size_t need2 = al(sizeof(int) * nw) + al(sizeof(int64_t) * (nw + 1)) + al(sizeof(double) * nflag) +
+                         al(sizeof(int64_t) * nflag) + al(sizeof(uint64_t) * nflag);
+    void* base2;
+    rc = scratch(ctx, 1, need2, &base2);
+    if (rc) return rc;
+    char* q = (char*)base2;
+    int* d_traj = (int*)q; q += al(sizeof(int) * nw);
+    int64_t* d_first = (int64_t*)q; q += al(sizeof(int64_t) * (nw + 1));
+    double* d_ul = (double*)q; q += al(sizeof(double) * nflag);
+    int64_t* d_slot = (int64_t*)q; q += al(sizeof(int64_t) * nflag);
+    uint64_t* d_out2 = (uint64_t*)q;
+    // local slots: resolved values are written to d_out2[local index]
+    std::vector<int64_t> local(nflag);
+    for (int64_t i = 0; i < nflag; ++i) local[i] = i;
+    QSE_HIP(hipMemcpyAsync(d_traj, traj.data(), sizeof(int) * nw, hipMemcpyHostToDevice, ctx->stream));
+    QSE_HIP(hipMemcpyAsync(d_first, first.data(), sizeof(int64_t) * (nw + 1), hipMemcpyHostToDevice, ctx->stream));
+    QSE_HIP(hipMemcpyAsync(d_ul, ulist.data(), sizeof(double) * nflag, hipMemcpyHostToDevice, ctx->stream));
+    QSE_HIP(hipMemcpyAsync(d_slot, local.data(), sizeof(int64_t) * nflag, hipMemcpyHostToDevice, ctx->stream));
+    TimedLaunch t2;
+    launch_begin(ctx, FAM_READOUT, &t2);
+    k_exact_resolve<R><<<nw, RB_THREADS, 0, ctx->stream>>>(s, N, framed ? d_x : nullptr, d_traj, d_first, d_ul, d_slot,
+                                                           d_out2);
+    QSE_HIP(hipGetLastError());
+    launch_end(ctx, &t2, 2.0 * amp_bytes(st->prec) * N * nw);
+    std::vector<uint64_t> res(nflag);
+    QSE_HIP(hipMemcpyAsync(res.data(), d_out2, sizeof(uint64_t) * nflag, hipMemcpyDeviceToHost, ctx->stream));
+    QSE_HIP(hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < nflag; ++i) out_h[slot[i]] = res[i];
+    return QSE_OK;
+}
+
+int sample_frames(qse_ctx* ctx, qse_state* st, const uint64_t* xmask, int ntraj, const double* u, int64_t nshots,
+                  uint64_t* out) {
+    if (st->prec == QSE_C128) return sample_frames_t<double>(ctx, st, xmask, ntraj, u, nshots, out);
+    return sample_frames_t<float>(ctx, st, xmask, ntraj, u, nshots, out);
+}
+
+// ---- reductions: sum over i of |a_i|^2 * f(i) ----
+// f(i) = 1 (norm), [bit q set] (prob of qubit q), or a weighted Z-string sum.
+enum RedMode { RED_NORM = 0, RED_BIT = 1, RED_ZSTR = 2 };
+
+template <class R, int MODE>
+__global__ __launch_bounds__(RB_THREADS) void k_reduce(const cx<R>* __restrict__ s, uint64_t nk, int q,
+                                                       const uint64_t* __restrict__ masks,
+                                                       const double* __restrict__ w, int nterms,
+                                                       double* __restrict__ part) {
+    double acc = 0.0;
+    for (uint64_t k = gtid(); k < nk; k += gstride()) {
+        if constexpr (MODE == RED_NORM) {
+            acc += prob(s[k]);
+        } else if constexpr (MODE == RED_BIT) {
+            const uint64_t i = ins0(k, q) | (1ULL << q);
+            acc += prob(s[i]);
+        } else {
+            double f = 0.0;
+            for (int t = 0; t < nterms; ++t) f += (__popcll(k & masks[t]) & 1) ? -w[t] : w[t];
+            acc += prob(s[k]) * f;
+        }
+    }
+    acc = wave_sum(acc);
+    __shared__ double sh[RB_THREADS / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = 0.0;
+        for (int i = 0; i < RB_THREADS / 64; ++i) v += sh[i];
+        part[blockIdx.x] = v;
+    }
+}
+
+// Deterministic final sum of partials by one thread (fixed order).
+__global__ void k_sum_partials(const double* __restrict__ part, int n, double* __restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double v = 0.0;
+        for (int i = 0; i < n; ++i) v += part[i];
+        *out = v;
+    }
+}
+
+template <class R>
+static int reduce_t(qse_ctx* ctx, const qse_state* st, int mode, int q, const uint64_t* masks_h, const double* w_h,
+                    int nterms, double* out_h) {
+    const uint64_t N = 1ULL << st->n;
+    const uint64_t nk = (mode == RED_BIT) ? N / 2 : N;
+    const unsigned g = std::min<unsigned>(grid_for(ctx, nk, RB_THREADS, 8), 2048u);
+    const size_t szm = sizeof(uint64_t) * std::max(nterms, 1), szw = sizeof(double) * std::max(nterms, 1);
+    void* base;
+    int rc = scratch(ctx, 2, 256 + szm + szw + sizeof(double) * (g + 1) + 256, &base);
+    if (rc) return rc;
+    char* p = (char*)base;
+    uint64_t* d_m = (uint64_t*)p; p += (szm + 255) & ~(size_t)255;
+    double* d_w = (double*)p; p += (szw + 255) & ~(size_t)255;
+    double* d_part = (double*)p;
+    double* d_out = d_part + g;
+    if (mode == RED_ZSTR && nterms > 0) {
+        QSE_HIP(hipMemcpyAsync(d_m, masks_h, sizeof(uint64_t) * nterms, hipMemcpyHostToDevice, ctx->stream));
+        QSE_HIP(hipMemcpyAsync(d_w, w_h, sizeof(double) * nterms, hipMemcpyHostToDevice, ctx->stream));
+    }
+    const cx<R>* s = (const cx<R>*)st->d;
+    TimedLaunch t;
+    launch_begin(ctx, FAM_READOUT, &t);
+    if (mode == RED_NORM) k_reduce<R, RED_NORM><<<g, RB_THREADS, 0, ctx->stream>>>(s, nk, q, d_m, d_w, nterms, d_part);
+    else if (mode == RED_BIT) k_reduce<R, RED_BIT><<<g, RB_THREADS, 0, ctx->stream>>>(s, nk, q, d_m, d_w, nterms, d_part);
+    else k_reduce<R, RED_ZSTR><<<g, RB_THREADS, 0, ctx->stream>>>(s, nk, q, d_m, d_w, nterms, d_part);
+    QSE_HIP(hipGetLastError());
+    launch_end(ctx, &t, (double)amp_bytes(st->prec) * nk);
+    k_sum_partials<<<1, 64, 0, ctx->stream>>>(d_part, (int)g, d_out);
+    QSE_HIP(hipGetLastError());
+    void* pin;
+    rc = pinned(ctx, sizeof(double), &pin);
+    if (rc) return rc;
+    QSE_HIP(hipMemcpyAsync(pin, d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    QSE_HIP(hipStreamSynchronize(ctx->stream));
+    *out_h = *(double*)pin;
+    return QSE_OK;
+}
+
+int reduce(qse_ctx* ctx, const qse_state* st, int mode, int q, const uint64_t* masks, const double* w, int nterms,
+           double* out) {
+    if (st->prec == QSE_C128) return reduce_t<double>(ctx, st, mode, q, masks, w, nterms, out);
+    return reduce_t<float>(ctx, st, mode, q, masks, w, nterms, out);
+}
+
+// Single-qubit collapse (StateVector.cpp:158-221): keep the `outcome` half
+// scaled by 1/nf; with reset, move it to the |0> half and zero the |1> half.
+template <class R>
+__global__ __launch_bounds__(RB_THREADS) void k_collapse(cx<R>* __restrict__ s, uint64_t nk, int q, int outcome,
+                                                         int reset, double nf) {
+    const uint64_t bit = 1ULL << q;
+    for (uint64_t k = gtid(); k < nk; k += gstride()) {
+        const uint64_t i0 = ins0(k, q), i1 = i0 | bit;
+        const cx<R> a0 = s[i0], a1 = s[i1];
+        const cx<R> z = {(R)0, (R)0};
+        if (!reset) {
+            if (outcome) {
+                s[i0] = z;
+                s[i1] = {(R)(a1.re / nf), (R)(a1.im / nf)};
+            } else {
+                s[i0] = {(R)(a0.re / nf), (R)(a0.im / nf)};
+                s[i1] = z;
+            }
+        } else {
+            const cx<R> keep = outcome ? a1 : a0;
+            s[i0] = {(R)(keep.re / nf), (R)(keep.im / nf)};
+            s[i1] = z;
+        }
+    }
+}
+
+int launch_collapse(qse_ctx* ctx, qse_state* st, int q, int outcome, int reset, double nf) {
+    const uint64_t nk = (1ULL << st->n) / 2;
+    const unsigned g = grid_for(ctx, nk, RB_THREADS);
+    if (st->prec == QSE_C128)
+        k_collapse<double><<<g, RB_THREADS, 0, ctx->stream>>>((cx<double>*)st->d, nk, q, outcome, reset, nf);
+    else
+        k_collapse<float><<<g, RB_THREADS, 0, ctx->stream>>>((cx<float>*)st->d, nk, q, outcome, reset, nf);
+    QSE_HIP(hipGetLastError());
+    return QSE_OK;
+}
+
+}  // namespace qse

@@ -1,0 +1,105 @@
+// qse_run.cpp — gate-list execution entry points: resolves each qse_op into
+// the single-gate sweeps (the replay loop of Circuit::execute,
+// Circuit.cpp:73-112), or hands the list to the fusion planner.
+#include <cmath>
+#include <string>
+
+#include "qse_internal.hpp"
+
+namespace qse {
+
+int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int* passes_out,
+              bool dry_run);
+
+double op_angle(const qse_op& op, const double* params) {
+    if (op.param >= 0 && params) return op.theta + op.scale * params[op.param];
+    return op.theta;
+}
+
+int validate_ops(int n, const qse_op* ops, int nops, const double* params) {
+    for (int k = 0; k < nops; ++k) {
+        const qse_op& o = ops[k];
+        auto badq = [&](int q) { return q < 0 || q >= n; };
+        switch (o.kind) {
+            case QSE_GATE_IDENTITY: break;
+            case QSE_GATE_X: case QSE_GATE_Y: case QSE_GATE_Z: case QSE_GATE_H: case QSE_GATE_PHASE:
+            case QSE_GATE_RX: case QSE_GATE_RY: case QSE_GATE_RZ: case QSE_GATE_MATRIX:
+                if (badq(o.target)) return set_error(QSE_ERR_INVALID_ARG, "op " + std::to_string(k) + ": bad target");
+                break;
+            case QSE_GATE_CNOT: case QSE_GATE_CPHASE: case QSE_GATE_SWAP:
+                if (badq(o.target) || badq(o.control))
+                    return set_error(QSE_ERR_INVALID_ARG, "op " + std::to_string(k) + ": bad qubit");
+                break;
+            case QSE_GATE_MCPHASE:
+                if (badq(o.target) || (n < 64 && (o.controls >> n)))
+                    return set_error(QSE_ERR_INVALID_ARG, "op " + std::to_string(k) + ": bad qubit");
+                break;
+            default:
+                return set_error(QSE_ERR_UNSUPPORTED, "op " + std::to_string(k) + ": unsupported gate kind " +
+                                                          std::to_string(o.kind));
+        }
+        if (o.param >= 0 && !params)
+            return set_error(QSE_ERR_INVALID_ARG, "op " + std::to_string(k) + " is parameterised but params is NULL");
+    }
+    return QSE_OK;
+}
+
+// One op through the single-gate sweeps.
+int apply_one(qse_ctx* ctx, qse_state* st, const qse_op& o, const double* params) {
+    const double th = op_angle(o, params);
+    switch (o.kind) {
+        case QSE_GATE_IDENTITY: return QSE_OK;
+        case QSE_GATE_H: return qse_apply_hadamard(ctx, st, o.target);
+        case QSE_GATE_X: return qse_apply_x(ctx, st, o.target);
+        case QSE_GATE_Y: return qse_apply_y(ctx, st, o.target);
+        case QSE_GATE_Z: return qse_apply_z(ctx, st, o.target);
+        case QSE_GATE_PHASE: return qse_apply_phase(ctx, st, o.target, th);
+        case QSE_GATE_RX: return qse_apply_rx(ctx, st, o.target, th);
+        case QSE_GATE_RY: return qse_apply_ry(ctx, st, o.target, th);
+        case QSE_GATE_RZ: return qse_apply_rz(ctx, st, o.target, th);
+        case QSE_GATE_CNOT: return qse_apply_cnot(ctx, st, o.control, o.target);
+        case QSE_GATE_CPHASE: return qse_apply_cphase(ctx, st, o.control, o.target, th);
+        case QSE_GATE_MCPHASE: {
+            int c[64], nc = 0;
+            for (int q = 0; q < 64; ++q)
+                if ((o.controls >> q) & 1) c[nc++] = q;
+            return qse_apply_mcphase(ctx, st, c, nc, o.target, th);
+        }
+        case QSE_GATE_SWAP: return qse_apply_swap(ctx, st, o.target, o.control);
+        case QSE_GATE_MATRIX: return qse_apply_matrix(ctx, st, o.target, o.m);
+        default: return set_error(QSE_ERR_UNSUPPORTED, "unsupported gate kind");
+    }
+}
+
+int run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params) {
+    for (int k = 0; k < nops; ++k) {
+        int rc = apply_one(ctx, st, ops[k], params);
+        if (rc) return rc;
+    }
+    return QSE_OK;
+}
+
+}  // namespace qse
+
+using namespace qse;
+
+extern "C" {
+
+int qse_run(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params) {
+    if (!ctx || !st || !st->d) return set_error(QSE_ERR_INVALID_ARG, "null context or state");
+    if (nops < 0 || (nops > 0 && !ops)) return set_error(QSE_ERR_INVALID_ARG, "bad op list");
+    int rc = validate_ops(st->n, ops, nops, params);
+    if (rc) return rc;
+    if (nops == 0) return QSE_OK;
+    return run_fused(ctx, st, ops, nops, params, nullptr, false);
+}
+
+int qse_run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params) {
+    if (!ctx || !st || !st->d) return set_error(QSE_ERR_INVALID_ARG, "null context or state");
+    if (nops < 0 || (nops > 0 && !ops)) return set_error(QSE_ERR_INVALID_ARG, "bad op list");
+    int rc = validate_ops(st->n, ops, nops, params);
+    if (rc) return rc;
+    return run_sweeps(ctx, st, ops, nops, params);
+}
+
+}  // extern "C"

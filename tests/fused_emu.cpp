@@ -1,0 +1,174 @@
+// fused_emu.cpp — CPU emulation of the fused-pass kernel (test infrastructure).
+//
+// Builds the same plan as the library (qse_plan.cpp, compiled in) and executes
+// it with the kernel's addressing and op semantics on the host, thread by
+// thread, so the planner (pass splitting, register/thread bit maps, in-tile
+// SWAP relabelling, monomial flushing, external slots) can be checked against
+// the CPU oracle without a GPU. Mirrors k_fused in qse_fused.hip; it is not
+// linked into the product library.
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <utility>
+#include <vector>
+
+#include "qse_fused.hpp"
+#include "qse_hip.h"
+
+namespace qse {
+double op_angle(const qse_op& op, const double* params) {
+    if (op.param >= 0 && params) return op.theta + op.scale * params[op.param];
+    return op.theta;
+}
+int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P, bool relabel);
+}  // namespace qse
+
+using namespace qse;
+using cd = std::complex<double>;
+
+static uint32_t swz(uint32_t li) { return li ^ (((li >> 4) ^ (li >> 8) ^ (li >> 12)) & 15u); }
+
+static void apply_op(const HostPlan& P, const FOp& op, cd* a, uint32_t toff, uint64_t base, const cd* slotv) {
+    if (op.type == FOP_MONO) {
+        cd g = 1.0;
+        for (uint32_t t = op.term_begin; t < op.term_end; ++t) {
+            const FTerm& T = P.terms[t];
+            if ((toff & T.tmask) != T.tmask) continue;
+            if (T.slot >= 0) g *= slotv[T.slot];
+            else if ((base & T.emask) == T.emask) g *= cd(T.d[0], T.d[1]);
+        }
+        for (int j = 0; j < 16; ++j)
+            if (((uint32_t)j & op.rctl) == op.rctl) a[j] *= g;
+        return;
+    }
+    if ((toff & op.tctl) != op.tctl || (base & op.ectl) != op.ectl) return;
+    const int p = op.p;
+    const cd m00(op.m[0], op.m[1]), m01(op.m[2], op.m[3]), m10(op.m[4], op.m[5]), m11(op.m[6], op.m[7]);
+    for (int jj = 0; jj < 8; ++jj) {
+        const int j0 = ((jj >> p) << (p + 1)) | (jj & ((1 << p) - 1)), j1 = j0 | (1 << p);
+        if (((uint32_t)j0 & op.rctl) != op.rctl) continue;
+        const cd v0 = a[j0], v1 = a[j1];
+        if (op.type == FOP_H) {
+            const double h = 0.7071067811865475244;
+            a[j0] = h * (v0 + v1);
+            a[j1] = h * (v0 - v1);
+        } else if (op.type == FOP_XSWAP) {
+            a[j0] = v1;
+            a[j1] = v0;
+        } else {
+            a[j0] = m00 * v0 + m01 * v1;
+            a[j1] = m10 * v0 + m11 * v1;
+        }
+    }
+}
+
+extern "C" int emu_run_fused(int n, int k, int L, const qse_op* ops, int nops, const double* params, double* psi,
+                             int* npasses, int relabel) {
+    HostPlan P;
+    int rc = build_plan(n, k, L, ops, nops, params, P, relabel != 0);
+    if (rc) return rc;
+    *npasses = (int)P.passes.size();
+    const uint64_t N = 1ULL << n;
+    std::vector<cd> bufA(reinterpret_cast<cd*>(psi), reinterpret_cast<cd*>(psi) + N), bufB(N);
+    cd* s = bufA.data();
+    cd* shadow = bufB.data();
+    const int NT = 1 << (k - 4), TB = k - 4;
+    std::vector<cd> tile(1u << k), slotv(FUSE_MAX_SLOTS);
+    for (const FPass& fp : P.passes) {
+        // every local bit must be covered exactly once by the tile positions
+        const uint64_t ntiles = 1ULL << (n - k);
+        cd* dst = fp.out_of_place ? shadow : s;
+        for (uint64_t ti = 0; ti < ntiles; ++ti) {
+            uint64_t base = 0, obase = 0, t = ti;
+            for (int e = 0; e < fp.next; ++e) {
+                base |= (t & 1ULL) << fp.epos[e];
+                obase |= (t & 1ULL) << fp.eout[e];
+                t >>= 1;
+            }
+            for (int sl = 0; sl < fp.nslots; ++sl) {
+                const FSlot& S = P.slots[fp.slot_begin + sl];
+                cd g = 1.0;
+                for (uint32_t e = S.begin; e < S.end; ++e)
+                    if ((base & P.eterms[e].emask) == P.eterms[e].emask) g *= cd(P.eterms[e].d[0], P.eterms[e].d[1]);
+                slotv[sl] = g;
+            }
+            for (int tid = 0; tid < NT; ++tid)
+                for (int i = 0; i < 16; ++i) {
+                    const uint32_t li = (uint32_t)(i * NT + tid);
+                    uint64_t g = base;
+                    for (int b = 0; b < k; ++b)
+                        if ((li >> b) & 1) g |= 1ULL << fp.tq[b];
+                    tile[swz(li)] = s[g];
+                }
+            for (int sg = 0; sg < fp.nstages; ++sg) {
+                const FStage& S = P.stages[fp.stage_begin + sg];
+                for (int tid = 0; tid < NT; ++tid) {
+                    uint32_t toff = 0;
+                    for (int tb = 0; tb < TB; ++tb)
+                        if ((tid >> tb) & 1) toff |= 1u << S.tmap[tb];
+                    uint32_t li[16];
+                    cd a[16];
+                    for (int j = 0; j < 16; ++j) {
+                        uint32_t l = toff;
+                        for (int p = 0; p < 4; ++p)
+                            if ((j >> p) & 1) l |= 1u << S.rmap[p];
+                        li[j] = swz(l);
+                        a[j] = tile[li[j]];
+                    }
+                    for (uint32_t oi = S.op_begin; oi < S.op_end; ++oi)
+                        apply_op(P, P.ops[oi], a, toff, base, slotv.data());
+                    for (int j = 0; j < 16; ++j) tile[li[j]] = a[j];
+                }
+            }
+            for (int tid = 0; tid < NT; ++tid)
+                for (int i = 0; i < 16; ++i) {
+                    const uint32_t go = (uint32_t)(i * NT + tid);
+                    uint64_t g = obase;
+                    uint32_t ls = 0;
+                    for (int b = 0; b < k; ++b)
+                        if ((go >> b) & 1) {
+                            g |= 1ULL << fp.so_q[b];
+                            ls |= 1u << fp.so_l[b];
+                        }
+                    dst[g] = tile[swz(ls)];
+                }
+        }
+        if (fp.out_of_place) std::swap(s, shadow);
+    }
+    std::memcpy(psi, s, sizeof(cd) * N);
+    return 0;
+}
+
+// Structural checks of a plan: stage maps partition the tile bits, the store
+// permutation is a permutation, passes respect the tile size.
+extern "C" int emu_check_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, int relabel) {
+    HostPlan P;
+    if (build_plan(n, k, L, ops, nops, params, P, relabel != 0)) return -1;
+    for (const FPass& fp : P.passes) {
+        if (fp.k != k || fp.next != n - k) return 1;
+        uint64_t seen = 0, dq = 0, sq = 0;
+        for (int b = 0; b < k; ++b) {
+            seen |= 1ULL << fp.so_l[b];
+            dq |= 1ULL << fp.so_q[b];
+            sq |= 1ULL << fp.tq[b];
+            if (b > 0 && fp.so_q[b] <= fp.so_q[b - 1]) return 6;
+        }
+        if (seen != ((1ULL << k) - 1)) return 2;
+        for (int e = 0; e < fp.next; ++e) dq |= 1ULL << fp.eout[e];
+        if (dq != ((1ULL << n) - 1)) return 7;
+        if (!fp.out_of_place && sq != (dq & sq)) return 8;
+        for (int b = 0; b < L; ++b)
+            if (fp.so_q[b] != b) return 9;
+        for (int b = 0; b < L; ++b)
+            if (fp.tq[b] != b) return 3;
+        for (int sg = 0; sg < fp.nstages; ++sg) {
+            const FStage& S = P.stages[fp.stage_begin + sg];
+            uint64_t cov = 0;
+            for (int p = 0; p < 4; ++p) cov |= 1ULL << S.rmap[p];
+            for (int t = 0; t < k - 4; ++t) cov |= 1ULL << S.tmap[t];
+            if (cov != ((1ULL << k) - 1)) return 4;
+        }
+        if (fp.nslots > FUSE_MAX_SLOTS) return 5;
+    }
+    return 0;
+}
