@@ -32,77 +32,190 @@ int run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const d
 
 __device__ __forceinline__ uint32_t swz(uint32_t li) { return li ^ (((li >> 4) ^ (li >> 8) ^ (li >> 12)) & 15u); }
 
+// One 2x2 op on register position P over the 8 register pairs (j0, j0|2^P).
+// Pairs whose register controls fail get the identity (a wave-uniform select
+// of coefficients, not a branch). H and X use the same formula: X = [0,1;1,0]
+// is exact, H differs from s*(v0 +- v1) by at most 1 ulp.
+// Uncontrolled op with real entries (RY, real 2x2): 4 FP ops per amplitude.
 template <class R, int P>
-__device__ __forceinline__ void dense_p(cx<R>* a, const FOp& op) {
-    const uint32_t rctl = op.rctl;
-    const cx<R> m00 = {(R)op.m[0], (R)op.m[1]}, m01 = {(R)op.m[2], (R)op.m[3]};
-    const cx<R> m10 = {(R)op.m[4], (R)op.m[5]}, m11 = {(R)op.m[6], (R)op.m[7]};
-    const uint8_t type = op.type;
+__device__ __forceinline__ void real_p(cx<R>* a, const FOp& op) {
+    const R m00 = (R)op.m[0], m01 = (R)op.m[2], m10 = (R)op.m[4], m11 = (R)op.m[6];
 #pragma unroll
     for (int jj = 0; jj < 8; ++jj) {
         const int j0 = ((jj >> P) << (P + 1)) | (jj & ((1 << P) - 1));
         const int j1 = j0 | (1 << P);
-        if (((uint32_t)j0 & rctl) != rctl) continue;
         const cx<R> v0 = a[j0], v1 = a[j1];
-        if (type == FOP_H) {
-            const R h = (R)0.7071067811865475244;
-            a[j0] = {h * (v0.re + v1.re), h * (v0.im + v1.im)};
-            a[j1] = {h * (v0.re - v1.re), h * (v0.im - v1.im)};
-        } else if (type == FOP_XSWAP) {
-            a[j0] = v1;
-            a[j1] = v0;
-        } else {
-            a[j0] = cadd(cmul(m00, v0), cmul(m01, v1));
-            a[j1] = cadd(cmul(m10, v0), cmul(m11, v1));
-        }
+        a[j0] = {m00 * v0.re + m01 * v1.re, m00 * v0.im + m01 * v1.im};
+        a[j1] = {m10 * v0.re + m11 * v1.re, m10 * v0.im + m11 * v1.im};
     }
 }
 
-template <class R>
-__device__ __forceinline__ void apply_op(const FOp& op, cx<R>* a, uint32_t toff, uint64_t base, const FProgram& prog,
-                                         const cx<double>* slotv) {
-    if (op.type == FOP_MONO) {
-        double gr = 1.0, gi = 0.0;
-        for (uint32_t t = op.term_begin; t < op.term_end; ++t) {
-            const FTerm& T = prog.terms[t];
-            if ((toff & T.tmask) != T.tmask) continue;
-            double dr, di;
-            if (T.slot >= 0) {
-                dr = slotv[T.slot].re;
-                di = slotv[T.slot].im;
-            } else {
-                if ((base & T.emask) != T.emask) continue;
-                dr = T.d[0];
-                di = T.d[1];
-            }
+// Uncontrolled op [c0, i s0; i s1, c1] with real c, s (RX): 4 FP ops per amplitude.
+template <class R, int P>
+__device__ __forceinline__ void rxlike_p(cx<R>* a, const FOp& op) {
+    const R c0 = (R)op.m[0], s0 = (R)op.m[3], s1 = (R)op.m[5], c1 = (R)op.m[6];
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+        const int j0 = ((jj >> P) << (P + 1)) | (jj & ((1 << P) - 1));
+        const int j1 = j0 | (1 << P);
+        const cx<R> v0 = a[j0], v1 = a[j1];
+        a[j0] = {c0 * v0.re - s0 * v1.im, c0 * v0.im + s0 * v1.re};
+        a[j1] = {c1 * v1.re - s1 * v0.im, c1 * v1.im + s1 * v0.re};
+    }
+}
+
+// Uncontrolled X: exchange registers.
+template <class R, int P>
+__device__ __forceinline__ void xswap_p(cx<R>* a) {
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+        const int j0 = ((jj >> P) << (P + 1)) | (jj & ((1 << P) - 1));
+        const int j1 = j0 | (1 << P);
+        const cx<R> v0 = a[j0];
+        a[j0] = a[j1];
+        a[j1] = v0;
+    }
+}
+
+template <class R, int P>
+__device__ __forceinline__ void dense_p(cx<R>* a, const FOp& op) {
+    const uint32_t rctl = op.rctl;
+    const R m00r = (R)op.m[0], m00i = (R)op.m[1], m01r = (R)op.m[2], m01i = (R)op.m[3];
+    const R m10r = (R)op.m[4], m10i = (R)op.m[5], m11r = (R)op.m[6], m11i = (R)op.m[7];
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+        const int j0 = ((jj >> P) << (P + 1)) | (jj & ((1 << P) - 1));
+        const int j1 = j0 | (1 << P);
+        const bool act = ((uint32_t)j0 & rctl) == rctl;  // wave-uniform
+        const cx<R> c00 = {act ? m00r : (R)1, act ? m00i : (R)0}, c01 = {act ? m01r : (R)0, act ? m01i : (R)0};
+        const cx<R> c10 = {act ? m10r : (R)0, act ? m10i : (R)0}, c11 = {act ? m11r : (R)1, act ? m11i : (R)0};
+        const cx<R> v0 = a[j0], v1 = a[j1];
+        a[j0] = cadd(cmul(c00, v0), cmul(c01, v1));
+        a[j1] = cadd(cmul(c10, v0), cmul(c11, v1));
+    }
+}
+
+// Uncontrolled Hadamard on register position P: s(v0+v1), s(v0-v1), the
+// reference's rounding (CircuitUnitaryOperation.cpp:101-102).
+template <class R, int P>
+__device__ __forceinline__ void hadamard_p(cx<R>* a) {
+    const R h = (R)0.7071067811865475244;
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+        const int j0 = ((jj >> P) << (P + 1)) | (jj & ((1 << P) - 1));
+        const int j1 = j0 | (1 << P);
+        const cx<R> v0 = a[j0], v1 = a[j1];
+        a[j0] = {h * (v0.re + v1.re), h * (v0.im + v1.im)};
+        a[j1] = {h * (v0.re - v1.re), h * (v0.im - v1.im)};
+    }
+}
+
+// Program arrays are kernel arguments of their own, const and __restrict__:
+// the compiler then reads ops and terms (wave-uniform addresses) with scalar
+// loads through the constant cache instead of vector loads.
+struct KArgs {
+    const FPass* __restrict__ passes;
+    const FStage* __restrict__ stages;
+    const FOp* __restrict__ ops;
+    const FTerm* __restrict__ terms;
+    const FSlot* __restrict__ slots;
+    const FExtTerm* __restrict__ eterms;
+    const double* __restrict__ alphas;
+};
+
+struct SlotVal {
+    double ang, c, s, pad;
+};
+
+// Per-tile external factors of one pass, for every tile at once: entry
+// (tile, slot) = angle sum of the slot's external terms whose qubits are all
+// set in the tile's external bits, with its cos and sin. The fused pass reads
+// one 32-byte entry per slot per tile instead of summing the terms itself.
+__global__ __launch_bounds__(256) void k_slot_table(const FPass* __restrict__ passes, const FSlot* __restrict__ slots,
+                                                    const FExtTerm* __restrict__ eterms, int pass_idx,
+                                                    uint64_t ntiles, SlotVal* __restrict__ table) {
+    const FPass& P = passes[pass_idx];
+    const uint64_t total = ntiles * (uint64_t)P.nslots;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t ti = g / (uint64_t)P.nslots;
+        const int sl = (int)(g - ti * (uint64_t)P.nslots);
+        uint64_t base = 0, t = ti;
+        for (int e = 0; e < P.next; ++e) {
+            base |= (t & 1ULL) << P.epos[e];
+            t >>= 1;
+        }
+        const FSlot S = slots[P.slot_begin + sl];
+        double a = 0.0;
+        for (uint32_t e = S.begin; e < S.end; ++e)
+            if ((base & eterms[e].emask) == eterms[e].emask) a += eterms[e].ang;
+        double sn, cs;
+        sincos(a, &sn, &cs);
+        table[g] = {a, cs, sn, 0.0};
+    }
+}
+
+// Factor of a MONO op for this thread: constant x product over the thread-id
+// bits set in both tid and op.tctl of their factor x product of the terms whose
+// thread/external conditions hold (external ones from the per-tile slots).
+template <int TB>
+__device__ __forceinline__ void mono_factor(const FOp& op, const FTerm* __restrict__ terms,
+                                            const double* __restrict__ alphas, int tid, uint32_t toff,
+                                            uint64_t base, const SlotVal* slotv, double& gr, double& gi) {
+    gr = op.m[0];
+    gi = op.m[1];
+    uint32_t am = op.tctl;
+    if (am) {
+        const double* al = alphas + op.alpha_begin;
+        while (am) {
+            const int t = __builtin_ctz(am);
+            am &= am - 1;
+            const bool on = (tid >> t) & 1;
+            const double dr = on ? al[2 * t] : 1.0, di = on ? al[2 * t + 1] : 0.0;
             const double nr = gr * dr - gi * di;
             gi = gr * di + gi * dr;
             gr = nr;
         }
-        const cx<R> g = {(R)gr, (R)gi};
-        const uint32_t rctl = op.rctl;
-#pragma unroll
-        for (int j = 0; j < 16; ++j)
-            if (((uint32_t)j & rctl) == rctl) a[j] = cmul(g, a[j]);
-        return;
     }
-    if ((toff & op.tctl) != op.tctl || (base & op.ectl) != op.ectl) return;
-    switch (op.p) {
-        case 0: dense_p<R, 0>(a, op); break;
-        case 1: dense_p<R, 1>(a, op); break;
-        case 2: dense_p<R, 2>(a, op); break;
-        default: dense_p<R, 3>(a, op); break;
+    for (uint32_t t = op.term_begin; t < op.term_end; ++t) {
+        const FTerm& T = terms[t];
+        double dr, di;
+        bool on = (toff & T.tmask) == T.tmask;
+        if (T.slot >= 0) {
+            dr = slotv[T.slot].c;
+            di = slotv[T.slot].s;
+        } else {
+            on = on && ((base & T.emask) == T.emask);
+            dr = T.d[0];
+            di = T.d[1];
+        }
+        dr = on ? dr : 1.0;
+        di = on ? di : 0.0;
+        const double nr = gr * dr - gi * di;
+        gi = gr * di + gi * dr;
+        gr = nr;
     }
 }
 
+// Two waves per SIMD: two workgroups share a CU (one computes while the other
+// waits on memory).
 template <class R, int K>
-__global__ __launch_bounds__(1 << (K - 4)) void k_fused(const cx<R>* src, cx<R>* dst, FProgram prog, int pass_idx,
+__global__ __launch_bounds__(1 << (K - 4)) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_fused(const cx<R>* src, cx<R>* dst,
+                                                        const FPass* __restrict__ passes,
+                                                        const FStage* __restrict__ stages,
+                                                        const FOp* __restrict__ ops,
+                                                        const FTerm* __restrict__ terms,
+                                                        const FSlot* __restrict__ slots,
+                                                        const FExtTerm* __restrict__ eterms,
+                                                        const double* __restrict__ alphas,
+                                                        const SlotVal* __restrict__ slot_table, int pass_idx,
                                                         uint64_t ntiles) {
+    const KArgs A{passes, stages, ops, terms, slots, eterms, alphas};
     constexpr int NT = 1 << (K - 4);
     constexpr int TB = K - 4;
     __shared__ cx<R> tile[1 << K];
-    __shared__ cx<double> slotv[FUSE_MAX_SLOTS];
-    const FPass& P = prog.passes[pass_idx];
+    __shared__ SlotVal slotv[FUSE_MAX_SLOTS];
+    const FPass& P = A.passes[pass_idx];
     const int tid = threadIdx.x;
     uint64_t gt = 0, ot = 0;
     uint32_t st_t = 0;
@@ -122,6 +235,7 @@ __global__ __launch_bounds__(1 << (K - 4)) void k_fused(const cx<R>* src, cx<R>*
         sw[c] = 1u << P.so_l[TB + c];
     }
     const int next = P.next, nstages = P.nstages, nslots = P.nslots;
+    const uint32_t stage_begin = P.stage_begin;
     for (uint64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         uint64_t base = 0, obase = 0;
         {
@@ -132,19 +246,8 @@ __global__ __launch_bounds__(1 << (K - 4)) void k_fused(const cx<R>* src, cx<R>*
                 t >>= 1;
             }
         }
-        for (int sl = tid; sl < nslots; sl += NT) {
-            const FSlot S = prog.slots[P.slot_begin + sl];
-            double gr = 1.0, gi = 0.0;
-            for (uint32_t e = S.begin; e < S.end; ++e) {
-                const FExtTerm& E = prog.eterms[e];
-                if ((base & E.emask) != E.emask) continue;
-                const double nr = gr * E.d[0] - gi * E.d[1];
-                gi = gr * E.d[1] + gi * E.d[0];
-                gr = nr;
-            }
-            slotv[sl] = {gr, gi};
-        }
-        // load
+        // load (issued first so the slot sums below overlap the memory latency)
+        cx<R> ld[16];
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             uint64_t g = base | gt;
@@ -152,27 +255,82 @@ __global__ __launch_bounds__(1 << (K - 4)) void k_fused(const cx<R>* src, cx<R>*
             if (i & 2) g |= gw[1];
             if (i & 4) g |= gw[2];
             if (i & 8) g |= gw[3];
-            tile[swz((uint32_t)(i * NT + tid))] = src[g];
+            ld[i] = src[g];
         }
+        // per-tile external factors (precomputed by k_slot_table)
+        for (int sl = tid; sl < nslots; sl += NT) slotv[sl] = slot_table[ti * (uint64_t)nslots + sl];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) tile[swz((uint32_t)(i * NT + tid))] = ld[i];
         __syncthreads();
         for (int sg = 0; sg < nstages; ++sg) {
-            const FStage& S = prog.stages[P.stage_begin + sg];
+            const FStage& S = A.stages[stage_begin + sg];
             uint32_t toff = 0;
 #pragma unroll
             for (int t = 0; t < TB; ++t)
                 if ((tid >> t) & 1) toff |= 1u << S.tmap[t];
             const uint32_t r0 = 1u << S.rmap[0], r1 = 1u << S.rmap[1], r2 = 1u << S.rmap[2], r3 = 1u << S.rmap[3];
-            uint32_t li[16];
             cx<R> a[16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                li[j] = swz(toff | ((j & 1) ? r0 : 0u) | ((j & 2) ? r1 : 0u) | ((j & 4) ? r2 : 0u) |
-                            ((j & 8) ? r3 : 0u));
-                a[j] = tile[li[j]];
-            }
-            for (uint32_t oi = S.op_begin; oi < S.op_end; ++oi) apply_op<R>(prog.ops[oi], a, toff, base, prog, slotv);
+            for (int j = 0; j < 16; ++j)
+                a[j] = tile[swz(toff | ((j & 1) ? r0 : 0u) | ((j & 2) ? r1 : 0u) | ((j & 4) ? r2 : 0u) |
+                                ((j & 8) ? r3 : 0u))];
+            const uint32_t oe = S.op_end;
+            for (uint32_t oi = S.op_begin; oi < oe; ++oi) {
+                const FOp& op = A.ops[oi];
+                if (op.type == FOP_MONO) {
+                    double gr, gi;
+                    mono_factor<TB>(op, A.terms, A.alphas, tid, toff, base, slotv, gr, gi);
+                    const cx<R> g = {(R)gr, (R)gi};
+                    const uint32_t rctl = op.rctl;
+                    if (rctl == 0) {
 #pragma unroll
-            for (int j = 0; j < 16; ++j) tile[li[j]] = a[j];
+                        for (int j = 0; j < 16; ++j) a[j] = cmul(g, a[j]);
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 16; ++j)
+                            if (((uint32_t)j & rctl) == rctl) a[j] = cmul(g, a[j]);  // wave-uniform branch
+                    }
+                } else if (op.type == FOP_H) {
+                    switch (op.p) {
+                        case 0: hadamard_p<R, 0>(a); break;
+                        case 1: hadamard_p<R, 1>(a); break;
+                        case 2: hadamard_p<R, 2>(a); break;
+                        default: hadamard_p<R, 3>(a); break;
+                    }
+                } else if (op.type == FOP_REAL) {
+                    switch (op.p) {
+                        case 0: real_p<R, 0>(a, op); break;
+                        case 1: real_p<R, 1>(a, op); break;
+                        case 2: real_p<R, 2>(a, op); break;
+                        default: real_p<R, 3>(a, op); break;
+                    }
+                } else if (op.type == FOP_RXLIKE) {
+                    switch (op.p) {
+                        case 0: rxlike_p<R, 0>(a, op); break;
+                        case 1: rxlike_p<R, 1>(a, op); break;
+                        case 2: rxlike_p<R, 2>(a, op); break;
+                        default: rxlike_p<R, 3>(a, op); break;
+                    }
+                } else if (op.type == FOP_XSWAP && op.rctl == 0 && op.tctl == 0 && op.ectl == 0) {
+                    switch (op.p) {
+                        case 0: xswap_p<R, 0>(a); break;
+                        case 1: xswap_p<R, 1>(a); break;
+                        case 2: xswap_p<R, 2>(a); break;
+                        default: xswap_p<R, 3>(a); break;
+                    }
+                } else if (((toff & op.tctl) == op.tctl) && ((base & op.ectl) == op.ectl)) {
+                    switch (op.p) {
+                        case 0: dense_p<R, 0>(a, op); break;
+                        case 1: dense_p<R, 1>(a, op); break;
+                        case 2: dense_p<R, 2>(a, op); break;
+                        default: dense_p<R, 3>(a, op); break;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                tile[swz(toff | ((j & 1) ? r0 : 0u) | ((j & 2) ? r1 : 0u) | ((j & 4) ? r2 : 0u) | ((j & 8) ? r3 : 0u))] =
+                    a[j];
             __syncthreads();
         }
         // store
@@ -190,29 +348,29 @@ __global__ __launch_bounds__(1 << (K - 4)) void k_fused(const cx<R>* src, cx<R>*
     }
 }
 
-
 namespace {
 
 size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct Serialized {
     std::vector<char> bytes;
-    size_t off[6];
+    size_t off[7];
 };
 
 void serialize(const HostPlan& P, Serialized& S) {
-    const size_t sz[6] = {sizeof(FPass) * P.passes.size(), sizeof(FStage) * P.stages.size(),
-                          sizeof(FOp) * P.ops.size(), sizeof(FTerm) * P.terms.size(),
-                          sizeof(FSlot) * P.slots.size(), sizeof(FExtTerm) * P.eterms.size()};
-    const void* src[6] = {P.passes.data(), P.stages.data(), P.ops.data(),
-                          P.terms.data(), P.slots.data(), P.eterms.data()};
+    const size_t sz[7] = {sizeof(FPass) * P.passes.size(), sizeof(FStage) * P.stages.size(),
+                          sizeof(FOp) * P.ops.size(),       sizeof(FTerm) * P.terms.size(),
+                          sizeof(FSlot) * P.slots.size(),   sizeof(FExtTerm) * P.eterms.size(),
+                          sizeof(double) * P.alphas.size()};
+    const void* src[7] = {P.passes.data(), P.stages.data(), P.ops.data(),   P.terms.data(),
+                          P.slots.data(),  P.eterms.data(), P.alphas.data()};
     size_t tot = 0;
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < 7; ++i) {
         S.off[i] = tot;
         tot += al256(std::max<size_t>(sz[i], 1));
     }
     S.bytes.assign(tot, 0);
-    for (int i = 0; i < 6; ++i)
+    for (int i = 0; i < 7; ++i)
         if (sz[i]) std::memcpy(S.bytes.data() + S.off[i], src[i], sz[i]);
 }
 
@@ -247,23 +405,26 @@ int program_on_device(qse_ctx* ctx, const Serialized& S, char** dev) {
 }
 
 template <class R, int K>
-static void launch_k(qse_ctx* ctx, const cx<R>* src, cx<R>* dst, const FProgram& prog, int pass, uint64_t ntiles) {
+static void launch_k(qse_ctx* ctx, const cx<R>* src, cx<R>* dst, const FProgram& prog, const SlotVal* table,
+                     int pass, uint64_t ntiles) {
     const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)ctx->num_cus * 16);
-    k_fused<R, K><<<grid, 1 << (K - 4), 0, ctx->stream>>>(src, dst, prog, pass, ntiles);
+    k_fused<R, K><<<grid, 1 << (K - 4), 0, ctx->stream>>>(src, dst, prog.passes, prog.stages, prog.ops, prog.terms,
+                                                          prog.slots, prog.eterms, prog.alphas, table, pass, ntiles);
 }
 
 template <class R>
-static int launch_pass(qse_ctx* ctx, qse_state* st, int K, const FProgram& prog, int pass, bool oop) {
+static int launch_pass(qse_ctx* ctx, qse_state* st, int K, const FProgram& prog, const SlotVal* table, int pass,
+                       bool oop) {
     const uint64_t ntiles = 1ULL << (st->n - K);
     const cx<R>* s = (const cx<R>*)st->d;
     cx<R>* d = oop ? (cx<R>*)st->shadow : (cx<R>*)st->d;
     switch (K) {
-        case 8: launch_k<R, 8>(ctx, s, d, prog, pass, ntiles); break;
-        case 9: launch_k<R, 9>(ctx, s, d, prog, pass, ntiles); break;
-        case 10: launch_k<R, 10>(ctx, s, d, prog, pass, ntiles); break;
-        case 11: launch_k<R, 11>(ctx, s, d, prog, pass, ntiles); break;
-        case 12: launch_k<R, 12>(ctx, s, d, prog, pass, ntiles); break;
-        case 13: launch_k<R, 13>(ctx, s, d, prog, pass, ntiles); break;
+        case 8: launch_k<R, 8>(ctx, s, d, prog, table, pass, ntiles); break;
+        case 9: launch_k<R, 9>(ctx, s, d, prog, table, pass, ntiles); break;
+        case 10: launch_k<R, 10>(ctx, s, d, prog, table, pass, ntiles); break;
+        case 11: launch_k<R, 11>(ctx, s, d, prog, table, pass, ntiles); break;
+        case 12: launch_k<R, 12>(ctx, s, d, prog, table, pass, ntiles); break;
+        case 13: launch_k<R, 13>(ctx, s, d, prog, table, pass, ntiles); break;
         default: return set_error(QSE_ERR_UNSUPPORTED, "tile size outside [8, 13]");
     }
     QSE_HIP(hipGetLastError());
@@ -280,6 +441,7 @@ int choose_plan(const qse_ctx* ctx, int n, int k, int L, const qse_op* ops, int 
     (void)ctx;
     HostPlan A, B;
     int rc = build_plan(n, k, L, ops, nops, params, A, true);
+    if (rc == QSE_ERR_UNSUPPORTED) return build_plan(n, k, L, ops, nops, params, P, false);
     if (rc) return rc;
     if (A.passes.empty() || !A.passes.back().out_of_place) {
         P = std::move(A);
@@ -327,13 +489,30 @@ int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const do
     prog.terms = (const FTerm*)(dev + S.off[3]);
     prog.slots = (const FSlot*)(dev + S.off[4]);
     prog.eterms = (const FExtTerm*)(dev + S.off[5]);
+    prog.alphas = (const double*)(dev + S.off[6]);
     const double bytes = 2.0 * (double)amp_bytes(st->prec) * (double)(1ULL << st->n);
+    const uint64_t ntiles = 1ULL << (st->n - k);
+    int max_slots = 0;
+    for (const FPass& fp : P.passes) max_slots = std::max(max_slots, fp.nslots);
+    SlotVal* table = nullptr;
+    if (max_slots > 0) {
+        void* tb;
+        rc = scratch(ctx, 5, sizeof(SlotVal) * ntiles * (uint64_t)max_slots, &tb);
+        if (rc) return rc;
+        table = (SlotVal*)tb;
+    }
     for (int p = 0; p < (int)P.passes.size(); ++p) {
+        if (P.passes[p].nslots > 0) {
+            const uint64_t work = ntiles * (uint64_t)P.passes[p].nslots;
+            k_slot_table<<<grid_for(ctx, work, 256, 8), 256, 0, ctx->stream>>>(prog.passes, prog.slots, prog.eterms,
+                                                                              p, ntiles, table);
+            QSE_HIP(hipGetLastError());
+        }
         TimedLaunch t;
         launch_begin(ctx, FAM_FUSED, &t);
         const bool oop = P.passes[p].out_of_place != 0;
-        rc = (st->prec == QSE_C128) ? launch_pass<double>(ctx, st, k, prog, p, oop)
-                                    : launch_pass<float>(ctx, st, k, prog, p, oop);
+        rc = (st->prec == QSE_C128) ? launch_pass<double>(ctx, st, k, prog, table, p, oop)
+                                    : launch_pass<float>(ctx, st, k, prog, table, p, oop);
         if (rc) return rc;
         launch_end(ctx, &t, bytes);
         if (oop) std::swap(st->d, st->shadow);

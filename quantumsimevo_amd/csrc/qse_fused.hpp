@@ -25,45 +25,55 @@ constexpr int FUSE_MAX_K = 13;     // largest tile (qubits)
 constexpr int FUSE_MIN_K = 8;      // smallest tile (16 threads)
 constexpr int FUSE_MAX_SLOTS = 256;  // per-tile external factor slots per pass
 
-enum FOpType : uint8_t {
+enum FOpType : uint32_t {
     FOP_DENSE = 0,  // 2x2 matrix on register position p
-    FOP_H = 1,      // Hadamard with the reference's rounding: s(v0+v1), s(v0-v1)
+    FOP_H = 1,      // uncontrolled Hadamard with the reference's rounding: s(v0+v1), s(v0-v1)
     FOP_XSWAP = 2,  // exchange (X, CNOT target)
-    FOP_MONO = 3    // multiply by the product of the op's terms
+    FOP_MONO = 3,   // multiply by the product of the op's terms
+    // Uncontrolled specialisations chosen by the planner (same math as FOP_DENSE):
+    FOP_REAL = 4,   // all four entries real
+    FOP_RXLIKE = 5  // real diagonal, imaginary off-diagonal
 };
 
+// All fields are 32/64-bit so the kernel reads them with scalar loads (no
+// byte-sized scalar loads on gfx950).
 struct FOp {
-    uint8_t type;
-    uint8_t p;        // register position of the target (dense ops)
-    uint8_t rctl;     // register-position mask that must be set
-    uint8_t pad;
-    uint32_t tctl;    // dense: non-register tile local bits that must be set
+    uint32_t type;
+    uint32_t p;       // register position of the target (dense ops)
+    uint32_t rctl;    // register-position mask that must be set
+    uint32_t mode;    // unused (0)
+    uint32_t tctl;    // dense: non-register tile local bits that must be set;
+                      // MONO: thread-id bits that carry a factor in alphas
     uint32_t term_begin, term_end;  // MONO: range of FTerm
-    uint32_t pad2;
+    uint32_t alpha_begin;           // MONO: FUSE_NO_ALPHA or index of k-4 complex thread-bit factors
     uint64_t ectl;    // dense: external qubits (global bit mask) that must be set
-    double m[8];      // dense: row-major 2x2, (re, im) interleaved
+    double m[8];      // dense: row-major 2x2, (re, im) interleaved; MONO: m[0..1] = constant factor
 };
+
+constexpr uint32_t FUSE_NO_ALPHA = 0xffffffffu;
 
 struct FTerm {
     uint32_t tmask;   // non-register tile local bits that must be set
-    int32_t slot;     // >= 0: factor from the per-tile external slot; else d
+    int32_t slot;     // >= 0: factor from the per-tile external slot; else d / ang
     uint64_t emask;   // slot < 0: external qubits (global mask) that must be set
-    double d[2];
+    double d[2];      // factor (unit modulus: every monomial comes from a phase gate)
+    double ang;       // its angle: d = (cos ang, sin ang)
+    double pad;
 };
 
 struct FStage {
-    uint8_t rmap[FUSE_REG];   // register position -> tile local bit
-    uint8_t tmap[16];         // thread-id bit -> tile local bit (k-4 entries)
+    uint32_t rmap[FUSE_REG];  // register position -> tile local bit
+    uint32_t tmap[16];        // thread-id bit -> tile local bit (k-4 entries)
     uint32_t op_begin, op_end;
 };
 
 struct FExtTerm {
     uint64_t emask;   // external qubits (global mask) that must be set
-    double d[2];
+    double ang;       // phase angle contributed when they are
 };
 
 struct FSlot {
-    uint32_t begin, end;  // range of FExtTerm; factor = product of passing terms
+    uint32_t begin, end;  // range of FExtTerm; slot angle = sum of passing terms' angles
 };
 
 struct FPass {
@@ -71,11 +81,11 @@ struct FPass {
     int32_t next;         // external qubits (n - k)
     int32_t nstages;
     int32_t nslots;
-    uint8_t tq[16];       // load: tile position b -> source qubit (ascending)
-    uint8_t so_q[16];     // store: store position c -> destination qubit (ascending)
-    uint8_t so_l[16];     // store: store position c -> tile local bit holding it
-    uint8_t epos[64];     // external source qubits, ascending (tile index bits)
-    uint8_t eout[64];     // destination qubit of external source qubit epos[e]
+    uint32_t tq[16];       // load: tile position b -> source qubit (ascending)
+    uint32_t so_q[16];     // store: store position c -> destination qubit (ascending)
+    uint32_t so_l[16];     // store: store position c -> tile local bit holding it
+    uint32_t epos[64];     // external source qubits, ascending (tile index bits)
+    uint32_t eout[64];     // destination qubit of external source qubit epos[e]
     uint32_t stage_begin;
     uint32_t slot_begin;
     int32_t out_of_place; // 1: write to the other buffer (qubit permutation)
@@ -90,6 +100,7 @@ struct FProgram {
     const FTerm* terms;
     const FSlot* slots;
     const FExtTerm* eterms;
+    const double* alphas;
 };
 
 // Host-side plan (built by qse_plan.cpp, serialized and launched by qse_fused.hip).
@@ -101,6 +112,7 @@ struct HostPlan {
     std::vector<FTerm> terms;
     std::vector<FSlot> slots;
     std::vector<FExtTerm> eterms;
+    std::vector<double> alphas;
 };
 
 }  // namespace qse

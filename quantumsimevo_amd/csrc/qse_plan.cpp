@@ -38,7 +38,7 @@ enum ItemType { IT_DENSE, IT_MONO, IT_SWAP };
 struct Item {
     ItemType t;
     int q = -1, q2 = -1;   // dense target / swap qubits
-    uint8_t dtype = FOP_DENSE;
+    uint32_t dtype = FOP_DENSE;
     uint64_t cmask = 0;    // dense: control qubits
     cd m[4];
     uint64_t qmask = 0;    // mono: qubits that must be 1
@@ -54,7 +54,7 @@ struct Mono {
 struct SOp {
     bool mono;
     // dense
-    uint8_t dtype;
+    uint32_t dtype;
     int lt;
     uint64_t lctl, ectl;
     cd m[4];
@@ -73,7 +73,7 @@ void to_items(const qse_op* ops, int nops, const double* params, std::vector<Ite
         const qse_op& o = ops[k];
         const double th = op_angle(o, params);
         Item it;
-        auto dense = [&](uint8_t dt, cd a, cd b, cd c, cd d) {
+        auto dense = [&](uint32_t dt, cd a, cd b, cd c, cd d) {
             it.t = IT_DENSE;
             it.q = o.target;
             it.dtype = dt;
@@ -209,6 +209,8 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
         // tile qubits of the last pass for coalesced writes
         uint64_t need = 0;
         for (int q = 0; q < L; ++q) need |= 1ULL << phys[q];
+        if (popc(lowT | need) > k)
+            return set_error(QSE_ERR_UNSUPPORTED, "relabelled plan: read and write runs do not fit one tile");
         PassItems* cur = &passes.back();
         if (popc(cur->T | need) > k) passes.push_back({lowT | need, {}});
         else cur->T |= need;
@@ -225,10 +227,10 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
         for (int q = 0; q < n; ++q) ((pi.T >> q) & 1 ? tq : epos).push_back(q);
         std::vector<int> loc(64, -1);
         for (int b = 0; b < k; ++b) {
-            fp.tq[b] = (uint8_t)tq[b];
+            fp.tq[b] = (uint32_t)tq[b];
             loc[tq[b]] = b;
         }
-        for (size_t e = 0; e < epos.size(); ++e) fp.epos[e] = (uint8_t)epos[e];
+        for (size_t e = 0; e < epos.size(); ++e) fp.epos[e] = (uint32_t)epos[e];
 
         std::vector<SStage> stages(1);
         std::vector<Mono> pending;
@@ -293,10 +295,10 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
             for (int b = 0; b < k; ++b) byd.push_back({dest[tq[b]], loc[tq[b]]});
             std::sort(byd.begin(), byd.end());
             for (int c = 0; c < k; ++c) {
-                fp.so_q[c] = (uint8_t)byd[c].first;
-                fp.so_l[c] = (uint8_t)byd[c].second;
+                fp.so_q[c] = (uint32_t)byd[c].first;
+                fp.so_l[c] = (uint32_t)byd[c].second;
             }
-            for (size_t e = 0; e < epos.size(); ++e) fp.eout[e] = (uint8_t)dest[epos[e]];
+            for (size_t e = 0; e < epos.size(); ++e) fp.eout[e] = (uint32_t)dest[epos[e]];
         }
 
         // ---- emit stages ----
@@ -327,18 +329,27 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
             for (int l : rest)
                 if (std::find(tm.begin(), tm.end(), l) == tm.end()) tm.push_back(l);
             FStage fs{};
-            for (int p = 0; p < FUSE_REG; ++p) fs.rmap[p] = (uint8_t)st.R[p];
-            for (size_t t = 0; t < tm.size(); ++t) fs.tmap[t] = (uint8_t)tm[t];
+            for (int p = 0; p < FUSE_REG; ++p) fs.rmap[p] = (uint32_t)st.R[p];
+            for (size_t t = 0; t < tm.size(); ++t) fs.tmap[t] = (uint32_t)tm[t];
             fs.op_begin = (uint32_t)P.ops.size();
             for (SOp& so : st.ops) {
                 if (!so.mono) {
                     FOp f{};
                     f.type = so.dtype;
-                    f.p = (uint8_t)pos[so.lt];
+                    if (so.lctl != 0 || so.ectl != 0) {
+                        // controlled: the general formula (controls evaluated per pair/thread)
+                        f.type = (so.dtype == FOP_XSWAP) ? FOP_XSWAP : FOP_DENSE;
+                    } else if (so.dtype == FOP_DENSE) {
+                        auto im0 = [](cd z) { return z.imag() == 0.0; };
+                        auto re0 = [](cd z) { return z.real() == 0.0; };
+                        if (im0(so.m[0]) && im0(so.m[1]) && im0(so.m[2]) && im0(so.m[3])) f.type = FOP_REAL;
+                        else if (im0(so.m[0]) && im0(so.m[3]) && re0(so.m[1]) && re0(so.m[2])) f.type = FOP_RXLIKE;
+                    }
+                    f.p = (uint32_t)pos[so.lt];
                     uint32_t rctl = 0;
                     for (int l = 0; l < k; ++l)
                         if (((so.lctl >> l) & 1) && pos[l] >= 0) rctl |= 1u << pos[l];
-                    f.rctl = (uint8_t)rctl;
+                    f.rctl = (uint32_t)rctl;
                     f.tctl = (uint32_t)(so.lctl & ~Rmask);
                     f.ectl = so.ectl;
                     for (int j = 0; j < 4; ++j) {
@@ -359,7 +370,7 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
                 for (auto& kv : byR) {
                     FOp f{};
                     f.type = FOP_MONO;
-                    f.rctl = (uint8_t)kv.first;
+                    f.rctl = (uint32_t)kv.first;
                     f.term_begin = (uint32_t)P.terms.size();
                     std::map<uint32_t, cd> cst;                         // tmask -> product
                     std::map<uint32_t, std::vector<const Mono*>> ext;  // tmask -> external monomials
@@ -373,14 +384,45 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
                             ext[tmask].push_back(m);
                         }
                     }
+                    // Constant single-thread-bit terms become a per-thread-bit factor
+                    // vector (one select + multiply per set bit of the thread id), the
+                    // tmask-0 constant goes to m[0..1]; the rest stay terms.
+                    f.mode = 1;
+                    f.alpha_begin = FUSE_NO_ALPHA;
+                    std::vector<cd> alpha(k - FUSE_REG, cd(1.0, 0.0));
+                    uint32_t amask = 0;
+                    cd c0(1.0, 0.0);
                     for (auto& c : cst) {
+                        const uint32_t tmask = c.first;
+                        if (tmask == 0) {
+                            c0 *= c.second;
+                            continue;
+                        }
+                        if (popc(tmask) == 1) {
+                            const int l = __builtin_ctz(tmask);
+                            const int t = (int)(std::find(tm.begin(), tm.end(), l) - tm.begin());
+                            alpha[t] *= c.second;
+                            amask |= 1u << t;
+                            continue;
+                        }
                         FTerm t{};
-                        t.tmask = c.first;
+                        t.tmask = tmask;
                         t.slot = -1;
                         t.emask = 0;
                         t.d[0] = c.second.real();
                         t.d[1] = c.second.imag();
+                        t.ang = std::arg(c.second);
                         P.terms.push_back(t);
+                    }
+                    f.m[0] = c0.real();
+                    f.m[1] = c0.imag();
+                    f.tctl = amask;  // MONO: thread-id bits with a factor
+                    if (amask) {
+                        f.alpha_begin = (uint32_t)P.alphas.size();
+                        for (const cd& a : alpha) {
+                            P.alphas.push_back(a.real());
+                            P.alphas.push_back(a.imag());
+                        }
                     }
                     for (auto& e : ext) {
                         if (nslots < FUSE_MAX_SLOTS) {
@@ -389,8 +431,7 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
                             for (const Mono* m : e.second) {
                                 FExtTerm et;
                                 et.emask = m->emask;
-                                et.d[0] = m->d.real();
-                                et.d[1] = m->d.imag();
+                                et.ang = std::arg(m->d);
                                 P.eterms.push_back(et);
                             }
                             sl.end = (uint32_t)P.eterms.size();
@@ -410,6 +451,7 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
                                 t.emask = m->emask;
                                 t.d[0] = m->d.real();
                                 t.d[1] = m->d.imag();
+                                t.ang = std::arg(m->d);
                                 P.terms.push_back(t);
                             }
                         }

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -16,6 +17,8 @@
 #include "qse_hip.h"
 
 namespace qse {
+// The planner reports errors through this hook (libqse_hip keeps a message).
+int set_error(int status, const std::string&) { return status; }
 double op_angle(const qse_op& op, const double* params) {
     if (op.param >= 0 && params) return op.theta + op.scale * params[op.param];
     return op.theta;
@@ -28,13 +31,18 @@ using cd = std::complex<double>;
 
 static uint32_t swz(uint32_t li) { return li ^ (((li >> 4) ^ (li >> 8) ^ (li >> 12)) & 15u); }
 
-static void apply_op(const HostPlan& P, const FOp& op, cd* a, uint32_t toff, uint64_t base, const cd* slotv) {
+static void apply_op(const HostPlan& P, const FOp& op, cd* a, int tid, int TB, uint32_t toff, uint64_t base,
+                     const cd* slotv) {
     if (op.type == FOP_MONO) {
-        cd g = 1.0;
+        cd g(op.m[0], op.m[1]);
+        if (op.alpha_begin != FUSE_NO_ALPHA)
+            for (int t = 0; t < TB; ++t)
+                if (((op.tctl >> t) & 1) && ((tid >> t) & 1))
+                    g *= cd(P.alphas[op.alpha_begin + 2 * t], P.alphas[op.alpha_begin + 2 * t + 1]);
         for (uint32_t t = op.term_begin; t < op.term_end; ++t) {
             const FTerm& T = P.terms[t];
             if ((toff & T.tmask) != T.tmask) continue;
-            if (T.slot >= 0) g *= slotv[T.slot];
+            if (T.slot >= 0) g *= std::polar(1.0, slotv[T.slot].real());
             else if ((base & T.emask) == T.emask) g *= cd(T.d[0], T.d[1]);
         }
         for (int j = 0; j < 16; ++j)
@@ -48,7 +56,7 @@ static void apply_op(const HostPlan& P, const FOp& op, cd* a, uint32_t toff, uin
         const int j0 = ((jj >> p) << (p + 1)) | (jj & ((1 << p) - 1)), j1 = j0 | (1 << p);
         if (((uint32_t)j0 & op.rctl) != op.rctl) continue;
         const cd v0 = a[j0], v1 = a[j1];
-        if (op.type == FOP_H) {
+        if (op.type == FOP_H && op.rctl == 0) {
             const double h = 0.7071067811865475244;
             a[j0] = h * (v0 + v1);
             a[j1] = h * (v0 - v1);
@@ -66,6 +74,7 @@ extern "C" int emu_run_fused(int n, int k, int L, const qse_op* ops, int nops, c
                              int* npasses, int relabel) {
     HostPlan P;
     int rc = build_plan(n, k, L, ops, nops, params, P, relabel != 0);
+    if (rc && relabel) rc = build_plan(n, k, L, ops, nops, params, P, false);
     if (rc) return rc;
     *npasses = (int)P.passes.size();
     const uint64_t N = 1ULL << n;
@@ -87,10 +96,10 @@ extern "C" int emu_run_fused(int n, int k, int L, const qse_op* ops, int nops, c
             }
             for (int sl = 0; sl < fp.nslots; ++sl) {
                 const FSlot& S = P.slots[fp.slot_begin + sl];
-                cd g = 1.0;
+                double a = 0.0;
                 for (uint32_t e = S.begin; e < S.end; ++e)
-                    if ((base & P.eterms[e].emask) == P.eterms[e].emask) g *= cd(P.eterms[e].d[0], P.eterms[e].d[1]);
-                slotv[sl] = g;
+                    if ((base & P.eterms[e].emask) == P.eterms[e].emask) a += P.eterms[e].ang;
+                slotv[sl] = cd(a, 0.0);  // the slot's angle, carried in the real part
             }
             for (int tid = 0; tid < NT; ++tid)
                 for (int i = 0; i < 16; ++i) {
@@ -116,7 +125,7 @@ extern "C" int emu_run_fused(int n, int k, int L, const qse_op* ops, int nops, c
                         a[j] = tile[li[j]];
                     }
                     for (uint32_t oi = S.op_begin; oi < S.op_end; ++oi)
-                        apply_op(P, P.ops[oi], a, toff, base, slotv.data());
+                        apply_op(P, P.ops[oi], a, tid, TB, toff, base, slotv.data());
                     for (int j = 0; j < 16; ++j) tile[li[j]] = a[j];
                 }
             }
@@ -143,7 +152,9 @@ extern "C" int emu_run_fused(int n, int k, int L, const qse_op* ops, int nops, c
 // permutation is a permutation, passes respect the tile size.
 extern "C" int emu_check_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, int relabel) {
     HostPlan P;
-    if (build_plan(n, k, L, ops, nops, params, P, relabel != 0)) return -1;
+    int rc = build_plan(n, k, L, ops, nops, params, P, relabel != 0);
+    if (rc && relabel) rc = build_plan(n, k, L, ops, nops, params, P, false);
+    if (rc) return -1;
     for (const FPass& fp : P.passes) {
         if (fp.k != k || fp.next != n - k) return 1;
         uint64_t seen = 0, dq = 0, sq = 0;

@@ -44,7 +44,7 @@ def run(n, k, L, ops, psi, params=None, relabel=None):
     return outs[-1]
 
 
-SHAPES = [(10, 8, 4), (11, 8, 0), (12, 9, 3), (13, 10, 2), (12, 8, 1)]
+SHAPES = [(10, 8, 4), (11, 8, 0), (12, 9, 3), (13, 10, 2), (12, 8, 1), (14, 9, 5)]  # last: 2L > k
 
 
 @pytest.mark.parametrize("n,k,L", SHAPES)
