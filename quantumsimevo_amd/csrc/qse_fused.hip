@@ -30,7 +30,10 @@ void fusion_shape(const qse_ctx* ctx, int n, int prec, int* k_out, int* L_out);
 int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P, bool relabel);
 int run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params);
 
-__device__ __forceinline__ uint32_t swz(uint32_t li) { return li ^ (((li >> 4) ^ (li >> 8) ^ (li >> 12)) & 15u); }
+// The swizzle is linear over GF(2): swz(a ^ b) = swz(a) ^ swz(b), so the
+// address of (thread part | register/iteration part) is one XOR of
+// precomputed halves.
+__host__ __device__ constexpr uint32_t swz(uint32_t li) { return li ^ (((li >> 4) ^ (li >> 8) ^ (li >> 12)) & 15u); }
 
 // One 2x2 op on register position P over the 8 register pairs (j0, j0|2^P).
 // Pairs whose register controls fail get the identity (a wave-uniform select
@@ -232,8 +235,9 @@ __global__ __launch_bounds__(1 << (K - 4)) __attribute__((amdgpu_waves_per_eu(2,
     for (int c = 0; c < 4; ++c) {
         gw[c] = 1ULL << P.tq[TB + c];
         ow[c] = 1ULL << P.so_q[TB + c];
-        sw[c] = 1u << P.so_l[TB + c];
+        sw[c] = swz(1u << P.so_l[TB + c]);
     }
+    const uint32_t swz_tid = swz((uint32_t)tid), swz_st = swz(st_t);
     const int next = P.next, nstages = P.nstages, nslots = P.nslots;
     const uint32_t stage_begin = P.stage_begin;
     for (uint64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
@@ -260,7 +264,7 @@ __global__ __launch_bounds__(1 << (K - 4)) __attribute__((amdgpu_waves_per_eu(2,
         // per-tile external factors (precomputed by k_slot_table)
         for (int sl = tid; sl < nslots; sl += NT) slotv[sl] = slot_table[ti * (uint64_t)nslots + sl];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) tile[swz((uint32_t)(i * NT + tid))] = ld[i];
+        for (int i = 0; i < 16; ++i) tile[swz_tid ^ swz((uint32_t)(i * NT))] = ld[i];
         __syncthreads();
         for (int sg = 0; sg < nstages; ++sg) {
             const FStage& S = A.stages[stage_begin + sg];
@@ -268,12 +272,14 @@ __global__ __launch_bounds__(1 << (K - 4)) __attribute__((amdgpu_waves_per_eu(2,
 #pragma unroll
             for (int t = 0; t < TB; ++t)
                 if ((tid >> t) & 1) toff |= 1u << S.tmap[t];
-            const uint32_t r0 = 1u << S.rmap[0], r1 = 1u << S.rmap[1], r2 = 1u << S.rmap[2], r3 = 1u << S.rmap[3];
+            const uint32_t r0 = swz(1u << S.rmap[0]), r1 = swz(1u << S.rmap[1]), r2 = swz(1u << S.rmap[2]),
+                           r3 = swz(1u << S.rmap[3]);  // wave-uniform
+            const uint32_t st_off = swz(toff);
             cx<R> a[16];
 #pragma unroll
             for (int j = 0; j < 16; ++j)
-                a[j] = tile[swz(toff | ((j & 1) ? r0 : 0u) | ((j & 2) ? r1 : 0u) | ((j & 4) ? r2 : 0u) |
-                                ((j & 8) ? r3 : 0u))];
+                a[j] = tile[st_off ^ ((j & 1) ? r0 : 0u) ^ ((j & 2) ? r1 : 0u) ^ ((j & 4) ? r2 : 0u) ^
+                            ((j & 8) ? r3 : 0u)];
             const uint32_t oe = S.op_end;
             for (uint32_t oi = S.op_begin; oi < oe; ++oi) {
                 const FOp& op = A.ops[oi];
@@ -329,7 +335,7 @@ __global__ __launch_bounds__(1 << (K - 4)) __attribute__((amdgpu_waves_per_eu(2,
             }
 #pragma unroll
             for (int j = 0; j < 16; ++j)
-                tile[swz(toff | ((j & 1) ? r0 : 0u) | ((j & 2) ? r1 : 0u) | ((j & 4) ? r2 : 0u) | ((j & 8) ? r3 : 0u))] =
+                tile[st_off ^ ((j & 1) ? r0 : 0u) ^ ((j & 2) ? r1 : 0u) ^ ((j & 4) ? r2 : 0u) ^ ((j & 8) ? r3 : 0u)] =
                     a[j];
             __syncthreads();
         }
@@ -337,12 +343,12 @@ __global__ __launch_bounds__(1 << (K - 4)) __attribute__((amdgpu_waves_per_eu(2,
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             uint64_t g = obase | ot;
-            uint32_t ls = st_t;
-            if (i & 1) { g |= ow[0]; ls |= sw[0]; }
-            if (i & 2) { g |= ow[1]; ls |= sw[1]; }
-            if (i & 4) { g |= ow[2]; ls |= sw[2]; }
-            if (i & 8) { g |= ow[3]; ls |= sw[3]; }
-            dst[g] = tile[swz(ls)];
+            uint32_t ls = swz_st;
+            if (i & 1) { g |= ow[0]; ls ^= sw[0]; }
+            if (i & 2) { g |= ow[1]; ls ^= sw[1]; }
+            if (i & 4) { g |= ow[2]; ls ^= sw[2]; }
+            if (i & 8) { g |= ow[3]; ls ^= sw[3]; }
+            dst[g] = tile[ls];
         }
         __syncthreads();
     }

@@ -67,12 +67,44 @@ struct SStage {
     std::vector<SOp> ops;
 };
 
+}  // namespace
+
+// CNOT(c,t) RZ(theta, t) CNOT(c,t) with c != t is exp(-i theta Z_c Z_t / 2): a
+// diagonal (ZZ) phase, the cost-layer pattern of QAOA (QAOA.cpp:128-133).
+bool is_zz_triple(const qse_op* ops, int nops, int k) {
+    if (k + 2 >= nops) return false;
+    const qse_op &a = ops[k], &b = ops[k + 1], &c = ops[k + 2];
+    return a.kind == QSE_GATE_CNOT && c.kind == QSE_GATE_CNOT && b.kind == QSE_GATE_RZ && a.control == c.control &&
+           a.target == c.target && a.control != a.target && b.target == a.target;
+}
+
+namespace {
+
 void to_items(const qse_op* ops, int nops, const double* params, std::vector<Item>& out) {
     const double s = 0.7071067811865475244;
     for (int k = 0; k < nops; ++k) {
         const qse_op& o = ops[k];
         const double th = op_angle(o, params);
         Item it;
+        if (is_zz_triple(ops, nops, k)) {
+            // phase exp(-i th/2) on even parity of (c, t), exp(+i th/2) on odd:
+            // exp(-i th/2) * exp(i th x_c) * exp(i th x_t) * exp(-2 i th x_c x_t)
+            const double z = op_angle(ops[k + 1], params);
+            const uint64_t cb = 1ULL << o.control, tb2 = 1ULL << o.target;
+            auto push = [&](uint64_t mask, cd d) {
+                Item mi;
+                mi.t = IT_MONO;
+                mi.qmask = mask;
+                mi.d = d;
+                out.push_back(mi);
+            };
+            push(0, cd(std::cos(z / 2), -std::sin(z / 2)));
+            push(cb, cd(std::cos(z), std::sin(z)));
+            push(tb2, cd(std::cos(z), std::sin(z)));
+            push(cb | tb2, cd(std::cos(2 * z), -std::sin(2 * z)));
+            k += 2;
+            continue;
+        }
         auto dense = [&](uint32_t dt, cd a, cd b, cd c, cd d) {
             it.t = IT_DENSE;
             it.q = o.target;

@@ -1,0 +1,121 @@
+"""Adjoint gradients on the GPU (qse_adjoint_expect) against central finite
+differences (h = 1e-5) and against parameter-shift gradients evaluated on the
+CPU oracle. Bar: max |dg| <= 1e-6 against finite differences, <= 1e-10 against
+the oracle's parameter shift in complex128."""
+import math
+
+import numpy as np
+import pytest
+import torch  # imported at collection time (first import can be slow on a fresh box)
+
+import oracle_lib as O
+from quantumsimevo_amd import autograd as AG
+from quantumsimevo_amd import capi
+from quantumsimevo_amd import circuits as G
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_energy(n, ops, params, masks, weights):
+    psi = O.run_ops(O.basis(n), n, ops, params)
+    return O.expect_zstrings(psi, n, masks, weights)
+
+
+def oracle_shift(n, ops, params, masks, weights):
+    g = np.zeros(len(params))
+    for k in np.nonzero(ops["param"] >= 0)[0]:
+        for s in (+1.0, -1.0):
+            sh = ops.copy()
+            sh["theta"][k] += s * math.pi / 2
+            g[int(ops[k]["param"])] += s * 0.5 * float(ops[k]["scale"]) * oracle_energy(n, sh, params, masks, weights)
+    return g
+
+
+def mixed_circuit(n, seed):
+    """Every parameterisable kind (RX, RY, RZ, PHASE, CPHASE) mixed with fixed
+    H, CNOT, SWAP, MCPHASE, generic 2x2 and a CNOT-RZ-CNOT cost gate."""
+    rng = np.random.default_rng(seed)
+    g = G.GateList(n)
+    for q in range(n):
+        g.hadamard(q)
+    p = 0
+    for layer in range(3):
+        for q in range(n):
+            k = (q + layer) % 5
+            if k == 0: g.rotate_x(0.1, q, param=p, scale=1.0)
+            elif k == 1: g.rotate_y(-0.2, q, param=p, scale=2.0)
+            elif k == 2: g.rotate_z(0.3, q, param=p, scale=1.0)
+            elif k == 3: g.phase(0.0, q, param=p, scale=1.0)
+            else: g.cphase(0.4, q, (q + 1) % n, param=p, scale=1.5)
+            p += 1
+        for q in range(0, n - 1, 2):
+            g.cnot(q, q + 1)
+        g.cnot(0, n - 1); g.rotate_z(0.0, n - 1, param=p, scale=3.0); g.cnot(0, n - 1); p += 1
+        g.swap(1, n - 2)
+        g.mcphase(0.7, [0, 2], 3)
+        g.gate(np.array([[math.cos(0.3), -math.sin(0.3)], [math.sin(0.3), math.cos(0.3)]]), layer)
+    return g.ops(), p
+
+
+@pytest.fixture(scope="module")
+def qaoa_small():
+    n, p = 12, 2
+    edges = G.random_regular_graph(n, 3, seed=4)
+    ops = G.qaoa_maxcut(n, edges, p).ops()
+    masks, w = G.maxcut_observable(edges)
+    params = np.random.default_rng(1).random((3, 2 * p)) * math.pi
+    return n, ops, masks, w, params
+
+
+def test_qaoa_adjoint_vs_oracle_shift_and_fd(ctx, qaoa_small):
+    n, ops, masks, w, params = qaoa_small
+    E, Gr = ctx.adjoint_expect(n, capi.C128, ops, params, masks, w)
+    for b in range(len(params)):
+        assert abs(E[b] - oracle_energy(n, ops, params[b], masks, w)) < 1e-11
+        gs = oracle_shift(n, ops, params[b], masks, w)
+        assert np.max(np.abs(Gr[b] - gs)) < 1e-10
+    # central finite differences on the GPU, h = 1e-5
+    h = 1e-5
+    for j in range(params.shape[1]):
+        Pp, Pm = params.copy(), params.copy()
+        Pp[:, j] += h
+        Pm[:, j] -= h
+        fd = (ctx.batch_expect(n, capi.C128, ops, Pp, masks, w) - ctx.batch_expect(n, capi.C128, ops, Pm, masks, w)) / (2 * h)
+        assert np.max(np.abs(Gr[:, j] - fd)) < 1e-6
+
+
+def test_mixed_kinds_vs_oracle_shift(ctx):
+    n = 11
+    ops, npar = mixed_circuit(n, 3)
+    zm, zw = G.zsum_observable(n)
+    zm = np.concatenate([zm, [0b101, 0b110000]]).astype(np.uint64)
+    zw = np.concatenate([zw, [0.5, -0.25]])
+    params = np.random.default_rng(2).random(npar) * 2 * math.pi
+    E, Gr = ctx.adjoint_expect(n, capi.C128, ops, params, zm, zw)
+    assert abs(E[0] - oracle_energy(n, ops, params, zm, zw)) < 1e-11
+    assert np.max(np.abs(Gr[0] - oracle_shift(n, ops, params, zm, zw))) < 1e-10
+    # checkpointed reverse sweep gives the same gradient
+    E2, G2 = ctx.adjoint_expect(n, capi.C128, ops, params, zm, zw, ckpt_every=2)
+    assert abs(E2[0] - E[0]) < 1e-12 and np.max(np.abs(G2 - Gr)) < 1e-11
+
+
+def test_complex64_gradients(ctx, qaoa_small):
+    n, ops, masks, w, params = qaoa_small
+    E, Gr = ctx.adjoint_expect(n, capi.C128, ops, params, masks, w)
+    E32, G32 = ctx.adjoint_expect(n, capi.C64, ops, params, masks, w, ckpt_every=4)
+    assert np.max(np.abs(E32 - E)) < 1e-4
+    assert np.max(np.abs(G32 - Gr)) < 1e-3
+
+
+def test_torch_autograd_function(ctx, qaoa_small):
+    n, ops, masks, w, params = qaoa_small
+    eng = AG.AdjointEngine(ctx, n, ops, masks, w)
+    x = torch.tensor(params, dtype=torch.float64, requires_grad=True)
+    E = AG.expectation(x, eng)
+    (E * torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64)).sum().backward()
+    _, Gr = ctx.adjoint_expect(n, capi.C128, ops, params, masks, w)
+    want = Gr * np.array([1.0, -2.0, 0.5])[:, None]
+    assert np.max(np.abs(x.grad.numpy() - want)) < 1e-12
+    # parameter shift on the GPU agrees with the adjoint
+    gs = eng.parameter_shift(params[:1])
+    assert np.max(np.abs(gs - Gr[:1])) < 1e-10
