@@ -43,6 +43,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(n: int, precision: str, fused: bool):
+    """HBM bytes per launch of the fused pass as measured by rocprofv3 --pmc
+    FETCH_SIZE/WRITE_SIZE (calibrated on a copy kernel) for this workload,
+    from the committed summary; None for workloads without one."""
+    f = ROOT / "profiles" / f"k_fused_qft{n}_{precision}_traffic.md"
+    if not fused or not f.exists():
+        return None, None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1]), str(f.relative_to(ROOT))
+    return None, None
+
+
 def cpu_baseline(n: int, ops: np.ndarray, prefix: int) -> dict:
     """Reference AVX2/OpenMP backend (kind "reference") or the scalar oracle
     (kind "port") on the first `prefix` gates of the workload's gate list."""
@@ -73,6 +86,40 @@ def cpu_baseline(n: int, ops: np.ndarray, prefix: int) -> dict:
                       f"{secs:.2f} s) on {what}"}
 
 
+def qaoa_leg(ctx, batch: int, shift_sets: int) -> dict:
+    """QAOA Max-Cut, 24 qubits, seeded random 3-regular graph (36 edges), p = 8
+    (16 parameters, uniform in [0, pi)), cost sum_edges (1 - Z_i Z_j)/2 in FP64:
+    batched adjoint gradients vs plain per-parameter-shift on the same GPU."""
+    n, p = 24, 8
+    edges = C.random_regular_graph(n, 3, seed=2024)
+    ops = C.qaoa_maxcut(n, edges, p).ops()
+    masks, w = C.maxcut_observable(edges)
+    rng = np.random.default_rng(7)
+    params = rng.random((batch, 2 * p)) * np.pi
+    ctx.adjoint_expect(n, capi.C128, ops, params[:1], masks, w)  # warm-up (program cache)
+    ctx.sync()
+    t0 = time.perf_counter()
+    E, G = ctx.adjoint_expect(n, capi.C128, ops, params, masks, w)
+    t_adj = time.perf_counter() - t0
+    log(f"QAOA-24 adjoint: batch {batch} in {t_adj:.3f} s")
+    occ = np.nonzero(ops["param"] >= 0)[0]
+    t0 = time.perf_counter()
+    Gs = np.zeros((shift_sets, 2 * p))
+    for k in occ:
+        for sgn in (+1.0, -1.0):
+            sh = ops.copy()
+            sh["theta"][k] += sgn * np.pi / 2
+            Es = ctx.batch_expect(n, capi.C128, sh, params[:shift_sets], masks, w)
+            Gs[:, int(ops[k]["param"])] += sgn * 0.5 * float(ops[k]["scale"]) * Es
+    t_shift = time.perf_counter() - t0
+    log(f"QAOA-24 parameter shift: {shift_sets} set(s), {2 * len(occ)} evaluations each, {t_shift:.2f} s")
+    err = float(np.max(np.abs(G[:shift_sets] - Gs)))
+    return {"workload": "QAOA Max-Cut 24q, 3-regular graph (36 edges), p=8, 16 params, complex128, FP64 cost",
+            "batch": batch, "adjoint_grads_per_s": batch / t_adj, "shift_grads_per_s": shift_sets / t_shift,
+            "speedup": (batch / t_adj) / (shift_sets / t_shift), "max_abs_grad_diff_vs_shift": err,
+            "shift_evaluations_per_grad": int(2 * len(occ)), "mean_energy": float(np.mean(E))}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +131,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-prefix", type=int, default=32)
     ap.add_argument("--no-qaoa", action="store_true")
+    ap.add_argument("--qaoa-batch", type=int, default=64)
+    ap.add_argument("--qaoa-shift-sets", type=int, default=1)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run X on qubit 0 (one read + one write of every amplitude) for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -93,6 +144,14 @@ def main() -> int:
     n = args.qubits
     prec = capi.C128 if args.precision == "c128" else capi.C64
     b = 16 if prec == capi.C128 else 8
+    if args.calibrate_copy:
+        ctx = capi.Context(0)
+        st = ctx.state(n, prec)
+        for _ in range(args.steps):
+            st.x(0)
+        ctx.sync()
+        log(f"copy calibration: {args.steps} x k_pairs X(0) over {n} qubits, {2 * b * 2**n} bytes each")
+        return 0
     ops = C.qft(n).ops()
     upd_step = C.amp_updates(ops, n)
 
@@ -132,6 +191,7 @@ def main() -> int:
     log(f"dominant kernel {kname}: {dom['launches']} launches, mean {mean_ms:.4f} ms, "
         f"{bytes_per_launch / 1e9:.3f} GB/launch -> {achieved / 1e9:.1f} GB/s")
 
+    traffic, traffic_src = profiled_traffic(n, args.precision, fused and dom_f == capi.FAM_FUSED)
     # size-independent sanity: the norm stays 1 after K unitary steps
     nrm = st.norm2()
     if abs(nrm - 1.0) > (1e-9 if prec == capi.C128 else 1e-3):
@@ -154,12 +214,14 @@ def main() -> int:
         "config": {"workload": f"QFT-{n} {args.precision} (QFT.cpp:9-22), {len(ops)} gates, state resident in HBM",
                    "qubits": n, "precision": args.precision, "fused": fused, "passes_per_step": passes},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": bytes_per_launch, "mean_launch_ms": mean_ms},
         "pct_hbm_step": (bytes_per_launch * dom["launches"] / args.steps) / (ms_step * 1e-3) / HBM_PEAK,
     }
     st.close()
 
+    if not args.no_qaoa:
+        result["qaoa"] = qaoa_leg(ctx, args.qaoa_batch, args.qaoa_shift_sets)
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(n, ops, args.cpu_prefix)

@@ -179,7 +179,7 @@ void fusion_shape(const qse_ctx* ctx, int n, int prec, int* k_out, int* L_out) {
     int k = ctx ? ctx->tile_bits[prec] : 0;
     int L = ctx ? ctx->low_bits[prec] : 0;
     if (k <= 0) k = (prec == QSE_C128) ? 12 : 13;
-    if (L <= 0) L = (prec == QSE_C128) ? 4 : 5;
+    if (L <= 0) L = (prec == QSE_C128) ? 6 : 6;
     k = std::min(k, FUSE_MAX_K);
     k = std::min(k, n - 2);
     k = std::max(k, FUSE_MIN_K);

@@ -183,3 +183,35 @@ extern "C" int emu_check_plan(int n, int k, int L, const qse_op* ops, int nops, 
     }
     return 0;
 }
+
+// Plan summary for diagnostics: per pass, stages, dense ops, MONO ops, terms,
+// alpha vectors and slots.
+extern "C" int emu_plan_stats(int n, int k, int L, const qse_op* ops, int nops, const double* params, int* out,
+                              int max_passes) {
+    HostPlan P;
+    int rc = build_plan(n, k, L, ops, nops, params, P, true);
+    if (rc) rc = build_plan(n, k, L, ops, nops, params, P, false);
+    if (rc) return -1;
+    int np = 0;
+    for (const FPass& fp : P.passes) {
+        if (np >= max_passes) break;
+        int dense = 0, mono = 0, terms = 0, alpha = 0;
+        for (int sg = 0; sg < fp.nstages; ++sg) {
+            const FStage& S = P.stages[fp.stage_begin + sg];
+            for (uint32_t oi = S.op_begin; oi < S.op_end; ++oi) {
+                const FOp& op = P.ops[oi];
+                if (op.type == FOP_MONO) {
+                    ++mono;
+                    terms += (int)(op.term_end - op.term_begin);
+                    alpha += __builtin_popcount(op.tctl);
+                } else {
+                    ++dense;
+                }
+            }
+        }
+        int* o = out + 6 * np;
+        o[0] = fp.nstages; o[1] = dense; o[2] = mono; o[3] = terms; o[4] = alpha; o[5] = fp.nslots;
+        ++np;
+    }
+    return np;
+}
