@@ -128,6 +128,8 @@ def main() -> int:
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--precision", choices=["c128", "c64"], default="c128")
     ap.add_argument("--unfused", action="store_true", help="one sweep per gate instead of fused passes")
+    ap.add_argument("--jit", choices=["auto", "off", "always"], default="auto",
+                    help="run-time specialised passes: compiled on the program's second run (auto), never, or at once")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-prefix", type=int, default=32)
     ap.add_argument("--no-qaoa", action="store_true")
@@ -156,6 +158,7 @@ def main() -> int:
     upd_step = C.amp_updates(ops, n)
 
     ctx = capi.Context(0)
+    ctx.set_jit({"auto": capi.JIT_AUTO, "off": capi.JIT_OFF, "always": capi.JIT_ALWAYS}[args.jit])
     info = capi.device_info(0)
     log(f"device: {info['name']} CUs={info['compute_units']} HBM={info['total_bytes'] / 2**30:.1f} GiB")
     st = ctx.state(n, prec)
@@ -172,12 +175,18 @@ def main() -> int:
     ctx.reset_timing()
     ctx.enable_timing(True)
     ctx.sync()
+    jit0 = ctx.jit_stats()
     t0 = time.perf_counter()
     for i in range(args.steps):
         st.run(ops, fused=fused)
     ctx.sync()
     t1 = time.perf_counter()
     ctx.enable_timing(False)
+    jit1 = ctx.jit_stats()
+    jit_timed = jit1["launches"] > jit0["launches"]
+    if jit1["compiled"] != jit0["compiled"]:
+        log("a program was compiled inside the timed region: raise --warmup")
+        return 3
     ms_step = (t1 - t0) * 1e3 / args.steps
     log(f"timed {args.steps} steps: {ms_step:.3f} ms/step")
 
@@ -187,7 +196,9 @@ def main() -> int:
     mean_ms = dom["ms"] / max(dom["launches"], 1)
     bytes_per_launch = dom["bytes"] / max(dom["launches"], 1)
     achieved = bytes_per_launch / (mean_ms * 1e-3) if mean_ms > 0 else 0.0
-    kname = {capi.FAM_SWEEP: "k_pairs (one-gate sweep)", capi.FAM_FUSED: "k_fused (LDS-tiled pass)"}[dom_f]
+    kname = {capi.FAM_SWEEP: "k_pairs (one-gate sweep)",
+             capi.FAM_FUSED: "qse_jit_pass* (LDS-tiled pass, run-time specialised)" if jit_timed
+             else "k_fused (LDS-tiled pass)"}[dom_f]
     log(f"dominant kernel {kname}: {dom['launches']} launches, mean {mean_ms:.4f} ms, "
         f"{bytes_per_launch / 1e9:.3f} GB/launch -> {achieved / 1e9:.1f} GB/s")
 
@@ -212,7 +223,8 @@ def main() -> int:
         "dtype": "f64" if prec == capi.C128 else "f32",
         "data": "synthetic: seeded random normalised state generated on the device",
         "config": {"workload": f"QFT-{n} {args.precision} (QFT.cpp:9-22), {len(ops)} gates, state resident in HBM",
-                   "qubits": n, "precision": args.precision, "fused": fused, "passes_per_step": passes},
+                   "qubits": n, "precision": args.precision, "fused": fused, "passes_per_step": passes,
+                   "jit": args.jit, "jit_timed": jit_timed, "jit_compile_ms": jit1["compile_ms"]},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": bytes_per_launch, "mean_launch_ms": mean_ms},

@@ -120,6 +120,19 @@ int qse_ctx_sync(qse_ctx* ctx);
  * 2^low_bits amplitudes). 0 selects the tuned default for the precision. */
 int qse_ctx_set_fusion(qse_ctx* ctx, int precision, int tile_bits, int low_bits);
 int qse_ctx_get_fusion(qse_ctx* ctx, int precision, int* tile_bits, int* low_bits);
+/* Run-time specialisation of fused passes (no counterpart in the reference,
+ * whose GPU backend launches one fixed kernel per gate,
+ * CircuitUnitaryOperationGPU.cu:194-330). A fused program is compiled with
+ * hiprtc into straight-line kernels with its coefficients as literals:
+ * QSE_JIT_AUTO (default) when the same program (gates and angles) runs a
+ * second time, QSE_JIT_ALWAYS on first use, QSE_JIT_OFF never (the generic
+ * program-interpreting kernel). Results agree with the generic kernel to
+ * rounding. */
+enum qse_jit_mode { QSE_JIT_OFF = 0, QSE_JIT_AUTO = 1, QSE_JIT_ALWAYS = 2 };
+int qse_ctx_set_jit(qse_ctx* ctx, int mode);
+/* Programs compiled, passes launched through compiled kernels, and total
+ * compile time in milliseconds since the context was created. */
+int qse_ctx_jit_stats(qse_ctx* ctx, int64_t* compiled, int64_t* jit_launches, double* compile_ms);
 /* Kernel timing: when enabled, every kernel launched by the context records HIP
  * events; qse_ctx_timing() returns the count and summed milliseconds per kernel
  * family. Families: 0 pair sweep, 1 fused tile pass, 2 readout, 3 adjoint. */

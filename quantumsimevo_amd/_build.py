@@ -33,7 +33,7 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 LIB_SOURCES = ["qse_capi.cpp", "qse_run.cpp", "qse_gates.hip", "qse_readout.hip", "qse_fused.hip",
-               "qse_plan.cpp", "qse_adjoint.hip"]
+               "qse_plan.cpp", "qse_adjoint.hip", "qse_jit.cpp", "qse_jit_gen.cpp"]
 HOST_SOURCES = ["StateVector.cpp", "Noise.cpp", "CircuitUnitaryOperationHIP.cpp",
                 "CircuitUnitaryOperationFactory.cpp", "Circuit.cpp", "HardwareCheck.cpp"]
 
@@ -67,7 +67,8 @@ def build_lib(verbose: bool = False, jobs: int = 8) -> Path:
         list(ex.map(compile_one, range(len(srcs))))
     out = lib_path()
     tmp = out.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
+          "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"], verbose)
     os.replace(tmp, out)
     return out
 
@@ -109,7 +110,7 @@ def build_test_emulator(verbose: bool = False) -> Path:
     out = out_dir / "libfused_emu.so"
     _run([os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__",
           "-I/opt/rocm/include", f"-I{INCLUDE}", f"-I{CSRC}", "-o", str(out),
-          str(ROOT / "tests" / "fused_emu.cpp"), str(CSRC / "qse_plan.cpp")], verbose)
+          str(ROOT / "tests" / "fused_emu.cpp"), str(CSRC / "qse_plan.cpp"), str(CSRC / "qse_jit_gen.cpp")], verbose)
     return out
 
 

@@ -138,6 +138,35 @@ def random_layered(n: int, layers: int, seed: int) -> GateList:
     return g
 
 
+def random_mixed(n: int, count: int, seed: int) -> GateList:
+    """`count` gates drawn uniformly from every kind the gate list supports (H,
+    X, Y, Z, phase, RX, RY, RZ, CNOT, CPhase, SWAP, MCPhase with up to three
+    controls, and the generic 2x2 [[cos t, -i sin t], [0.3i, 0.7]], which is not
+    unitary) on uniform random qubits; control == target draws are kept (they
+    exercise the degenerate cases the reference allows)."""
+    rng = np.random.default_rng(seed)
+    g = GateList(n)
+    for _ in range(count):
+        kind = int(rng.integers(0, 13))
+        q = int(rng.integers(0, n))
+        c = int(rng.integers(0, n))
+        th = float(rng.random() * 6.3)
+        if kind == 0: g.hadamard(q)
+        elif kind == 1: g.pauli_x(q)
+        elif kind == 2: g.pauli_y(q)
+        elif kind == 3: g.pauli_z(q)
+        elif kind == 4: g.phase(th, q)
+        elif kind == 5: g.rotate_x(th, q)
+        elif kind == 6: g.rotate_y(th, q)
+        elif kind == 7: g.rotate_z(th, q)
+        elif kind == 8: g.cnot(c, q)
+        elif kind == 9: g.cphase(th, c, q)
+        elif kind == 10: g.swap(q, c)
+        elif kind == 11: g.mcphase(th, sorted(set(int(x) for x in rng.integers(0, n, 3))), q)
+        else: g.gate(np.array([[math.cos(th), -1j * math.sin(th)], [0.3j, 0.7]]), q)
+    return g
+
+
 def random_state(n: int, seed: int) -> np.ndarray:
     """Complex Gaussian amplitudes, normalised in FP64."""
     rng = np.random.default_rng(seed)
@@ -148,6 +177,8 @@ def random_state(n: int, seed: int) -> np.ndarray:
 
 def random_regular_graph(n: int, degree: int, seed: int) -> list[tuple[int, int]]:
     """Random d-regular simple graph by the configuration model with rejection."""
+    if (n * degree) % 2 or degree >= n:
+        raise ValueError(f"no simple {degree}-regular graph on {n} vertices")
     rng = np.random.default_rng(seed)
     while True:
         stubs = np.repeat(np.arange(n), degree)

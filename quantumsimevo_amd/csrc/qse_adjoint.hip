@@ -32,7 +32,7 @@
 namespace qse {
 
 int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int* passes_out,
-              bool dry_run);
+              bool dry_run, bool allow_jit);
 int validate_ops(int n, const qse_op* ops, int nops, const double* params);
 double op_angle(const qse_op& op, const double* params);
 bool is_zz_triple(const qse_op* ops, int nops, int k);
@@ -320,7 +320,7 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
                 const AdjItem& it = items[blocks[b][ii]];
                 for (int k = 0; k < it.count; ++k) seg.push_back(ops[it.first + k]);
             }
-            rc = run_fused(ctx, ws.psi, seg.data(), (int)seg.size(), params, nullptr, false);
+            rc = run_fused(ctx, ws.psi, seg.data(), (int)seg.size(), params, nullptr, false, false);
             if (rc) return rc;
             // checkpoint the state after this block every ckpt_every blocks
             if (b > 0 && ((nb - b) % ckpt_every) == 0 && slot < (int)ws.ckpt.size()) {
@@ -330,7 +330,7 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
             }
         }
     } else {
-        rc = run_fused(ctx, ws.psi, ops, nops, params, nullptr, false);
+        rc = run_fused(ctx, ws.psi, ops, nops, params, nullptr, false, false);
         if (rc) return rc;
     }
     rc = reduce(ctx, ws.psi, 2, 0, obs_m_h, obs_w_h, nterms, energy);
@@ -383,10 +383,10 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
         }
         if (b + 1 < (int)blocks.size()) {
             if (ckpt_at.find(b + 1) == ckpt_at.end()) {
-                rc = run_fused(ctx, ws.psi, dag.data(), (int)dag.size(), nullptr, nullptr, false);
+                rc = run_fused(ctx, ws.psi, dag.data(), (int)dag.size(), nullptr, nullptr, false, false);
                 if (rc) return rc;
             }
-            rc = run_fused(ctx, ws.lam, dag.data(), (int)dag.size(), nullptr, nullptr, false);
+            rc = run_fused(ctx, ws.lam, dag.data(), (int)dag.size(), nullptr, nullptr, false, false);
             if (rc) return rc;
         }
     }
@@ -475,7 +475,7 @@ int qse_batch_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops
     for (int b = 0; b < batch && rc == QSE_OK; ++b) {
         const double* pb = params ? params + (size_t)b * nparams : nullptr;
         rc = qse_state_set_basis(ctx, st, 0);
-        if (!rc) rc = run_fused(ctx, st, ops, nops, pb, nullptr, false);
+        if (!rc) rc = run_fused(ctx, st, ops, nops, pb, nullptr, false, false);
         if (!rc) rc = reduce(ctx, st, 2, 0, obs_masks, obs_weights, nterms, &energies[b]);
     }
     qse_state_destroy(ctx, st);

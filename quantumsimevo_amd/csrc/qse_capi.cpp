@@ -224,7 +224,10 @@ int qse_ctx_destroy(qse_ctx* ctx) {
     for (auto e : ctx->event_pool) hipEventDestroy(e);
     for (auto& s : ctx->scratch)
         if (s.ptr) hipFree(s.ptr);
-    for (auto& e : ctx->prog_cache) hipFree(e.dev);
+    for (auto& e : ctx->prog_cache) {
+        if (e.mod) hipModuleUnload(e.mod);
+        hipFree(e.dev);
+    }
     if (ctx->pinned) hipHostFree(ctx->pinned);
     hipStreamDestroy(ctx->stream);
     delete ctx;

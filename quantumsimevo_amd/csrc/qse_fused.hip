@@ -17,7 +17,9 @@
 // LDS is XOR-swizzled in 16-byte units (position ^ fold of bits 4.. into bits
 // 0..3) so that both the contiguous load/store and the strided stage gathers
 // spread a wavefront's accesses over the banks.
+#include <chrono>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "qse_device.hpp"
@@ -29,6 +31,9 @@ namespace qse {
 void fusion_shape(const qse_ctx* ctx, int n, int prec, int* k_out, int* L_out);
 int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P, bool relabel);
 int run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params);
+std::string jit_source(const HostPlan& P, int prec);
+int jit_compile(const std::string& src, int npasses, hipModule_t* mod, std::vector<hipFunction_t>& fns);
+int jit_launch(qse_ctx* ctx, hipFunction_t fn, const void* src, void* dst, const void* table, uint64_t ntiles, int K);
 
 // The swizzle is linear over GF(2): swz(a ^ b) = swz(a) ^ swz(b), so the
 // address of (thread part | register/iteration part) is one XOR of
@@ -383,11 +388,11 @@ void serialize(const HostPlan& P, Serialized& S) {
 }  // namespace
 
 // Device copy of a serialized program, reused when the same program runs again.
-int program_on_device(qse_ctx* ctx, const Serialized& S, char** dev) {
+int program_on_device(qse_ctx* ctx, const Serialized& S, ProgEntry** entry) {
     for (auto& e : ctx->prog_cache) {
         if (e.bytes.size() == S.bytes.size() && std::memcmp(e.bytes.data(), S.bytes.data(), S.bytes.size()) == 0) {
             e.stamp = ++ctx->prog_clock;
-            *dev = (char*)e.dev;
+            *entry = &e;
             return QSE_OK;
         }
     }
@@ -396,6 +401,7 @@ int program_on_device(qse_ctx* ctx, const Serialized& S, char** dev) {
         for (size_t i = 1; i < ctx->prog_cache.size(); ++i)
             if (ctx->prog_cache[i].stamp < ctx->prog_cache[oldest].stamp) oldest = i;
         QSE_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->prog_cache[oldest].mod) QSE_HIP(hipModuleUnload(ctx->prog_cache[oldest].mod));
         QSE_HIP(hipFree(ctx->prog_cache[oldest].dev));
         ctx->prog_cache.erase(ctx->prog_cache.begin() + oldest);
     }
@@ -406,7 +412,32 @@ int program_on_device(qse_ctx* ctx, const Serialized& S, char** dev) {
     QSE_HIP(hipMemcpyAsync(e.dev, S.bytes.data(), S.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
     e.stamp = ++ctx->prog_clock;
     ctx->prog_cache.push_back(std::move(e));
-    *dev = (char*)ctx->prog_cache.back().dev;
+    *entry = &ctx->prog_cache.back();
+    return QSE_OK;
+}
+
+// Compiled kernels for the program of `e` when the context's policy asks for
+// them (QSE_JIT_ALWAYS, or QSE_JIT_AUTO from the program's second run on).
+// Under QSE_JIT_AUTO a failed compilation is remembered and the generic kernel
+// runs; under QSE_JIT_ALWAYS it is the caller's error.
+int jit_for(qse_ctx* ctx, ProgEntry& e, const HostPlan& P, int prec, bool* use) {
+    *use = false;
+    ++e.runs;
+    if (ctx->jit_mode == QSE_JIT_OFF || e.jit_failed) return QSE_OK;
+    if (ctx->jit_mode == QSE_JIT_AUTO && e.runs < 2) return QSE_OK;
+    if (!e.mod) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const int rc = jit_compile(jit_source(P, prec), (int)P.passes.size(), &e.mod, e.fns);
+        ctx->jit_compile_ms +=
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (rc) {
+            if (ctx->jit_mode == QSE_JIT_ALWAYS) return rc;
+            e.jit_failed = true;
+            return QSE_OK;
+        }
+        ++ctx->jit_compiled;
+    }
+    *use = true;
     return QSE_OK;
 }
 
@@ -460,7 +491,7 @@ int choose_plan(const qse_ctx* ctx, int n, int k, int L, const qse_op* ops, int 
 }
 
 int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int* passes_out,
-              bool dry_run) {
+              bool dry_run, bool allow_jit) {
     if (st->n < FUSE_MIN_QUBITS) {
         if (passes_out) *passes_out = nops;
         return dry_run ? QSE_OK : run_sweeps(ctx, st, ops, nops, params);
@@ -485,9 +516,13 @@ int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const do
     }
     Serialized S;
     serialize(P, S);
-    char* dev = nullptr;
-    rc = program_on_device(ctx, S, &dev);
+    ProgEntry* entry = nullptr;
+    rc = program_on_device(ctx, S, &entry);
     if (rc) return rc;
+    bool jit = false;
+    if (allow_jit) rc = jit_for(ctx, *entry, P, st->prec, &jit);
+    if (rc) return rc;
+    char* dev = (char*)entry->dev;
     FProgram prog;
     prog.passes = (const FPass*)(dev + S.off[0]);
     prog.stages = (const FStage*)(dev + S.off[1]);
@@ -517,8 +552,13 @@ int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const do
         TimedLaunch t;
         launch_begin(ctx, FAM_FUSED, &t);
         const bool oop = P.passes[p].out_of_place != 0;
-        rc = (st->prec == QSE_C128) ? launch_pass<double>(ctx, st, k, prog, table, p, oop)
-                                    : launch_pass<float>(ctx, st, k, prog, table, p, oop);
+        if (jit) {
+            rc = jit_launch(ctx, entry->fns[p], st->d, oop ? st->shadow : st->d, table, ntiles, k);
+            ++ctx->jit_launches;
+        } else {
+            rc = (st->prec == QSE_C128) ? launch_pass<double>(ctx, st, k, prog, table, p, oop)
+                                        : launch_pass<float>(ctx, st, k, prog, table, p, oop);
+        }
         if (rc) return rc;
         launch_end(ctx, &t, bytes);
         if (oop) std::swap(st->d, st->shadow);

@@ -25,11 +25,16 @@ struct Scratch {
     size_t bytes = 0;
 };
 
-// A fused-pass program resident on the device, keyed by its serialized bytes.
+// A fused-pass program resident on the device, keyed by its serialized bytes,
+// with its run-time compiled kernels once it has them (one per pass).
 struct ProgEntry {
     std::vector<char> bytes;
     void* dev = nullptr;
     uint64_t stamp = 0;
+    int runs = 0;
+    bool jit_failed = false;
+    hipModule_t mod = nullptr;
+    std::vector<hipFunction_t> fns;
 };
 
 }  // namespace qse
@@ -50,6 +55,11 @@ struct qse_ctx {
     // Fusion shape per precision (tile qubits k, contiguous low qubits L).
     int tile_bits[2] = {0, 0};
     int low_bits[2] = {0, 0};
+    // Run-time specialisation (qse_jit_mode) and its counters.
+    int jit_mode = QSE_JIT_AUTO;
+    int64_t jit_compiled = 0;
+    int64_t jit_launches = 0;
+    double jit_compile_ms = 0.0;
     // Timing.
     bool timing = false;
     std::vector<qse::TimedLaunch> pending;

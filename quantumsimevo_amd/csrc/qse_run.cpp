@@ -9,7 +9,7 @@
 namespace qse {
 
 int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int* passes_out,
-              bool dry_run);
+              bool dry_run, bool allow_jit);
 
 double op_angle(const qse_op& op, const double* params) {
     if (op.param >= 0 && params) return op.theta + op.scale * params[op.param];
@@ -91,7 +91,7 @@ int qse_run(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const doub
     int rc = validate_ops(st->n, ops, nops, params);
     if (rc) return rc;
     if (nops == 0) return QSE_OK;
-    return run_fused(ctx, st, ops, nops, params, nullptr, false);
+    return run_fused(ctx, st, ops, nops, params, nullptr, false, true);
 }
 
 int qse_run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params) {

@@ -180,6 +180,37 @@ extern "C" int emu_check_plan(int n, int k, int L, const qse_op* ops, int nops, 
             if (cov != ((1ULL << k) - 1)) return 4;
         }
         if (fp.nslots > FUSE_MAX_SLOTS) return 5;
+        // every index the kernels follow stays inside the program's arrays
+        if ((size_t)fp.stage_begin + fp.nstages > P.stages.size()) return 10;
+        if ((size_t)fp.slot_begin + fp.nslots > P.slots.size()) return 11;
+        for (int sl = 0; sl < fp.nslots; ++sl) {
+            const FSlot& S = P.slots[fp.slot_begin + sl];
+            if (S.begin > S.end || S.end > P.eterms.size()) return 12;
+        }
+        for (int sg = 0; sg < fp.nstages; ++sg) {
+            const FStage& S = P.stages[fp.stage_begin + sg];
+            if (S.op_begin > S.op_end || S.op_end > P.ops.size()) return 13;
+            for (uint32_t oi = S.op_begin; oi < S.op_end; ++oi) {
+                const FOp& op = P.ops[oi];
+                if (op.rctl >= 16) return 14;
+                if (op.type == FOP_MONO) {
+                    if (op.term_begin > op.term_end || op.term_end > P.terms.size()) return 15;
+                    if (op.tctl >> (k - 4)) return 16;
+                    if (op.tctl && (op.alpha_begin == FUSE_NO_ALPHA ||
+                                    (size_t)op.alpha_begin + 2 * (k - 4) > P.alphas.size()))
+                        return 17;
+                    for (uint32_t t = op.term_begin; t < op.term_end; ++t) {
+                        const FTerm& T = P.terms[t];
+                        if (T.slot >= fp.nslots) return 18;
+                        if (T.tmask >> k) return 19;
+                    }
+                } else {
+                    if (op.p >= 4) return 20;
+                    if (op.tctl >> k) return 21;
+                    if (op.type != FOP_DENSE && op.type != FOP_XSWAP && (op.rctl || op.tctl || op.ectl)) return 22;
+                }
+            }
+        }
     }
     return 0;
 }
@@ -214,4 +245,62 @@ extern "C" int emu_plan_stats(int n, int k, int L, const qse_op* ops, int nops, 
         ++np;
     }
     return np;
+}
+
+// ---- run-time specialised passes (qse_jit_gen.cpp) ----
+// emu_jit_prepare plans like the library (relabelling plan, in-place plan when
+// it is not realisable), keeps the plan for the calls below and returns the
+// length of the generated HIP source, copied into `out` when `cap` allows.
+namespace qse {
+std::string jit_source(const HostPlan& P, int prec);
+}
+static HostPlan g_jit_plan;
+static int g_jit_n = 0;
+
+extern "C" long emu_jit_prepare(int n, int k, int L, const qse_op* ops, int nops, const double* params, int relabel,
+                                int prec, char* out, long cap) {
+    g_jit_plan = HostPlan();
+    int rc = build_plan(n, k, L, ops, nops, params, g_jit_plan, relabel != 0);
+    if (rc && relabel) rc = build_plan(n, k, L, ops, nops, params, g_jit_plan, false);
+    if (rc) return -rc;
+    g_jit_n = n;
+    const std::string src = jit_source(g_jit_plan, prec);
+    if (out && cap > (long)src.size()) std::memcpy(out, src.c_str(), src.size() + 1);
+    return (long)src.size();
+}
+
+// Pass count of the prepared plan, and per pass: tile qubits, out-of-place flag, slots.
+extern "C" int emu_jit_pass_info(int pass, int* k, int* out_of_place, int* nslots) {
+    if (pass < 0) return (int)g_jit_plan.passes.size();
+    const FPass& fp = g_jit_plan.passes[pass];
+    *k = fp.k;
+    *out_of_place = fp.out_of_place;
+    *nslots = fp.nslots;
+    return 0;
+}
+
+// Slot table of `pass` as k_slot_table writes it: per tile, per slot
+// {angle, cos, sin, 0}.
+extern "C" int emu_jit_slot_table(int pass, double* table) {
+    const FPass& fp = g_jit_plan.passes[pass];
+    const uint64_t ntiles = 1ULL << (g_jit_n - fp.k);
+    for (uint64_t ti = 0; ti < ntiles; ++ti) {
+        uint64_t base = 0, t = ti;
+        for (int e = 0; e < fp.next; ++e) {
+            base |= (t & 1ULL) << fp.epos[e];
+            t >>= 1;
+        }
+        for (int sl = 0; sl < fp.nslots; ++sl) {
+            const FSlot& S = g_jit_plan.slots[fp.slot_begin + sl];
+            double a = 0.0;
+            for (uint32_t e = S.begin; e < S.end; ++e)
+                if ((base & g_jit_plan.eterms[e].emask) == g_jit_plan.eterms[e].emask) a += g_jit_plan.eterms[e].ang;
+            double* row = table + 4 * (ti * (uint64_t)fp.nslots + sl);
+            row[0] = a;
+            row[1] = std::cos(a);
+            row[2] = std::sin(a);
+            row[3] = 0.0;
+        }
+    }
+    return 0;
 }

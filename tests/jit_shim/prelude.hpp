@@ -1,0 +1,22 @@
+// Host shim for run-time generated pass kernels (test infrastructure): the
+// generated HIP source is compiled with g++ after this prelude, and each
+// workgroup thread runs as a std::thread; __syncthreads is a barrier over them
+// and __shared__ arrays are function statics shared by those threads.
+#include <pthread.h>
+
+#include <thread>
+#include <vector>
+
+struct shim_dim3 {
+    unsigned x, y, z;
+};
+extern thread_local shim_dim3 threadIdx;
+extern shim_dim3 blockIdx, gridDim;
+void shim_syncthreads();
+
+#define __global__
+#define __device__
+#define __forceinline__ inline
+#define __shared__ static
+#define __launch_bounds__(x)
+#define __syncthreads() shim_syncthreads()

@@ -1,0 +1,103 @@
+"""Run-time specialised fused passes (hiprtc, qse_ctx_set_jit) on the GPU.
+
+Bar: complex128 within 1e-12 max abs of the oracle and within 1e-13 of the
+generic fused kernel (same products, possibly different FMA contraction);
+complex64 within 2e-5 of the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from quantumsimevo_amd import capi
+from quantumsimevo_amd import circuits as G
+
+pytestmark = pytest.mark.gpu
+
+
+def run_mode(ctx, mode, n, ops, psi, params=None, prec=capi.C128):
+    ctx.set_jit(mode)
+    try:
+        st = ctx.state(n, prec)
+        st.upload(psi)
+        st.run(ops, params)
+        out = st.download()
+        st.close()
+    finally:
+        ctx.set_jit(capi.JIT_AUTO)
+    return out
+
+
+def check(ctx, n, ops, params=None, prec=capi.C128, seed=11):
+    psi = G.random_state(n, seed)
+    want = O.run_ops(psi, n, ops, params)
+    before = ctx.jit_stats()
+    jit = run_mode(ctx, capi.JIT_ALWAYS, n, ops, psi, params, prec)
+    after = ctx.jit_stats()
+    assert after["launches"] > before["launches"], "compiled kernels were not launched"
+    generic = run_mode(ctx, capi.JIT_OFF, n, ops, psi, params, prec)
+    if prec == capi.C128:
+        assert np.max(np.abs(jit - want)) < 1e-12
+        assert np.max(np.abs(jit - generic)) < 1e-13
+    else:
+        assert np.max(np.abs(jit - want)) < 2e-5
+
+
+def test_qft_iqft_default_shape(ctx):
+    n = 14
+    check(ctx, n, G.qft(n).ops())
+    check(ctx, n, G.iqft(n).ops())
+
+
+def test_random_layered_and_mixed(ctx):
+    check(ctx, 16, G.random_layered(16, 8, seed=4).ops())
+    check(ctx, 13, G.random_mixed(13, 300, 8).ops())
+
+
+def test_small_tiles_with_external_slots(ctx):
+    # 9-qubit tiles with 3 low qubits: terms on external qubits go through slots
+    ctx.set_fusion(capi.C128, 9, 3)
+    try:
+        n = 14
+        edges = G.random_regular_graph(n, 3, seed=3)
+        ops = G.qaoa_maxcut(n, edges, 3).ops()
+        params = np.random.default_rng(5).random(6) * math.pi
+        check(ctx, n, ops, params)
+        check(ctx, n, G.random_mixed(n, 200, 9).ops())
+    finally:
+        ctx.set_fusion(capi.C128, 0, 0)
+
+
+def test_complex64(ctx):
+    check(ctx, 16, G.random_layered(16, 6, seed=2).ops(), prec=capi.C64)
+    check(ctx, 15, G.qft(15).ops(), prec=capi.C64)
+
+
+def test_auto_compiles_on_second_run(ctx):
+    n = 14
+    ops = G.random_layered(n, 5, seed=123).ops()
+    psi = G.random_state(n, 1)
+    ctx.set_jit(capi.JIT_AUTO)
+    st = ctx.state(n)
+    st.upload(psi)
+    s0 = ctx.jit_stats()
+    st.run(ops)
+    s1 = ctx.jit_stats()
+    assert s1["compiled"] == s0["compiled"] and s1["launches"] == s0["launches"]
+    st.run(ops)
+    s2 = ctx.jit_stats()
+    assert s2["compiled"] == s1["compiled"] + 1 and s2["launches"] > s1["launches"]
+    st.run(ops)
+    s3 = ctx.jit_stats()
+    assert s3["compiled"] == s2["compiled"] and s3["launches"] > s2["launches"]
+    want = psi
+    for _ in range(3):
+        want = O.run_ops(want, n, ops)
+    assert np.max(np.abs(st.download() - want)) < 1e-12
+    st.close()
+
+
+def test_set_jit_rejects_bad_mode(ctx):
+    with pytest.raises(capi.QSEError):
+        ctx.set_jit(3)
+    ctx.set_jit(capi.JIT_AUTO)
