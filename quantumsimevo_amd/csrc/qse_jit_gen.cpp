@@ -14,6 +14,8 @@
 #include <cstdio>
 #include <sstream>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "qse_fused.hpp"
 #include "qse_hip.h"
@@ -37,13 +39,37 @@ std::string lit(double v) {
 
 std::string u64(uint64_t v) { return std::to_string(v) + "ull"; }
 
+// Sum of c*x over the given terms with literal coefficients; terms with a zero
+// coefficient are dropped and +-1 needs no product (both exact), so only
+// products that can round are emitted.
+std::string lin(std::initializer_list<std::pair<double, const char*>> terms) {
+    std::string e;
+    for (const auto& t : terms) {
+        if (t.first == 0.0) continue;
+        std::string term;
+        if (t.first == 1.0) term = t.second;
+        else if (t.first == -1.0) term = std::string("-") + t.second;
+        else term = "(R)" + lit(t.first) + " * " + t.second;
+        if (e.empty()) e = term;
+        else if (term[0] == '-') e += " - " + term.substr(1);
+        else e += " + " + term;
+    }
+    return e.empty() ? "(R)0" : e;
+}
+
+std::string sum2(const std::string& a, const std::string& b) {
+    if (a == "(R)0") return b;
+    if (b == "(R)0") return a;
+    return "(" + a + ") + (" + b + ")";
+}
+
 void emit_mul_g(std::ostringstream& o, const std::string& dr, const std::string& di) {
     o << "      { const double nr = gr * " << dr << " - gi * " << di << "; gi = gr * " << di << " + gi * " << dr
       << "; gr = nr; }\n";
 }
 
 // One pass as kernel `name`.
-void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::string& name) {
+void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::string& name, bool prefetch) {
     const FPass& fp = P.passes[pi];
     const int K = fp.k, TB = K - 4, NT = 1 << TB;
     const int nslots = fp.nslots;
@@ -57,35 +83,71 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
         o << "  if (tid & " << (1u << b) << "u) { gt |= " << u64(1ULL << fp.tq[b]) << "; ot |= "
           << u64(1ULL << fp.so_q[b]) << "; st_t |= " << (1u << fp.so_l[b]) << "u; }\n";
     o << "  const unsigned swz_tid = swz(tid), swz_st = swz(st_t);\n";
+    // With `prefetch`, the next tile's amplitudes are loaded into registers
+    // while this tile's stages run (tiles belong to one workgroup each, so the
+    // prefetch never reads what another workgroup writes). It is used for
+    // complex64 only: for complex128 the second set of 16 amplitudes exceeds
+    // the register budget of two waves per SIMD and spills, and two
+    // workgroups per CU already overlap one's memory traffic with the other's
+    // stages.
+    auto emit_loads = [&](const char* tile, const char* b) {
+        o << "      unsigned long long " << b << " = 0;\n";
+        for (int e = 0; e < fp.next; ++e)
+            o << "      " << b << " |= ((" << tile << " >> " << e << ") & 1ull) << " << fp.epos[e] << ";\n";
+        for (int i = 0; i < 16; ++i) {
+            uint64_t g = 0;
+            for (int c = 0; c < 4; ++c)
+                if ((i >> c) & 1) g |= 1ULL << fp.tq[TB + c];
+            o << "      ld" << i << " = src[" << b << " | gt | " << u64(g) << "];\n";
+        }
+    };
+    o << "  cx ld0, ld1, ld2, ld3, ld4, ld5, ld6, ld7, ld8, ld9, ld10, ld11, ld12, ld13, ld14, ld15;\n";
+    if (prefetch) {
+        o << "  if ((unsigned long long)blockIdx.x < ntiles) {\n    const unsigned long long t0 = blockIdx.x;\n";
+        emit_loads("t0", "b0");
+        o << "  }\n";
+    }
     o << "  for (unsigned long long ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {\n";
     o << "    unsigned long long base = 0, obase = 0;\n";
     for (int e = 0; e < fp.next; ++e)
         o << "    base |= ((ti >> " << e << ") & 1ull) << " << fp.epos[e] << "; obase |= ((ti >> " << e
           << ") & 1ull) << " << fp.eout[e] << ";\n";
-    o << "    cx ld[16];\n";
-    for (int i = 0; i < 16; ++i) {
-        uint64_t g = 0;
-        for (int c = 0; c < 4; ++c)
-            if ((i >> c) & 1) g |= 1ULL << fp.tq[TB + c];
-        o << "    ld[" << i << "] = src[base | gt | " << u64(g) << "];\n";
+    if (!prefetch) {
+        o << "    {\n";
+        emit_loads("ti", "bt");
+        o << "    }\n";
     }
     if (nslots > 0)
         o << "    for (int sl = tid; sl < " << nslots << "; sl += " << NT << ") slotv[sl] = slot_table[ti * "
           << nslots << "ull + sl];\n";
-    for (int i = 0; i < 16; ++i) o << "    tile[swz_tid ^ " << swz_h((uint32_t)(i * NT)) << "u] = ld[" << i << "];\n";
+    for (int i = 0; i < 16; ++i) o << "    tile[swz_tid ^ " << swz_h((uint32_t)(i * NT)) << "u] = ld" << i << ";\n";
     o << "    __syncthreads();\n";
+    if (prefetch) {
+        o << "    if (ti + gridDim.x < ntiles) {\n      const unsigned long long tn = ti + gridDim.x;\n";
+        emit_loads("tn", "bn");
+        o << "    }\n";
+    }
+    // Stage code works on scalar components (ar<v>, ai<v>) of 16 amplitude
+    // variables. nm[j] names the variable holding register j: an X/CNOT whose
+    // condition is known at generation time is a renaming, not code. (Whole
+    // struct copies for register exchanges made the compiler keep the
+    // amplitudes in scratch memory.)
     for (int sg = 0; sg < fp.nstages; ++sg) {
         const FStage& S = P.stages[fp.stage_begin + sg];
         o << "    {\n      unsigned toff = 0;\n";
         for (int t = 0; t < TB; ++t) o << "      if (tid & " << (1u << t) << "u) toff |= " << (1u << S.tmap[t]) << "u;\n";
-        o << "      const unsigned so = swz(toff);\n      cx a[16];\n";
+        o << "      const unsigned so = swz(toff);\n      R";
+        for (int v = 0; v < 16; ++v) o << (v ? ", " : " ") << "ar" << v << ", ai" << v;
+        o << ";\n";
         uint32_t rj[16];
+        int nm[16];
         for (int j = 0; j < 16; ++j) {
             uint32_t r = 0;
-            for (int p = 0; p < FUSE_REG; ++p)
-                if ((j >> p) & 1) r |= 1u << S.rmap[p];
+            for (int q = 0; q < FUSE_REG; ++q)
+                if ((j >> q) & 1) r |= 1u << S.rmap[q];
             rj[j] = swz_h(r);
-            o << "      a[" << j << "] = tile[so ^ " << rj[j] << "u];\n";
+            nm[j] = j;
+            o << "      { const cx t = tile[so ^ " << rj[j] << "u]; ar" << j << " = t.re; ai" << j << " = t.im; }\n";
         }
         for (uint32_t oi = S.op_begin; oi < S.op_end; ++oi) {
             const FOp& op = P.ops[oi];
@@ -103,69 +165,82 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
                     if (T.slot < 0 && T.emask) cond += " && ((base & " + u64(T.emask) + ") == " + u64(T.emask) + ")";
                     o << "      if (" << cond << ")\n";
                     if (T.slot >= 0) {
-                        const std::string s = "slotv[" + std::to_string(T.slot) + "]";
-                        emit_mul_g(o, s + ".c", s + ".s");
+                        const std::string sv = "slotv[" + std::to_string(T.slot) + "]";
+                        emit_mul_g(o, sv + ".c", sv + ".s");
                     } else {
                         emit_mul_g(o, lit(T.d[0]), lit(T.d[1]));
                     }
                 }
-                o << "      cx g; g.re = (R)gr; g.im = (R)gi;\n";
+                o << "      const R g_r = (R)gr, g_i = (R)gi;\n";
                 for (int j = 0; j < 16; ++j)
-                    if (((uint32_t)j & op.rctl) == op.rctl) o << "      a[" << j << "] = cmul(g, a[" << j << "]);\n";
+                    if (((uint32_t)j & op.rctl) == op.rctl) {
+                        const std::string v = std::to_string(nm[j]);
+                        o << "      { const R x = ar" << v << ", y = ai" << v << "; ar" << v << " = g_r * x - g_i * y; ai"
+                          << v << " = g_r * y + g_i * x; }\n";
+                    }
                 o << "      }\n";
                 continue;
             }
             const bool guard = op.tctl != 0 || op.ectl != 0;
-            if (guard)
-                o << "      if (((toff & " << op.tctl << "u) == " << op.tctl << "u) && ((base & " << u64(op.ectl)
-                  << ") == " << u64(op.ectl) << ")) {\n";
-            else
-                o << "      {\n";
-            if (op.type == FOP_DENSE)
-                o << "      const cx m00 = {(R)" << lit(op.m[0]) << ", (R)" << lit(op.m[1]) << "}, m01 = {(R)"
-                  << lit(op.m[2]) << ", (R)" << lit(op.m[3]) << "};\n"
-                  << "      const cx m10 = {(R)" << lit(op.m[4]) << ", (R)" << lit(op.m[5]) << "}, m11 = {(R)"
-                  << lit(op.m[6]) << ", (R)" << lit(op.m[7]) << "};\n";
-            const int p = (int)op.p;
+            const std::string cond = "((toff & " + std::to_string(op.tctl) + "u) == " + std::to_string(op.tctl) +
+                                     "u) && ((base & " + u64(op.ectl) + ") == " + u64(op.ectl) + ")";
+            const int pp = (int)op.p;
+            std::vector<std::pair<int, int>> prs;
             for (int jj = 0; jj < 8; ++jj) {
-                const int j0 = ((jj >> p) << (p + 1)) | (jj & ((1 << p) - 1));
-                const int j1 = j0 | (1 << p);
+                const int j0 = ((jj >> pp) << (pp + 1)) | (jj & ((1 << pp) - 1));
                 // REAL, RXLIKE and H are uncontrolled (rctl = 0); a failed register
                 // control of DENSE/XSWAP is the identity, i.e. nothing to emit.
-                if (((uint32_t)j0 & op.rctl) != op.rctl) continue;
-                const std::string A0 = "a[" + std::to_string(j0) + "]", A1 = "a[" + std::to_string(j1) + "]";
-                o << "      { const cx v0 = " << A0 << ", v1 = " << A1 << ";\n";
+                if (((uint32_t)j0 & op.rctl) == op.rctl) prs.push_back({j0, j0 | (1 << pp)});
+            }
+            if (op.type == FOP_XSWAP) {
+                if (!guard) {
+                    for (auto& pr : prs) std::swap(nm[pr.first], nm[pr.second]);
+                    continue;
+                }
+                o << "      { const bool c = " << cond << ";\n";
+                for (auto& pr : prs) {
+                    const std::string a = std::to_string(nm[pr.first]), b = std::to_string(nm[pr.second]);
+                    o << "      { const R x0 = ar" << a << ", y0 = ai" << a << ", x1 = ar" << b << ", y1 = ai" << b << ";\n"
+                      << "        ar" << a << " = c ? x1 : x0; ai" << a << " = c ? y1 : y0; ar" << b << " = c ? x0 : x1; ai"
+                      << b << " = c ? y0 : y1; }\n";
+                }
+                o << "      }\n";
+                continue;
+            }
+            o << (guard ? "      if (" + cond + ") {\n" : std::string("      {\n"));
+            for (auto& pr : prs) {
+                const std::string a = std::to_string(nm[pr.first]), b = std::to_string(nm[pr.second]);
+                o << "      { const R x0 = ar" << a << ", y0 = ai" << a << ", x1 = ar" << b << ", y1 = ai" << b << ";\n";
                 switch (op.type) {
                     case FOP_H:
-                        o << "        " << A0 << ".re = H * (v0.re + v1.re); " << A0 << ".im = H * (v0.im + v1.im);\n"
-                          << "        " << A1 << ".re = H * (v0.re - v1.re); " << A1 << ".im = H * (v0.im - v1.im); }\n";
-                        break;
-                    case FOP_XSWAP:
-                        o << "        " << A0 << " = v1; " << A1 << " = v0; }\n";
+                        o << "        ar" << a << " = H * (x0 + x1); ai" << a << " = H * (y0 + y1); ar" << b
+                          << " = H * (x0 - x1); ai" << b << " = H * (y0 - y1); }\n";
                         break;
                     case FOP_REAL:
-                        o << "        " << A0 << ".re = (R)" << lit(op.m[0]) << " * v0.re + (R)" << lit(op.m[2])
-                          << " * v1.re; " << A0 << ".im = (R)" << lit(op.m[0]) << " * v0.im + (R)" << lit(op.m[2])
-                          << " * v1.im;\n        " << A1 << ".re = (R)" << lit(op.m[4]) << " * v0.re + (R)"
-                          << lit(op.m[6]) << " * v1.re; " << A1 << ".im = (R)" << lit(op.m[4]) << " * v0.im + (R)"
-                          << lit(op.m[6]) << " * v1.im; }\n";
+                        o << "        ar" << a << " = " << lin({{op.m[0], "x0"}, {op.m[2], "x1"}}) << "; ai" << a << " = "
+                          << lin({{op.m[0], "y0"}, {op.m[2], "y1"}}) << ";\n        ar" << b << " = "
+                          << lin({{op.m[4], "x0"}, {op.m[6], "x1"}}) << "; ai" << b << " = "
+                          << lin({{op.m[4], "y0"}, {op.m[6], "y1"}}) << "; }\n";
                         break;
                     case FOP_RXLIKE:
-                        o << "        " << A0 << ".re = (R)" << lit(op.m[0]) << " * v0.re - (R)" << lit(op.m[3])
-                          << " * v1.im; " << A0 << ".im = (R)" << lit(op.m[0]) << " * v0.im + (R)" << lit(op.m[3])
-                          << " * v1.re;\n        " << A1 << ".re = (R)" << lit(op.m[6]) << " * v1.re - (R)"
-                          << lit(op.m[5]) << " * v0.im; " << A1 << ".im = (R)" << lit(op.m[6]) << " * v1.im + (R)"
-                          << lit(op.m[5]) << " * v0.re; }\n";
+                        o << "        ar" << a << " = " << lin({{op.m[0], "x0"}, {-op.m[3], "y1"}}) << "; ai" << a << " = "
+                          << lin({{op.m[0], "y0"}, {op.m[3], "x1"}}) << ";\n        ar" << b << " = "
+                          << lin({{op.m[6], "x1"}, {-op.m[5], "y0"}}) << "; ai" << b << " = "
+                          << lin({{op.m[6], "y1"}, {op.m[5], "x0"}}) << "; }\n";
                         break;
-                    default:  // FOP_DENSE
-                        o << "        " << A0 << " = cadd(cmul(m00, v0), cmul(m01, v1)); " << A1
-                          << " = cadd(cmul(m10, v0), cmul(m11, v1)); }\n";
+                    default:  // FOP_DENSE: (m00 v0) + (m01 v1), (m10 v0) + (m11 v1)
+                        o << "        ar" << a << " = " << sum2(lin({{op.m[0], "x0"}, {-op.m[1], "y0"}}), lin({{op.m[2], "x1"}, {-op.m[3], "y1"}}))
+                          << "; ai" << a << " = " << sum2(lin({{op.m[0], "y0"}, {op.m[1], "x0"}}), lin({{op.m[2], "y1"}, {op.m[3], "x1"}}))
+                          << ";\n        ar" << b << " = " << sum2(lin({{op.m[4], "x0"}, {-op.m[5], "y0"}}), lin({{op.m[6], "x1"}, {-op.m[7], "y1"}}))
+                          << "; ai" << b << " = " << sum2(lin({{op.m[4], "y0"}, {op.m[5], "x0"}}), lin({{op.m[6], "y1"}, {op.m[7], "x1"}}))
+                          << "; }\n";
                         break;
                 }
             }
             o << "      }\n";
         }
-        for (int j = 0; j < 16; ++j) o << "      tile[so ^ " << rj[j] << "u] = a[" << j << "];\n";
+        for (int j = 0; j < 16; ++j)
+            o << "      { cx t; t.re = ar" << nm[j] << "; t.im = ai" << nm[j] << "; tile[so ^ " << rj[j] << "u] = t; }\n";
         o << "      __syncthreads();\n    }\n";
     }
     for (int i = 0; i < 16; ++i) {
@@ -197,7 +272,7 @@ std::string jit_source(const HostPlan& P, int prec) {
          "__device__ __forceinline__ cx cadd(cx a, cx b) { cx r; r.re = a.re + b.re; r.im = a.im + b.im; return r; }\n"
          "__device__ __forceinline__ unsigned swz(unsigned li) { return li ^ (((li >> 4) ^ (li >> 8) ^ (li >> 12)) & "
          "15u); }\n\n";
-    for (int p = 0; p < (int)P.passes.size(); ++p) emit_pass(o, P, p, jit_kernel_name(p));
+    for (int p = 0; p < (int)P.passes.size(); ++p) emit_pass(o, P, p, jit_kernel_name(p), prec == QSE_C64);
     return o.str();
 }
 

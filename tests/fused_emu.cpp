@@ -304,3 +304,10 @@ extern "C" int emu_jit_slot_table(int pass, double* table) {
     }
     return 0;
 }
+
+// Tile qubits of pass `pass` of the prepared plan (load order: tq[0..k-1]).
+extern "C" int emu_jit_pass_qubits(int pass, uint32_t* tq) {
+    const FPass& fp = g_jit_plan.passes[pass];
+    for (int b = 0; b < fp.k; ++b) tq[b] = fp.tq[b];
+    return fp.k;
+}
