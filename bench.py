@@ -43,12 +43,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def profiled_traffic(n: int, precision: str, fused: bool):
-    """HBM bytes per launch of the fused pass as measured by rocprofv3 --pmc
+def profiled_traffic(kernel: str, n: int, precision: str):
+    """HBM bytes per launch of the timed kernel as measured by rocprofv3 --pmc
     FETCH_SIZE/WRITE_SIZE (calibrated on a copy kernel) for this workload,
     from the committed summary; None for workloads without one."""
-    f = ROOT / "profiles" / f"k_fused_qft{n}_{precision}_traffic.md"
-    if not fused or not f.exists():
+    f = ROOT / "profiles" / f"{kernel}_qft{n}_{precision}_traffic.md"
+    if not f.exists():
         return None, None
     for line in f.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch:"):
@@ -163,8 +163,10 @@ def main() -> int:
     log(f"device: {info['name']} CUs={info['compute_units']} HBM={info['total_bytes'] / 2**30:.1f} GiB")
     st = ctx.state(n, prec)
     st.set_random(12345)
-    passes = ctx.plan_passes(n, prec, ops) if not args.unfused else len(ops)
-    log(f"QFT-{n} {args.precision}: {len(ops)} gates, {upd_step:.4e} amp-updates/step, planner passes={passes}")
+    plan = ctx.plan_info(n, prec, ops) if not args.unfused else {"passes": len(ops), "tile_bits": 0, "low_bits": 0,
+                                                                  "stages": 0}
+    passes = plan["passes"]
+    log(f"QFT-{n} {args.precision}: {len(ops)} gates, {upd_step:.4e} amp-updates/step, plan {plan}")
 
     fused = not args.unfused
     for i in range(args.warmup):
@@ -202,7 +204,10 @@ def main() -> int:
     log(f"dominant kernel {kname}: {dom['launches']} launches, mean {mean_ms:.4f} ms, "
         f"{bytes_per_launch / 1e9:.3f} GB/launch -> {achieved / 1e9:.1f} GB/s")
 
-    traffic, traffic_src = profiled_traffic(n, args.precision, fused and dom_f == capi.FAM_FUSED)
+    prof_kernel = None
+    if fused and dom_f == capi.FAM_FUSED:
+        prof_kernel = "qse_jit_pass" if jit_timed else "k_fused"
+    traffic, traffic_src = profiled_traffic(prof_kernel, n, args.precision) if prof_kernel else (None, None)
     # size-independent sanity: the norm stays 1 after K unitary steps
     nrm = st.norm2()
     if abs(nrm - 1.0) > (1e-9 if prec == capi.C128 else 1e-3):
@@ -210,7 +215,7 @@ def main() -> int:
         return 3
 
     result = {
-        "metric": "amp-updates/s (QFT, 30q complex128 resident state)",
+        "metric": f"amp-updates/s (QFT, {n}q {'complex128' if prec == capi.C128 else 'complex64'} resident state)",
         "value": upd_step * args.steps / (t1 - t0),
         "unit": "amp-updates/s",
         "n_gpus": 1,
@@ -224,6 +229,7 @@ def main() -> int:
         "data": "synthetic: seeded random normalised state generated on the device",
         "config": {"workload": f"QFT-{n} {args.precision} (QFT.cpp:9-22), {len(ops)} gates, state resident in HBM",
                    "qubits": n, "precision": args.precision, "fused": fused, "passes_per_step": passes,
+                   "tile_qubits": plan["tile_bits"], "contiguous_low_qubits": plan["low_bits"],
                    "jit": args.jit, "jit_timed": jit_timed, "jit_compile_ms": jit1["compile_ms"]},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,

@@ -117,7 +117,12 @@ int qse_ctx_destroy(qse_ctx* ctx);
 int qse_ctx_sync(qse_ctx* ctx);
 /* Fusion shape used by qse_run(): tile_bits = qubits per LDS tile (k),
  * low_bits = contiguous low qubits always in the tile (coalescing run of
- * 2^low_bits amplitudes). 0 selects the tuned default for the precision. */
+ * 2^low_bits amplitudes). tile_bits = 0 (the default) lets the shape tuner
+ * pick the shape per gate list: every candidate shape is planned and the one
+ * of least estimated time (passes x measured relative pass cost) runs;
+ * low_bits = 0 with a set tile_bits selects the default run length. No
+ * counterpart in the reference (one kernel launch per gate). qse_ctx_get_fusion
+ * returns the settings (0 = tuned). */
 int qse_ctx_set_fusion(qse_ctx* ctx, int precision, int tile_bits, int low_bits);
 int qse_ctx_get_fusion(qse_ctx* ctx, int precision, int* tile_bits, int* low_bits);
 /* Run-time specialisation of fused passes (no counterpart in the reference,
@@ -192,6 +197,11 @@ int qse_run(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const doub
 int qse_run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params);
 /* Number of HBM passes qse_run() would make for this list (planner report). */
 int qse_plan_passes(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops, int nops, int* passes);
+/* The plan qse_run() would execute: passes, tile shape (0, 0 below 10 qubits,
+ * where every gate is its own sweep) and total LDS stages. Any out pointer may
+ * be NULL. */
+int qse_plan_info(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops, int nops, int* passes,
+                  int* tile_bits, int* low_bits, int* stages);
 
 /* ---- readout (StateVector.cpp:77-156), driven by host-supplied uniforms ----
  * u are canonical uniforms in [0,1) (std::generate_canonical<double,53>); the

@@ -29,7 +29,11 @@
 namespace qse {
 
 void fusion_shape(const qse_ctx* ctx, int n, int prec, int* k_out, int* L_out);
-int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P, bool relabel);
+int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P, int flags);
+int choose_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P,
+                bool allow_oop);
+int tuned_plan(const qse_ctx* ctx, int n, int prec, const qse_op* ops, int nops, const double* params, HostPlan& P,
+               bool allow_oop, int* k_out, int* L_out);
 int run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params);
 std::string jit_source(const HostPlan& P, int prec);
 int jit_compile(const std::string& src, int npasses, hipModule_t* mod, std::vector<hipFunction_t>& fns);
@@ -388,9 +392,10 @@ void serialize(const HostPlan& P, Serialized& S) {
 }  // namespace
 
 // Device copy of a serialized program, reused when the same program runs again.
-int program_on_device(qse_ctx* ctx, const Serialized& S, ProgEntry** entry) {
+int program_on_device(qse_ctx* ctx, const Serialized& S, int prec, ProgEntry** entry) {
     for (auto& e : ctx->prog_cache) {
-        if (e.bytes.size() == S.bytes.size() && std::memcmp(e.bytes.data(), S.bytes.data(), S.bytes.size()) == 0) {
+        if (e.prec == prec && e.bytes.size() == S.bytes.size() &&
+            std::memcmp(e.bytes.data(), S.bytes.data(), S.bytes.size()) == 0) {
             e.stamp = ++ctx->prog_clock;
             *entry = &e;
             return QSE_OK;
@@ -407,6 +412,7 @@ int program_on_device(qse_ctx* ctx, const Serialized& S, ProgEntry** entry) {
     }
     ProgEntry e;
     e.bytes = S.bytes;
+    e.prec = prec;
     QSE_HIP(hipMalloc(&e.dev, S.bytes.size()));
     // pageable source: the copy is staged before this call returns
     QSE_HIP(hipMemcpyAsync(e.dev, S.bytes.data(), S.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -471,23 +477,20 @@ static int launch_pass(qse_ctx* ctx, qse_state* st, int K, const FProgram& prog,
 // Smallest state that goes through the fused path (below: one sweep per gate).
 constexpr int FUSE_MIN_QUBITS = 10;
 
-// The relabelling plan (SWAPs free, last pass out of place) when it saves
-// passes, else the in-place plan.
-int choose_plan(const qse_ctx* ctx, int n, int k, int L, const qse_op* ops, int nops, const double* params,
-                HostPlan& P) {
-    (void)ctx;
-    HostPlan A, B;
-    int rc = build_plan(n, k, L, ops, nops, params, A, true);
-    if (rc == QSE_ERR_UNSUPPORTED) return build_plan(n, k, L, ops, nops, params, P, false);
-    if (rc) return rc;
-    if (A.passes.empty() || !A.passes.back().out_of_place) {
-        P = std::move(A);
-        return QSE_OK;
-    }
-    rc = build_plan(n, k, L, ops, nops, params, B, false);
-    if (rc) return rc;
-    P = (A.passes.size() < B.passes.size()) ? std::move(A) : std::move(B);
-    return QSE_OK;
+// Key of a qse_run input for the plan cache: everything the plan depends on.
+static std::vector<char> plan_key(const qse_ctx* ctx, const qse_state* st, const qse_op* ops, int nops,
+                                  const double* params) {
+    int maxp = -1;
+    for (int i = 0; i < nops; ++i) maxp = std::max(maxp, (int)ops[i].param);
+    const int head[6] = {st->n, st->prec, ctx->tile_bits[st->prec], ctx->low_bits[st->prec], nops, maxp + 1};
+    std::vector<char> key(sizeof(head) + sizeof(qse_op) * (size_t)nops + sizeof(double) * (size_t)(maxp + 1));
+    char* w = key.data();
+    std::memcpy(w, head, sizeof(head));
+    w += sizeof(head);
+    if (nops) std::memcpy(w, ops, sizeof(qse_op) * (size_t)nops);
+    w += sizeof(qse_op) * (size_t)nops;
+    if (maxp >= 0 && params) std::memcpy(w, params, sizeof(double) * (size_t)(maxp + 1));
+    return key;
 }
 
 int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int* passes_out,
@@ -496,31 +499,61 @@ int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const do
         if (passes_out) *passes_out = nops;
         return dry_run ? QSE_OK : run_sweeps(ctx, st, ops, nops, params);
     }
-    int k, L;
-    fusion_shape(ctx, st->n, st->prec, &k, &L);
+    // qse_run inputs (allow_jit) get the tuned shape and a cached plan; the
+    // adjoint's internal runs (many distinct small programs) plan directly with
+    // the default shape.
+    int k, L, rc;
     HostPlan P;
-    int rc = choose_plan(ctx, st->n, k, L, ops, nops, params, P);
-    if (rc) return rc;
-    if (passes_out) *passes_out = (int)P.passes.size();
+    const HostPlan* plan = &P;
+    if (allow_jit) {
+        std::vector<char> key = plan_key(ctx, st, ops, nops, params);
+        PlanEntry* pe = nullptr;
+        for (auto& e : ctx->plan_cache)
+            if (e.key == key) pe = &e;
+        if (!pe) {
+            PlanEntry e;
+            e.key = std::move(key);
+            rc = tuned_plan(ctx, st->n, st->prec, ops, nops, params, e.plan, true, &e.k, &e.L);
+            if (rc) return rc;
+            if (ctx->plan_cache.size() >= 32) {
+                size_t oldest = 0;
+                for (size_t i = 1; i < ctx->plan_cache.size(); ++i)
+                    if (ctx->plan_cache[i].stamp < ctx->plan_cache[oldest].stamp) oldest = i;
+                ctx->plan_cache.erase(ctx->plan_cache.begin() + oldest);
+            }
+            ctx->plan_cache.push_back(std::move(e));
+            pe = &ctx->plan_cache.back();
+        }
+        pe->stamp = ++ctx->prog_clock;
+        plan = &pe->plan;
+        k = pe->k;
+        L = pe->L;
+    } else {
+        fusion_shape(ctx, st->n, st->prec, &k, &L);
+        rc = choose_plan(st->n, k, L, ops, nops, params, P, true);
+        if (rc) return rc;
+    }
+    if (passes_out) *passes_out = (int)plan->passes.size();
     if (dry_run) return QSE_OK;
-    const bool needs_shadow = !P.passes.empty() && P.passes.back().out_of_place;
+    const bool needs_shadow = !plan->passes.empty() && plan->passes.back().out_of_place;
     if (needs_shadow && !st->shadow) {
         if (hipMalloc(&st->shadow, st->bytes) != hipSuccess) {
             // no room for a second buffer: realise SWAPs in place instead
             (void)hipGetLastError();
             st->shadow = nullptr;
             P = HostPlan();
-            rc = build_plan(st->n, k, L, ops, nops, params, P, false);
+            rc = choose_plan(st->n, k, L, ops, nops, params, P, false);
             if (rc) return rc;
+            plan = &P;
         }
     }
     Serialized S;
-    serialize(P, S);
+    serialize(*plan, S);
     ProgEntry* entry = nullptr;
-    rc = program_on_device(ctx, S, &entry);
+    rc = program_on_device(ctx, S, st->prec, &entry);
     if (rc) return rc;
     bool jit = false;
-    if (allow_jit) rc = jit_for(ctx, *entry, P, st->prec, &jit);
+    if (allow_jit) rc = jit_for(ctx, *entry, *plan, st->prec, &jit);
     if (rc) return rc;
     char* dev = (char*)entry->dev;
     FProgram prog;
@@ -534,7 +567,7 @@ int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const do
     const double bytes = 2.0 * (double)amp_bytes(st->prec) * (double)(1ULL << st->n);
     const uint64_t ntiles = 1ULL << (st->n - k);
     int max_slots = 0;
-    for (const FPass& fp : P.passes) max_slots = std::max(max_slots, fp.nslots);
+    for (const FPass& fp : plan->passes) max_slots = std::max(max_slots, fp.nslots);
     SlotVal* table = nullptr;
     if (max_slots > 0) {
         void* tb;
@@ -542,16 +575,16 @@ int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const do
         if (rc) return rc;
         table = (SlotVal*)tb;
     }
-    for (int p = 0; p < (int)P.passes.size(); ++p) {
-        if (P.passes[p].nslots > 0) {
-            const uint64_t work = ntiles * (uint64_t)P.passes[p].nslots;
+    for (int p = 0; p < (int)plan->passes.size(); ++p) {
+        if (plan->passes[p].nslots > 0) {
+            const uint64_t work = ntiles * (uint64_t)plan->passes[p].nslots;
             k_slot_table<<<grid_for(ctx, work, 256, 8), 256, 0, ctx->stream>>>(prog.passes, prog.slots, prog.eterms,
                                                                               p, ntiles, table);
             QSE_HIP(hipGetLastError());
         }
         TimedLaunch t;
         launch_begin(ctx, FAM_FUSED, &t);
-        const bool oop = P.passes[p].out_of_place != 0;
+        const bool oop = plan->passes[p].out_of_place != 0;
         if (jit) {
             rc = jit_launch(ctx, entry->fns[p], st->d, oop ? st->shadow : st->d, table, ntiles, k);
             ++ctx->jit_launches;
@@ -570,26 +603,37 @@ int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const do
 
 using namespace qse;
 
-extern "C" int qse_plan_passes(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops, int nops, int* passes) {
-    if (!passes || nqubits < 1 || nqubits > 40 || (precision != QSE_C128 && precision != QSE_C64) || nops < 0 ||
+extern "C" int qse_plan_info(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops, int nops, int* passes,
+                             int* tile_bits, int* low_bits, int* stages) {
+    if (nqubits < 1 || nqubits > 40 || (precision != QSE_C128 && precision != QSE_C64) || nops < 0 ||
         (nops > 0 && !ops))
         return set_error(QSE_ERR_INVALID_ARG, "bad argument");
-    int rc = QSE_OK;
+    if (nqubits < FUSE_MIN_QUBITS) {
+        // one sweep per gate
+        if (passes) *passes = nops;
+        if (tile_bits) *tile_bits = 0;
+        if (low_bits) *low_bits = 0;
+        if (stages) *stages = nops;
+        return QSE_OK;
+    }
     // parameters do not change the plan's shape: plan with a zero parameter vector
     int maxp = -1;
     for (int i = 0; i < nops; ++i) maxp = std::max(maxp, (int)ops[i].param);
     std::vector<double> zeros(maxp + 1, 0.0);
-    if (nqubits < FUSE_MIN_QUBITS) {
-        *passes = nops;
-        return QSE_OK;
-    }
     int k, L;
-    fusion_shape(ctx, nqubits, precision, &k, &L);
     HostPlan P;
-    rc = choose_plan(ctx, nqubits, k, L, ops, nops, zeros.data(), P);
+    const int rc = tuned_plan(ctx, nqubits, precision, ops, nops, zeros.data(), P, true, &k, &L);
     if (rc) return rc;
-    *passes = (int)P.passes.size();
+    if (passes) *passes = (int)P.passes.size();
+    if (tile_bits) *tile_bits = k;
+    if (low_bits) *low_bits = L;
+    if (stages) *stages = (int)P.stages.size();
     return QSE_OK;
+}
+
+extern "C" int qse_plan_passes(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops, int nops, int* passes) {
+    if (!passes) return set_error(QSE_ERR_INVALID_ARG, "null passes");
+    return qse_plan_info(ctx, nqubits, precision, ops, nops, passes, nullptr, nullptr, nullptr);
 }
 
 extern "C" int qse_ctx_set_fusion(qse_ctx* ctx, int precision, int tile_bits, int low_bits) {
@@ -604,9 +648,7 @@ extern "C" int qse_ctx_set_fusion(qse_ctx* ctx, int precision, int tile_bits, in
 
 extern "C" int qse_ctx_get_fusion(qse_ctx* ctx, int precision, int* tile_bits, int* low_bits) {
     if (!ctx || precision < 0 || precision > 1) return set_error(QSE_ERR_INVALID_ARG, "bad argument");
-    int k, L;
-    fusion_shape(ctx, 34, precision, &k, &L);
-    if (tile_bits) *tile_bits = k;
-    if (low_bits) *low_bits = L;
+    if (tile_bits) *tile_bits = ctx->tile_bits[precision];
+    if (low_bits) *low_bits = ctx->low_bits[precision];
     return QSE_OK;
 }

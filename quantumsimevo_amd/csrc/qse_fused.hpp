@@ -52,6 +52,12 @@ struct FOp {
 
 constexpr uint32_t FUSE_NO_ALPHA = 0xffffffffu;
 
+// Planner variants (build_plan flags).
+enum PlanFlags : int {
+    PLAN_RELABEL = 1,  // SWAPs as a qubit relabelling; last pass out of place
+    PLAN_DAG = 2       // passes by list scheduling over the gate dependency graph
+};
+
 struct FTerm {
     uint32_t tmask;   // non-register tile local bits that must be set
     int32_t slot;     // >= 0: factor from the per-tile external slot; else d / ang

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/qse_hip.h"
+#include "qse_fused.hpp"
 
 namespace qse {
 
@@ -25,16 +26,28 @@ struct Scratch {
     size_t bytes = 0;
 };
 
-// A fused-pass program resident on the device, keyed by its serialized bytes,
-// with its run-time compiled kernels once it has them (one per pass).
+// A fused-pass program resident on the device, keyed by its serialized bytes
+// and the state precision (the same plan serves both precisions, its compiled
+// kernels do not), with its run-time compiled kernels once it has them (one
+// per pass).
 struct ProgEntry {
     std::vector<char> bytes;
+    int prec = 0;
     void* dev = nullptr;
     uint64_t stamp = 0;
     int runs = 0;
     bool jit_failed = false;
     hipModule_t mod = nullptr;
     std::vector<hipFunction_t> fns;
+};
+
+// A tuned plan for one exact input (qubit count, precision, gate list,
+// parameter values, shape settings), so repeated runs skip planning.
+struct PlanEntry {
+    std::vector<char> key;
+    HostPlan plan;
+    int k = 0, L = 0;
+    uint64_t stamp = 0;
 };
 
 }  // namespace qse
@@ -49,6 +62,8 @@ struct qse_ctx {
     // Device copies of recently run fused programs.
     std::vector<qse::ProgEntry> prog_cache;
     uint64_t prog_clock = 0;
+    // Tuned plans of recent qse_run inputs.
+    std::vector<qse::PlanEntry> plan_cache;
     // Pinned host staging for small readbacks.
     void* pinned = nullptr;
     size_t pinned_bytes = 0;

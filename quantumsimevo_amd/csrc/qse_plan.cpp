@@ -2,12 +2,20 @@
 // the reference's GateCommand list replayed by Circuit::execute,
 // Circuit.cpp:64-120) into LDS-tiled passes for the gfx950 fused kernel.
 //
-// Phase 1 (passes): scan the gates in order, growing the current pass's set of
-// tile qubits T (|T| <= k, always holding the low qubits 0..L-1). A gate that
-// is non-diagonal on a qubit outside T, when T is full, closes the pass.
-// Diagonal gates (Z, phase, RZ, cphase, mcphase) never close a pass. A SWAP of
-// two tile qubits is a relabelling of tile bits (no data movement until the
-// pass's store, which writes each tile bit to its final qubit).
+// Phase 1 (passes): grow the current pass's set of tile qubits T (|T| <= k,
+// always holding the low qubits 0..L-1) and give the pass every gate whose
+// non-diagonal qubits lie in T. In gate order (PLAN_DAG clear), a gate that is
+// non-diagonal on a qubit outside T, when T is full, closes the pass. With
+// PLAN_DAG the gates are list-scheduled over their dependency graph instead
+// (gates commute when the qubits they share are diagonal in both): the pass
+// takes every ready gate that fits T, and T grows by the qubits of the
+// earliest ready gate that does not, while there is room; later gates of a
+// light cone inside T thus join the pass out of order. Diagonal gates (Z,
+// phase, RZ, cphase, mcphase) never close a pass. A SWAP of two tile qubits is
+// a relabelling of tile bits (no data movement until the pass's store, which
+// writes each tile bit to its final qubit). Inside a pass the gates are
+// ordered the same way over register sets of 4 target qubits, so that the
+// stage builder below packs them into few stages.
 // Phase 2 (stages): inside a pass, consecutive non-diagonal gates share a
 // stage while their targets fit in 4 register bits. Diagonal gates are kept
 // pending as monomials (d applied where all bits of a mask are 1) and flushed
@@ -19,6 +27,7 @@
 #include <cmath>
 #include <complex>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -173,13 +182,129 @@ int popc(uint64_t x) { return __builtin_popcountll(x); }
 }  // namespace
 
 
+namespace {
+
+// Qubits on which an item is not diagonal (must be tile/register qubits) and
+// qubits on which it is diagonal (controls and phase masks).
+uint64_t nd_mask(const Item& it) {
+    if (it.t == IT_DENSE) return 1ULL << it.q;
+    if (it.t == IT_SWAP) return (1ULL << it.q) | (1ULL << it.q2);
+    return 0;
+}
+uint64_t dg_mask(const Item& it) {
+    if (it.t == IT_DENSE) return it.cmask;
+    if (it.t == IT_MONO) return it.qmask;
+    return 0;
+}
+
+// Dependency graph of an item list: item j precedes item i (j < i) when they
+// share a qubit on which at least one of them is not diagonal.
+struct Dag {
+    std::vector<std::vector<int>> succ;
+    std::vector<int> indeg;
+};
+
+void build_dag(const std::vector<Item>& v, int n, Dag& D) {
+    const int N = (int)v.size();
+    D.succ.assign(N, {});
+    D.indeg.assign(N, 0);
+    std::vector<int> lastND(n, -1), mark(N, -1);
+    std::vector<std::vector<int>> diagSince(n);
+    for (int i = 0; i < N; ++i) {
+        auto dep = [&](int j) {
+            if (j >= 0 && mark[j] != i) {
+                mark[j] = i;
+                D.succ[j].push_back(i);
+                ++D.indeg[i];
+            }
+        };
+        const uint64_t nd = nd_mask(v[i]), dg = dg_mask(v[i]);
+        for (int q = 0; q < n; ++q) {
+            if ((nd >> q) & 1) {
+                dep(lastND[q]);
+                for (int j : diagSince[q]) dep(j);
+                lastND[q] = i;
+                diagSince[q].clear();
+            } else if ((dg >> q) & 1) {
+                dep(lastND[q]);
+                diagSince[q].push_back(i);
+            }
+        }
+    }
+}
+
+// List scheduling of `v` into groups: a group has a qubit set S (starting at
+// `start`, at most `cap` qubits) and takes every ready item whose required
+// qubits (`need`) lie in S; S grows by the requirement of the earliest ready
+// item that does not fit while |S| stays <= cap. Returns the groups in order;
+// their concatenation is a topological order of `v`.
+template <class Need>
+void list_schedule(const std::vector<Item>& v, int n, uint64_t start, int cap, Need need,
+                   std::vector<std::pair<uint64_t, std::vector<int>>>& groups) {
+    Dag D;
+    build_dag(v, n, D);
+    std::set<int> ready;
+    for (int i = 0; i < (int)v.size(); ++i)
+        if (D.indeg[i] == 0) ready.insert(i);
+    size_t done = 0;
+    while (done < v.size()) {
+        uint64_t S = start;
+        std::vector<int> g;
+        for (;;) {
+            bool any = false;
+            for (auto it = ready.begin(); it != ready.end();) {
+                const int i = *it;
+                if ((need(v[i]) & ~S) == 0) {
+                    it = ready.erase(it);
+                    g.push_back(i);
+                    ++done;
+                    any = true;
+                    for (int s : D.succ[i])
+                        if (--D.indeg[s] == 0) ready.insert(s);  // visited in this sweep when s > i
+                } else {
+                    ++it;
+                }
+            }
+            if (any) continue;
+            bool grown = false;
+            for (int i : ready) {
+                const uint64_t m = need(v[i]) & ~S;
+                if (popc(S | m) <= cap) {
+                    S |= m;
+                    grown = true;
+                    break;
+                }
+            }
+            if (!grown) break;
+        }
+        groups.push_back({S, std::move(g)});
+    }
+}
+
+// Reorder the items of one pass so that consecutive runs use at most
+// FUSE_REG target qubits (the stage builder then needs few stages). SWAPs
+// are tile-bit relabellings in phase 2 and need no register.
+void order_for_stages(std::vector<Item>& items, int n) {
+    std::vector<std::pair<uint64_t, std::vector<int>>> groups;
+    list_schedule(items, n, 0, FUSE_REG, [](const Item& it) { return it.t == IT_DENSE ? nd_mask(it) : 0ULL; },
+                  groups);
+    std::vector<Item> out;
+    out.reserve(items.size());
+    for (auto& g : groups)
+        for (int i : g.second) out.push_back(items[i]);
+    items.swap(out);
+}
+
+}  // namespace
+
 // Choose (k, L) for an n-qubit state of precision prec given the context
-// settings (0 = default).
+// settings (0 = default). The defaults are the shapes the tuner below picks
+// most often: contiguous runs of 2^L amplitudes of at least 128 bytes.
 void fusion_shape(const qse_ctx* ctx, int n, int prec, int* k_out, int* L_out) {
     int k = ctx ? ctx->tile_bits[prec] : 0;
     int L = ctx ? ctx->low_bits[prec] : 0;
     if (k <= 0) k = (prec == QSE_C128) ? 12 : 13;
-    if (L <= 0) L = (prec == QSE_C128) ? 6 : 6;
+    if (L <= 0) L = (prec == QSE_C128) ? 3 : 4;
     k = std::min(k, FUSE_MAX_K);
     k = std::min(k, n - 2);
     k = std::max(k, FUSE_MIN_K);
@@ -189,10 +314,12 @@ void fusion_shape(const qse_ctx* ctx, int n, int prec, int* k_out, int* L_out) {
     *L_out = L;
 }
 
-// relabel: SWAPs become a logical->physical qubit map (no data movement); the
-// last pass then stores out of place with the inverse map so the state ends in
-// the canonical layout. Otherwise SWAPs are in-tile relabellings (in place).
-int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P, bool relabel) {
+// PLAN_RELABEL: SWAPs become a logical->physical qubit map (no data movement);
+// the last pass then stores out of place with the inverse map so the state
+// ends in the canonical layout. Otherwise SWAPs are in-tile relabellings (in
+// place). PLAN_DAG: passes by list scheduling over the dependency graph.
+int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P, int flags) {
+    const bool relabel = (flags & PLAN_RELABEL) != 0;
     std::vector<Item> items;
     to_items(ops, nops, params, items);
     const uint64_t lowT = (L > 0) ? ((1ULL << L) - 1) : 0;
@@ -205,8 +332,8 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
         return r;
     };
     // ---- phase 1: passes ----
-    std::vector<PassItems> passes;
-    passes.push_back({lowT, {}});
+    std::vector<Item> mapped;
+    mapped.reserve(items.size());
     for (Item it : items) {
         if (relabel) {
             if (it.t == IT_SWAP) {
@@ -220,20 +347,37 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
                 it.qmask = map_mask(it.qmask);
             }
         }
-        PassItems* cur = &passes.back();
-        if (it.t == IT_DENSE || it.t == IT_SWAP) {
-            uint64_t need = 1ULL << it.q;
-            if (it.t == IT_SWAP) need |= 1ULL << it.q2;
-            const uint64_t miss = need & ~cur->T;
-            if (popc(cur->T | miss) > k) {
-                passes.push_back({lowT | need, {}});
-                cur = &passes.back();
-            } else {
-                cur->T |= miss;
-            }
-        }
-        cur->items.push_back(it);
+        mapped.push_back(it);
     }
+    std::vector<PassItems> passes;
+    if (flags & PLAN_DAG) {
+        std::vector<std::pair<uint64_t, std::vector<int>>> groups;
+        list_schedule(mapped, n, lowT, k, nd_mask, groups);
+        for (auto& g : groups) {
+            PassItems pi;
+            pi.T = g.first;
+            for (int i : g.second) pi.items.push_back(mapped[i]);
+            passes.push_back(std::move(pi));
+        }
+        if (passes.empty()) passes.push_back({lowT, {}});
+    } else {
+        passes.push_back({lowT, {}});
+        for (const Item& it : mapped) {
+            PassItems* cur = &passes.back();
+            const uint64_t need = nd_mask(it);
+            if (need) {
+                const uint64_t miss = need & ~cur->T;
+                if (popc(cur->T | miss) > k) {
+                    passes.push_back({lowT | need, {}});
+                    cur = &passes.back();
+                } else {
+                    cur->T |= miss;
+                }
+            }
+            cur->items.push_back(it);
+        }
+    }
+    for (PassItems& pi : passes) order_for_stages(pi.items, n);
     bool permuted = false;
     for (int q = 0; q < n; ++q) permuted |= (phys[q] != q);
     if (relabel && permuted) {
@@ -500,6 +644,85 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
         P.passes.push_back(fp);
     }
     return QSE_OK;
+}
+
+// The best of the planner variants: fewest passes, then fewest stages, then
+// in place (no second buffer). allow_oop = false keeps to in-place plans.
+int choose_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P,
+                bool allow_oop) {
+    bool have = false;
+    long best_cost = 0;
+    bool best_oop = false;
+    for (int flags : {int(PLAN_DAG), 0, int(PLAN_RELABEL | PLAN_DAG), int(PLAN_RELABEL)}) {
+        if (!allow_oop && (flags & PLAN_RELABEL)) continue;
+        HostPlan C;
+        const int rc = build_plan(n, k, L, ops, nops, params, C, flags);
+        if (rc == QSE_ERR_UNSUPPORTED) continue;
+        if (rc) return rc;
+        const bool oop = !C.passes.empty() && C.passes.back().out_of_place;
+        const long cost = (long)C.passes.size() * 100000L + (long)C.stages.size();
+        if (!have || cost < best_cost || (cost == best_cost && best_oop && !oop)) {
+            P = std::move(C);
+            best_cost = cost;
+            best_oop = oop;
+            have = true;
+        }
+    }
+    return have ? QSE_OK : set_error(QSE_ERR_UNSUPPORTED, "no fusion plan for this gate list");
+}
+
+
+// ---- tile shape tuner ----
+// Time of one pass relative to the fastest shape of the precision, measured on
+// MI355X (30-qubit QFT and random layered circuits, run-time specialised
+// passes): short contiguous runs (small L) cost HBM efficiency, 13-qubit
+// complex128 tiles leave one workgroup per CU (its stages then do not overlap
+// another workgroup's memory traffic), complex64 runs below 16 amplitudes
+// (128 bytes) lose most of the bandwidth and are not candidates.
+struct ShapeCost {
+    int k, L;
+    double cost;
+};
+const ShapeCost kShapesC128[] = {{12, 3, 1.10}, {12, 4, 1.04}, {12, 5, 1.02}, {12, 6, 1.00},
+                                 {11, 3, 1.12}, {11, 4, 1.04}, {11, 5, 1.015}, {11, 6, 1.00},
+                                 {13, 3, 1.50}, {13, 4, 1.43}, {13, 5, 1.30}, {13, 6, 1.20}};
+const ShapeCost kShapesC64[] = {{13, 4, 1.24}, {13, 5, 1.22}, {13, 6, 1.10},
+                                {12, 4, 1.11}, {12, 5, 1.01}, {12, 6, 1.00}};
+
+// The plan run_fused executes: with a tile size set on the context, that
+// shape; otherwise the candidate shape of least estimated time (passes x
+// relative pass cost, stages as a tie-break).
+int tuned_plan(const qse_ctx* ctx, int n, int prec, const qse_op* ops, int nops, const double* params, HostPlan& P,
+               bool allow_oop, int* k_out, int* L_out) {
+    int k0, L0;
+    fusion_shape(ctx, n, prec, &k0, &L0);
+    const bool fixed = ctx && ctx->tile_bits[prec] > 0;
+    if (!fixed) {
+        const ShapeCost* shapes = (prec == QSE_C128) ? kShapesC128 : kShapesC64;
+        const int ns = (prec == QSE_C128) ? (int)(sizeof(kShapesC128) / sizeof(ShapeCost))
+                                          : (int)(sizeof(kShapesC64) / sizeof(ShapeCost));
+        bool have = false;
+        double best = 0;
+        for (int i = 0; i < ns; ++i) {
+            const int k = shapes[i].k, L = shapes[i].L;
+            if (k > n - 2 || L > k - FUSE_REG) continue;
+            HostPlan C;
+            const int rc = choose_plan(n, k, L, ops, nops, params, C, allow_oop);
+            if (rc) return rc;
+            const double cost = (double)C.passes.size() * shapes[i].cost + 0.001 * (double)C.stages.size();
+            if (!have || cost < best) {
+                P = std::move(C);
+                best = cost;
+                have = true;
+                *k_out = k;
+                *L_out = L;
+            }
+        }
+        if (have) return QSE_OK;
+    }
+    *k_out = k0;
+    *L_out = L0;
+    return choose_plan(n, k0, L0, ops, nops, params, P, allow_oop);
 }
 
 }  // namespace qse

@@ -23,11 +23,26 @@ double op_angle(const qse_op& op, const double* params) {
     if (op.param >= 0 && params) return op.theta + op.scale * params[op.param];
     return op.theta;
 }
-int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P, bool relabel);
+int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P, int flags);
+int choose_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P,
+                bool allow_oop);
 }  // namespace qse
 
 using namespace qse;
 using cd = std::complex<double>;
+
+// Plan for the emulator entry points: flags >= 0 selects one planner variant
+// (PLAN_RELABEL / PLAN_DAG bits; a relabelling plan that is not realisable
+// falls back to in place), flags < 0 the library's own choice.
+static int plan_for(int n, int k, int L, const qse_op* ops, int nops, const double* params, int flags, HostPlan& P) {
+    if (flags < 0) return choose_plan(n, k, L, ops, nops, params, P, true);
+    int rc = build_plan(n, k, L, ops, nops, params, P, flags);
+    if (rc && (flags & PLAN_RELABEL)) {
+        P = HostPlan();
+        rc = build_plan(n, k, L, ops, nops, params, P, flags & ~PLAN_RELABEL);
+    }
+    return rc;
+}
 
 static uint32_t swz(uint32_t li) { return li ^ (((li >> 4) ^ (li >> 8) ^ (li >> 12)) & 15u); }
 
@@ -73,8 +88,7 @@ static void apply_op(const HostPlan& P, const FOp& op, cd* a, int tid, int TB, u
 extern "C" int emu_run_fused(int n, int k, int L, const qse_op* ops, int nops, const double* params, double* psi,
                              int* npasses, int relabel) {
     HostPlan P;
-    int rc = build_plan(n, k, L, ops, nops, params, P, relabel != 0);
-    if (rc && relabel) rc = build_plan(n, k, L, ops, nops, params, P, false);
+    int rc = plan_for(n, k, L, ops, nops, params, relabel, P);
     if (rc) return rc;
     *npasses = (int)P.passes.size();
     const uint64_t N = 1ULL << n;
@@ -152,8 +166,7 @@ extern "C" int emu_run_fused(int n, int k, int L, const qse_op* ops, int nops, c
 // permutation is a permutation, passes respect the tile size.
 extern "C" int emu_check_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, int relabel) {
     HostPlan P;
-    int rc = build_plan(n, k, L, ops, nops, params, P, relabel != 0);
-    if (rc && relabel) rc = build_plan(n, k, L, ops, nops, params, P, false);
+    int rc = plan_for(n, k, L, ops, nops, params, relabel, P);
     if (rc) return -1;
     for (const FPass& fp : P.passes) {
         if (fp.k != k || fp.next != n - k) return 1;
@@ -218,11 +231,9 @@ extern "C" int emu_check_plan(int n, int k, int L, const qse_op* ops, int nops, 
 // Plan summary for diagnostics: per pass, stages, dense ops, MONO ops, terms,
 // alpha vectors and slots.
 extern "C" int emu_plan_stats(int n, int k, int L, const qse_op* ops, int nops, const double* params, int* out,
-                              int max_passes) {
+                              int max_passes, int flags) {
     HostPlan P;
-    int rc = build_plan(n, k, L, ops, nops, params, P, true);
-    if (rc) rc = build_plan(n, k, L, ops, nops, params, P, false);
-    if (rc) return -1;
+    if (plan_for(n, k, L, ops, nops, params, flags, P)) return -1;
     int np = 0;
     for (const FPass& fp : P.passes) {
         if (np >= max_passes) break;
@@ -260,8 +271,7 @@ static int g_jit_n = 0;
 extern "C" long emu_jit_prepare(int n, int k, int L, const qse_op* ops, int nops, const double* params, int relabel,
                                 int prec, char* out, long cap) {
     g_jit_plan = HostPlan();
-    int rc = build_plan(n, k, L, ops, nops, params, g_jit_plan, relabel != 0);
-    if (rc && relabel) rc = build_plan(n, k, L, ops, nops, params, g_jit_plan, false);
+    int rc = plan_for(n, k, L, ops, nops, params, relabel, g_jit_plan);
     if (rc) return -rc;
     g_jit_n = n;
     const std::string src = jit_source(g_jit_plan, prec);

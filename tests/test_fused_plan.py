@@ -27,10 +27,15 @@ def emu():
     return _emu
 
 
+# planner variants: bit 0 relabelling SWAPs, bit 1 dependency-graph scheduling;
+# -1 is the library's choice among them
+VARIANTS = (0, 1, 2, 3, -1)
+
+
 def run(n, k, L, ops, psi, params=None, relabel=None):
-    """Emulated fused run; with relabel=None both planner modes must agree."""
+    """Emulated fused run; with relabel=None every planner variant must agree."""
     outs = []
-    for rl in ((0, 1) if relabel is None else (relabel,)):
+    for rl in (VARIANTS if relabel is None else (relabel,)):
         s = np.ascontiguousarray(psi, dtype=np.complex128).copy()
         p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
         npass = C.c_int(0)
@@ -39,8 +44,8 @@ def run(n, k, L, ops, psi, params=None, relabel=None):
         rc = emu().emu_run_fused(n, k, L, ops.ctypes.data, len(ops), pp, s.ctypes.data, C.byref(npass), rl)
         assert rc == 0
         outs.append((s, npass.value))
-    if len(outs) == 2:
-        assert np.max(np.abs(outs[0][0] - outs[1][0])) < 1e-13
+    for o in outs[1:]:
+        assert np.max(np.abs(outs[0][0] - o[0])) < 1e-12
     return outs[-1]
 
 
@@ -109,10 +114,44 @@ def test_pass_counts():
     lib = capi.load()
     p = C.c_int(0)
 
-    def passes(n, prec, ops):
-        capi.check(lib.qse_plan_passes(None, n, prec, ops.ctypes.data, len(ops), C.byref(p)))
-        return p.value
-    # QFT-30 complex128 (k=12, L=4): rows 0-11, 12-19, 20-27, rows 28-29 + the bit reversal
-    assert passes(30, capi.C128, G.qft(30).ops()) == 4
-    for rl in (0, 1):
-        assert emu().emu_check_plan(30, 12, 4, G.qft(30).ops().ctypes.data, 480, None, rl) == 0
+    def info(n, prec, ops):
+        v = [C.c_int(0) for _ in range(4)]
+        capi.check(lib.qse_plan_info(None, n, prec, ops.ctypes.data, len(ops), *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v[:3])
+
+    # QFT-30, tuned shape: complex128 12-qubit tiles with runs of 8 amplitudes:
+    # rows 0-11, rows 12-20, rows 21-29 with the final bit reversal folded into
+    # the last pass's store (relabelled SWAPs); complex64 13-qubit tiles, runs of 16
+    qft = G.qft(30).ops()
+    assert info(30, capi.C128, qft) == (3, 12, 3)
+    assert info(30, capi.C64, qft) == (3, 13, 4)
+    capi.check(lib.qse_plan_passes(None, 30, capi.C128, qft.ctypes.data, len(qft), C.byref(p)))
+    assert p.value == 3
+    # the 30-qubit random layered circuit (20 layers): dependency scheduling
+    assert info(30, capi.C128, G.random_layered(30, 20, seed=7).ops()) == (29, 12, 3)
+    for rl in (0, 1, 2, 3):
+        assert emu().emu_check_plan(30, 12, 3, qft.ctypes.data, 480, None, rl) == 0
+
+
+@pytest.mark.parametrize("seed", [11, 12, 13, 14])
+def test_dependency_scheduling_random_mixed(seed):
+    # out-of-order passes must respect every non-commuting pair: random gates of
+    # every kind (controls, multi-controlled phases, SWAPs, non-unitary 2x2)
+    n = 12
+    ops = G.random_mixed(n, 400, seed).ops()
+    psi = G.random_state(n, seed)
+    got, _ = run(n, 9, 3, ops, psi)
+    assert np.max(np.abs(got - O.run_ops(psi, n, ops))) < 1e-11
+
+
+def test_dependency_scheduling_cuts_passes():
+    # 20-qubit random layered circuit, 20 layers, 12-qubit tiles with 3 low qubits:
+    # in gate order one layer needs about two passes; list scheduling covers
+    # light cones of several layers per pass
+    lib = emu()
+    lib.emu_plan_stats.argtypes = [C.c_int] * 3 + [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+    ops = G.random_layered(20, 20, seed=7).ops()
+    out = np.zeros((256, 6), np.int32)
+    counts = {f: lib.emu_plan_stats(20, 12, 3, ops.ctypes.data, len(ops), None, out.ctypes.data, 256, f)
+              for f in (0, 2, -1)}
+    assert counts[0] == 39 and counts[2] <= 14 and counts[-1] == counts[2], counts

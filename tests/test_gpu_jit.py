@@ -101,3 +101,19 @@ def test_set_jit_rejects_bad_mode(ctx):
     with pytest.raises(capi.QSEError):
         ctx.set_jit(3)
     ctx.set_jit(capi.JIT_AUTO)
+
+
+def test_same_program_both_precisions(ctx):
+    # both precisions can plan a circuit identically; the compiled kernels of
+    # one precision must never run on a state of the other
+    n = 16
+    ops = G.random_layered(n, 8, seed=31).ops()
+    assert ctx.plan_info(n, capi.C128, ops)["tile_bits"] == ctx.plan_info(n, capi.C64, ops)["tile_bits"]
+    want = O.run_ops(O.basis(n), n, ops)
+    ctx.set_jit(capi.JIT_AUTO)
+    for prec, tol in ((capi.C128, 1e-12), (capi.C64, 2e-5), (capi.C128, 1e-12)):
+        for _ in range(2):
+            st = ctx.state(n, prec)
+            st.run(ops)
+            assert np.max(np.abs(st.download() - want)) < tol, prec
+            st.close()

@@ -58,7 +58,7 @@ def _compile_host(src, npasses, tmp):
     return lib
 
 
-def run_jit(n, k, L, ops, psi, params=None, relabel=1, prec=0):
+def run_jit(n, k, L, ops, psi, params=None, relabel=-1, prec=0):
     """Generated program executed on the host; returns the final state (complex128)."""
     pp = None if params is None else np.ascontiguousarray(params, np.float64)
     assert emu().emu_check_plan(n, k, L, ops.ctypes.data, len(ops), None if pp is None else pp.ctypes.data,
@@ -83,7 +83,7 @@ def run_jit(n, k, L, ops, psi, params=None, relabel=1, prec=0):
     return cur.astype(np.complex128), src, passes
 
 
-def check(n, k, L, ops, params=None, prec=0, seed=3, relabels=(0, 1)):
+def check(n, k, L, ops, params=None, prec=0, seed=3, relabels=(-1, 0)):
     psi = G.random_state(n, seed)
     want = O.run_ops(psi, n, ops, params)
     tol = 1e-12 if prec == 0 else 2e-5
@@ -96,19 +96,19 @@ def check(n, k, L, ops, params=None, prec=0, seed=3, relabels=(0, 1)):
 def test_qft_and_iqft():
     n = 12
     check(n, 9, 3, G.qft(n).ops())
-    check(n, 9, 3, G.iqft(n).ops(), relabels=(1,))
+    check(n, 9, 3, G.iqft(n).ops(), relabels=(1, 3))
 
 
 def test_random_layered_shapes():
     for (n, k, L) in ((13, 10, 2), (14, 9, 5), (12, 8, 0)):
-        check(n, k, L, G.random_layered(n, 6, seed=n + k).ops(), relabels=(1,))
+        check(n, k, L, G.random_layered(n, 6, seed=n + k).ops(), relabels=(-1,))
 
 
 def test_every_gate_kind():
     n = 11
     ops = G.random_mixed(n, 300, 5).ops()
     check(n, 8, 4, ops, seed=6)
-    check(n, 9, 0, ops, seed=6, relabels=(1,))
+    check(n, 9, 0, ops, seed=6, relabels=(2, 3))
 
 
 def test_qaoa_slots_and_terms():
@@ -117,18 +117,18 @@ def test_qaoa_slots_and_terms():
     edges = G.random_regular_graph(n, 3, seed=1)
     ops = G.qaoa_maxcut(n, edges, 3).ops()
     params = np.random.default_rng(2).random(6) * math.pi
-    check(n, 8, 2, ops, params, relabels=(1,))
+    check(n, 8, 2, ops, params, relabels=(-1,))
 
 
 def test_complex64():
     n = 12
-    check(n, 9, 3, G.random_layered(n, 5, seed=9).ops(), prec=1, relabels=(1,))
-    check(n, 9, 3, G.qft(n).ops(), prec=1, relabels=(0,))
+    check(n, 9, 3, G.random_layered(n, 5, seed=9).ops(), prec=1, relabels=(-1,))
+    check(n, 9, 3, G.qft(n).ops(), prec=1, relabels=(-1,))
 
 
 def test_default_shape_qft14():
     # complex128 default shape: 12 tile qubits, 6 contiguous low qubits
-    check(14, 12, 6, G.qft(14).ops(), relabels=(1,))
+    check(14, 12, 6, G.qft(14).ops(), relabels=(-1,))
 
 
 def test_hiprtc_compiles_for_gfx950():
@@ -137,7 +137,7 @@ def test_hiprtc_compiles_for_gfx950():
         pytest.skip("hiprtc not installed")
     rtc = C.CDLL(rtc_path)
     for prec in (0, 1):
-        src = _prepare(14, 12, 6, G.qft(14).ops(), None, 1, prec)
+        src = _prepare(14, 12, 6, G.qft(14).ops(), None, -1, prec)
         prog = C.c_void_p()
         assert rtc.hiprtcCreateProgram(C.byref(prog), src.encode(), b"t.hip", 0, None, None) == 0
         opts = [b"--offload-arch=gfx950", b"-O3", b"-Xclang", b"-target-feature", b"-Xclang", b"-packed-fp32-ops"]
