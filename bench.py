@@ -135,6 +135,9 @@ def main() -> int:
     ap.add_argument("--no-qaoa", action="store_true")
     ap.add_argument("--qaoa-batch", type=int, default=64)
     ap.add_argument("--qaoa-shift-sets", type=int, default=1)
+    ap.add_argument("--verify", type=int, default=16, metavar="K",
+                    help="after timing, rerun one QFT from the seed and compare K output amplitudes with the FP64 "
+                         "DFT closed form evaluated on the GPU (qse_qft_probe)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run X on qubit 0 (one read + one write of every amplitude) for counter calibration")
     args = ap.parse_args()
@@ -236,6 +239,22 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": bytes_per_launch, "mean_launch_ms": mean_ms},
         "pct_hbm_step": (bytes_per_launch * dom["launches"] / args.steps) / (ms_step * 1e-3) / HBM_PEAK,
     }
+    if args.verify > 0:
+        # one QFT of the seeded state; QFT^k is not QFT for the timed k steps
+        st.set_random(12345)
+        st.run(ops, fused=fused)
+        rng = np.random.default_rng(n)
+        ks = np.concatenate([np.array([0, (1 << n) - 1], dtype=np.uint64),
+                             rng.integers(0, 1 << n, size=max(args.verify - 2, 0), dtype=np.uint64)])[:args.verify]
+        got = np.array([st.download_range(int(k), 1)[0] for k in ks])
+        t0 = time.perf_counter()
+        want = ctx.qft_probe(n, 12345, ks)
+        log(f"closed-form probe of {len(ks)} amplitudes: {time.perf_counter() - t0:.2f} s")
+        err = np.abs(got - want)
+        result["verify"] = {"amplitudes": int(len(ks)), "max_abs_err": float(err.max()),
+                            "max_rel_err": float(err.max() / np.abs(want).max()),
+                            "reference": "FP64 DFT closed form from the seed (qse_qft_probe), one QFT of the "
+                                         "seeded random state"}
     st.close()
 
     if not args.no_qaoa:

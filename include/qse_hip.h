@@ -203,6 +203,15 @@ int qse_plan_passes(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops,
 int qse_plan_info(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops, int nops, int* passes,
                   int* tile_bits, int* low_bits, int* stages);
 
+/* FP64 closed-form check of a full-size QFT run (no counterpart in the
+ * reference): out[2c], out[2c+1] = <indices[c]| QFT |psi> for the normalised
+ * state that qse_state_set_random(seed) produces on an nqubits state, with the
+ * QFT of QFT.cpp:9-22 (= bit-reversed unitary DFT, exponent +2 pi i x k / N).
+ * Evaluated from the seed by one FP64 sweep per 16 indices, without a state
+ * vector, so it can check runs whose input no longer fits beside the output.
+ * count <= 1024; synchronises. */
+int qse_qft_probe(qse_ctx* ctx, int nqubits, uint64_t seed, const uint64_t* indices, int count, double* out);
+
 /* ---- readout (StateVector.cpp:77-156), driven by host-supplied uniforms ----
  * u are canonical uniforms in [0,1) (std::generate_canonical<double,53>); the
  * target of shot s is u[s] * total, total = sum |a_i|^2, and the outcome is the

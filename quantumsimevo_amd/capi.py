@@ -39,7 +39,7 @@ EXPORTS = [
     "qse_state_download_range", "qse_state_set_basis", "qse_state_copy", "qse_state_set_random", "qse_apply_matrix",
     "qse_apply_hadamard", "qse_apply_x", "qse_apply_y", "qse_apply_z", "qse_apply_phase", "qse_apply_rx",
     "qse_apply_ry", "qse_apply_rz", "qse_apply_cnot", "qse_apply_cphase", "qse_apply_mcphase",
-    "qse_apply_swap", "qse_pauli_frame", "qse_run", "qse_run_sweeps", "qse_plan_passes", "qse_plan_info", "qse_sample",
+    "qse_apply_swap", "qse_pauli_frame", "qse_run", "qse_run_sweeps", "qse_plan_passes", "qse_plan_info", "qse_qft_probe", "qse_sample",
     "qse_measure", "qse_norm2", "qse_prob_one", "qse_collapse_qubit", "qse_expect_zstrings",
     "qse_adjoint_expect", "qse_batch_expect", "qse_batch_run_small", "qse_sample_frames",
 ]
@@ -109,6 +109,7 @@ def _sigs(lib):
         "qse_run_sweeps": ([_vp, _vp, _vp, _i, _vp], _i),
         "qse_plan_passes": ([_vp, _i, _i, _vp, _i, C.POINTER(_i)], _i),
         "qse_plan_info": ([_vp, _i, _i, _vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)], _i),
+        "qse_qft_probe": ([_vp, _i, C.c_uint64, _vp, _i, _vp], _i),
         "qse_sample": ([_vp, _vp, _vp, _i64, _vp], _i),
         "qse_measure": ([_vp, _vp, _d, _pu64], _i),
         "qse_norm2": ([_vp, _vp, _pd], _i),
@@ -241,6 +242,14 @@ class Context:
         check(self.lib.qse_plan_info(self.h, nqubits, precision, _ptr(a), len(a), *[C.byref(x) for x in v]),
               "qse_plan_info")
         return {"passes": v[0].value, "tile_bits": v[1].value, "low_bits": v[2].value, "stages": v[3].value}
+
+    def qft_probe(self, nqubits: int, seed: int, indices) -> np.ndarray:
+        """FP64 closed-form <k|QFT|psi_seed> for the normalised random state of
+        State.set_random(seed) (complex128 array, one entry per index)."""
+        idx = np.ascontiguousarray(indices, dtype=np.uint64)
+        out = np.empty(2 * len(idx))
+        check(self.lib.qse_qft_probe(self.h, nqubits, seed, _ptr(idx), len(idx), _ptr(out)), "qse_qft_probe")
+        return out.view(np.complex128)
 
     # ---- batched entry points ----
     def adjoint_expect(self, nqubits, precision, ops, params, obs_masks, obs_weights, ckpt_every=0):

@@ -1,0 +1,142 @@
+// qse_check.hip — closed-form checks of full-size runs, evaluated on the GPU
+// without a second state vector.
+//
+// A 33-qubit complex128 or 34-qubit complex64 state (128 GiB) leaves no room
+// on the device, and far too little on the host, to keep the input of a QFT
+// for comparison. The QFT of QFT.cpp:9-22 (H on qubit 0 first, final swaps)
+// is the bit-reversed unitary DFT with positive exponent:
+//     out[k] = (1/sqrt(N S)) sum_j psi[j] exp(+2 pi i rev(j) rev(k) / N),
+// rev = n-bit reversal, S = sum_j |psi[j]|^2, and the input psi of
+// qse_state_set_random(seed) is a pure function of (seed, j) (random_amp), so
+// any output amplitude can be evaluated in FP64 from the seed alone: one sweep
+// over j regenerates psi[j] and accumulates every requested k at once.
+#include <cmath>
+#include <vector>
+
+#include "qse_device.hpp"
+#include "qse_internal.hpp"
+
+namespace qse {
+
+namespace {
+
+constexpr int PROBE_K = 16;       // output amplitudes per sweep
+constexpr int PROBE_CHUNK = 1024;  // consecutive y = rev(j) per thread between exact re-anchoring
+constexpr int PROBE_BLOCK = 256;
+constexpr int PROBE_VALS = 2 * PROBE_K + 1;  // per block: K complex sums and S
+
+// Per block: sums over its chunks of psi[j] * exp(2 pi i y rk[c] / N) for
+// y = rev(j) (the twiddle advances by a fixed factor along y and is evaluated
+// exactly at each chunk start), and of |psi[j]|^2.
+__global__ __launch_bounds__(PROBE_BLOCK) void k_qft_probe(int n, uint64_t seed, const uint64_t* __restrict__ rk,
+                                                           int K, uint64_t chunk, double* __restrict__ partial) {
+    const uint64_t N = 1ULL << n, mask = N - 1;
+    const double inv_half_n = 2.0 / (double)N;  // sincospi argument per unit of (y * rk mod N)
+    double wr[PROBE_K], wi[PROBE_K], ar[PROBE_K], ai[PROBE_K];
+#pragma unroll
+    for (int c = 0; c < PROBE_K; ++c) {
+        ar[c] = ai[c] = 0.0;
+        const uint64_t r = c < K ? rk[c] : 0;
+        sincospi((double)(r & mask) * inv_half_n, &wi[c], &wr[c]);
+    }
+    double s = 0.0;
+    const uint64_t nchunks = N / chunk;
+    for (uint64_t ch = gtid(); ch < nchunks; ch += gstride()) {
+        const uint64_t y0 = ch * chunk;
+        double zr[PROBE_K], zi[PROBE_K];
+#pragma unroll
+        for (int c = 0; c < PROBE_K; ++c) {
+            const uint64_t r = c < K ? rk[c] : 0;
+            sincospi((double)((y0 * r) & mask) * inv_half_n, &zi[c], &zr[c]);
+        }
+        for (uint64_t t = 0; t < chunk; ++t) {
+            const uint64_t y = y0 + t;
+            const uint64_t j = __builtin_bitreverse64(y) >> (64 - n);
+            double pr, pi;
+            random_amp(seed, j, pr, pi);
+            s += pr * pr + pi * pi;
+#pragma unroll
+            for (int c = 0; c < PROBE_K; ++c) {
+                ar[c] += pr * zr[c] - pi * zi[c];
+                ai[c] += pr * zi[c] + pi * zr[c];
+                const double nr = zr[c] * wr[c] - zi[c] * wi[c];
+                zi[c] = zr[c] * wi[c] + zi[c] * wr[c];
+                zr[c] = nr;
+            }
+        }
+    }
+    __shared__ double red[PROBE_BLOCK / 64][PROBE_VALS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < PROBE_K; ++c) {
+        const double a = wave_sum(ar[c]), b = wave_sum(ai[c]);
+        if (lane == 0) {
+            red[wave][2 * c] = a;
+            red[wave][2 * c + 1] = b;
+        }
+    }
+    const double sw = wave_sum(s);
+    if (lane == 0) red[wave][2 * PROBE_K] = sw;
+    __syncthreads();
+    if (threadIdx.x < PROBE_VALS) {
+        double v = 0.0;
+        for (int w = 0; w < PROBE_BLOCK / 64; ++w) v += red[w][threadIdx.x];
+        partial[(uint64_t)blockIdx.x * PROBE_VALS + threadIdx.x] = v;
+    }
+}
+
+uint64_t bitrev(uint64_t v, int n) {
+    uint64_t r = 0;
+    for (int b = 0; b < n; ++b)
+        if ((v >> b) & 1) r |= 1ULL << (n - 1 - b);
+    return r;
+}
+
+}  // namespace
+
+}  // namespace qse
+
+using namespace qse;
+
+extern "C" int qse_qft_probe(qse_ctx* ctx, int nqubits, uint64_t seed, const uint64_t* indices, int count,
+                             double* out) {
+    if (!ctx || !indices || !out || count < 1 || count > 1024 || nqubits < 1 || nqubits > 40)
+        return set_error(QSE_ERR_INVALID_ARG, "bad argument");
+    const uint64_t N = 1ULL << nqubits;
+    for (int i = 0; i < count; ++i)
+        if (indices[i] >= N) return set_error(QSE_ERR_INVALID_ARG, "index out of range");
+    QSE_HIP(hipSetDevice(ctx->device));
+    const uint64_t chunk = std::min<uint64_t>(PROBE_CHUNK, N);
+    const uint64_t nchunks = N / chunk;
+    // one chunk per thread at most, at most 8 blocks per CU (grid-stride beyond)
+    const unsigned blocks = (unsigned)std::min<uint64_t>((nchunks + PROBE_BLOCK - 1) / PROBE_BLOCK,
+                                                         (uint64_t)ctx->num_cus * 8);
+    void* dptr = nullptr;
+    int rc = scratch(ctx, 6, sizeof(uint64_t) * PROBE_K + sizeof(double) * (size_t)blocks * PROBE_VALS, &dptr);
+    if (rc) return rc;
+    uint64_t* d_rk = (uint64_t*)dptr;
+    double* d_part = (double*)((char*)dptr + sizeof(uint64_t) * PROBE_K);
+    std::vector<double> part((size_t)blocks * PROBE_VALS);
+    for (int first = 0; first < count; first += PROBE_K) {
+        const int K = std::min(PROBE_K, count - first);
+        uint64_t rk[PROBE_K] = {0};
+        for (int c = 0; c < K; ++c) rk[c] = bitrev(indices[first + c], nqubits);
+        QSE_HIP(hipMemcpyAsync(d_rk, rk, sizeof(rk), hipMemcpyHostToDevice, ctx->stream));
+        k_qft_probe<<<blocks, PROBE_BLOCK, 0, ctx->stream>>>(nqubits, seed, d_rk, K, chunk, d_part);
+        QSE_HIP(hipGetLastError());
+        QSE_HIP(hipMemcpyAsync(part.data(), d_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        QSE_HIP(hipStreamSynchronize(ctx->stream));
+        double S = 0.0;
+        std::vector<double> acc(2 * K, 0.0);
+        for (unsigned b = 0; b < blocks; ++b) {
+            const double* v = &part[(size_t)b * PROBE_VALS];
+            for (int c = 0; c < 2 * K; ++c) acc[c] += v[c];
+            S += v[2 * PROBE_K];
+        }
+        // unitary DFT (1/sqrt N) of the normalised input (1/sqrt S)
+        const double f = 1.0 / (std::sqrt(S) * std::sqrt((double)N));
+        for (int c = 0; c < 2 * K; ++c) out[2 * first + c] = acc[c] * f;
+    }
+    return QSE_OK;
+}

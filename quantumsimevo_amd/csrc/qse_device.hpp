@@ -70,6 +70,30 @@ __device__ __forceinline__ uint64_t deposit(uint64_t k, const Deposit& d) {
 __device__ __forceinline__ uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
 
+// Seeded complex Gaussian amplitude of index i (before normalisation): two
+// splitmix64 draws from (seed, i), Box-Muller in FP64. A pure function of
+// (seed, i), so a kernel can regenerate any amplitude of a seeded random state
+// without reading it.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ void random_amp(uint64_t seed, uint64_t i, double& re, double& im) {
+    const double two_pi = 6.283185307179586476925;
+    const uint64_t h1 = splitmix64(seed ^ ((2 * i + 1) * 0x9E3779B97F4A7C15ULL));
+    const uint64_t h2 = splitmix64(h1);
+    const double u1 = (double)((h1 >> 11) + 1) * 0x1.0p-53;
+    const double u2 = (double)(h2 >> 11) * 0x1.0p-53;
+    const double r = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincos(two_pi * u2, &sn, &cs);
+    re = r * cs;
+    im = r * sn;
+}
+
 // Wave64 sum of a double with cross-lane shuffles.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

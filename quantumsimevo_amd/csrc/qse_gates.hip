@@ -166,25 +166,12 @@ int launch_convert_download(qse_ctx* ctx, const qse_state* st, double* dev_c128,
     return QSE_OK;
 }
 
-__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
-    z += 0x9E3779B97F4A7C15ULL;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    return z ^ (z >> 31);
-}
-
 template <class R>
 __global__ __launch_bounds__(256) void k_random(cx<R>* __restrict__ s, uint64_t n, uint64_t seed) {
-    const double two_pi = 6.283185307179586476925;
     for (uint64_t i = gtid(); i < n; i += gstride()) {
-        const uint64_t h1 = splitmix64(seed ^ ((2 * i + 1) * 0x9E3779B97F4A7C15ULL));
-        const uint64_t h2 = splitmix64(h1);
-        const double u1 = (double)((h1 >> 11) + 1) * 0x1.0p-53;
-        const double u2 = (double)(h2 >> 11) * 0x1.0p-53;
-        const double r = sqrt(-2.0 * log(u1));
-        double sn, cs;
-        sincos(two_pi * u2, &sn, &cs);
-        s[i] = {(R)(r * cs), (R)(r * sn)};
+        double re, im;
+        random_amp(seed, i, re, im);
+        s[i] = {(R)re, (R)im};
     }
 }
 
