@@ -140,7 +140,7 @@ int qse_ctx_set_jit(qse_ctx* ctx, int mode);
 int qse_ctx_jit_stats(qse_ctx* ctx, int64_t* compiled, int64_t* jit_launches, double* compile_ms);
 /* Kernel timing: when enabled, every kernel launched by the context records HIP
  * events; qse_ctx_timing() returns the count and summed milliseconds per kernel
- * family. Families: 0 pair sweep, 1 fused tile pass, 2 readout, 3 adjoint. */
+ * family. Families: 0 pair sweep, 1 fused tile pass, 2 readout, 3 adjoint, 4 batched circuits. */
 int qse_ctx_enable_timing(qse_ctx* ctx, int enable);
 int qse_ctx_timing(qse_ctx* ctx, int family, int64_t* launches, double* total_ms, double* bytes);
 int qse_ctx_reset_timing(qse_ctx* ctx);
@@ -260,12 +260,17 @@ int qse_batch_expect(qse_ctx* ctx, int nqubits, int precision,
                      double* energies);
 
 /* ---- batched circuits of uneven sizes in one persistent launch ----
- * ncirc circuits; circuit c has nq[c] qubits (<= 16 in this entry point: the
- * whole state lives in one workgroup's LDS) and the ops ops[op_off[c] ..
- * op_off[c+1]). Each circuit starts from |0...0>, runs its ops, and returns
- * E_c = <Z-string observable> (same masks/weights for every circuit, masks
- * restricted to the circuit's qubits) and, when grads != NULL, dE_c/dparams
- * for params[par_off[c] .. par_off[c+1]). */
+ * (no counterpart in the reference). ncirc circuits; circuit c has nq[c]
+ * qubits (8..30; its states live in HBM) and the ops ops[op_off[c] ..
+ * op_off[c+1]), whose param indices refer to params[par_off[c] ..
+ * par_off[c+1]). Each circuit starts from |0...0>, runs its ops, and returns
+ * energies[c] = <psi|O|psi> with O = sum_t w_t Z-string(m_t) over the terms
+ * whose qubits all lie below nq[c] (at most 64 terms, the same list for every
+ * circuit); with grads != NULL also grads[par_off[c] + j] = dE_c/dparams[..+j]
+ * by the adjoint method (parameterised kinds as qse_adjoint_expect). One
+ * workgroup runs each circuit's forward passes, observable, reductions and
+ * reverse sweep; circuits are taken largest first from a device work queue.
+ * Complex64 states accumulate energies and gradients in FP64. Synchronises. */
 int qse_batch_run_small(qse_ctx* ctx, int precision, int ncirc,
                         const int* nq, const int64_t* op_off, const qse_op* ops,
                         const int64_t* par_off, const double* params,

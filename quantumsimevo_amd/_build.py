@@ -33,7 +33,8 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 LIB_SOURCES = ["qse_capi.cpp", "qse_run.cpp", "qse_gates.hip", "qse_readout.hip", "qse_fused.hip",
-               "qse_plan.cpp", "qse_adjoint.hip", "qse_jit.cpp", "qse_jit_gen.cpp", "qse_check.hip"]
+               "qse_plan.cpp", "qse_adjoint.hip", "qse_jit.cpp", "qse_jit_gen.cpp", "qse_check.hip",
+               "qse_adjoint_plan.cpp", "qse_batch_plan.cpp", "qse_batch.hip"]
 HOST_SOURCES = ["StateVector.cpp", "Noise.cpp", "CircuitUnitaryOperationHIP.cpp",
                 "CircuitUnitaryOperationFactory.cpp", "Circuit.cpp", "HardwareCheck.cpp"]
 
@@ -110,7 +111,8 @@ def build_test_emulator(verbose: bool = False) -> Path:
     out = out_dir / "libfused_emu.so"
     _run([os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__",
           "-I/opt/rocm/include", f"-I{INCLUDE}", f"-I{CSRC}", "-o", str(out),
-          str(ROOT / "tests" / "fused_emu.cpp"), str(CSRC / "qse_plan.cpp"), str(CSRC / "qse_jit_gen.cpp")], verbose)
+          str(ROOT / "tests" / "fused_emu.cpp"), str(CSRC / "qse_plan.cpp"), str(CSRC / "qse_jit_gen.cpp"),
+          str(CSRC / "qse_adjoint_plan.cpp"), str(CSRC / "qse_batch_plan.cpp")], verbose)
     return out
 
 

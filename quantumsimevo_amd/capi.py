@@ -27,7 +27,7 @@ OP_DTYPE = np.dtype([("kind", "<i4"), ("target", "<i4"), ("control", "<i4"), ("p
                      ("controls", "<u8"), ("theta", "<f8"), ("scale", "<f8"), ("m", "<f8", (8,))])
 assert OP_DTYPE.itemsize == 104
 
-FAM_SWEEP, FAM_FUSED, FAM_READOUT, FAM_ADJOINT = range(4)
+FAM_SWEEP, FAM_FUSED, FAM_READOUT, FAM_ADJOINT, FAM_BATCH = range(5)
 JIT_OFF, JIT_AUTO, JIT_ALWAYS = range(3)
 
 # Every symbol include/qse_hip.h declares (checked by the CPU tests).
@@ -252,6 +252,22 @@ class Context:
         return out.view(np.complex128)
 
     # ---- batched entry points ----
+    def batch_run_small(self, precision, circuits, obs_masks, obs_weights, grads: bool = True):
+        """Many circuits [(n, ops, params), ...] of uneven sizes in one persistent
+        launch: energies (ncirc,) and, with grads, a list of gradient vectors."""
+        from .circuits import pack_batch
+        nq, op_off, ops, par_off, params = pack_batch(circuits)
+        ops = as_ops(ops) if len(ops) else np.zeros(0, dtype=OP_DTYPE)
+        m = np.ascontiguousarray(obs_masks, dtype=np.uint64)
+        w = np.ascontiguousarray(obs_weights, dtype=np.float64)
+        E = np.empty(len(circuits))
+        G = np.empty(max(len(params), 1)) if grads else None
+        check(self.lib.qse_batch_run_small(self.h, precision, len(circuits), _ptr(nq), _ptr(op_off), _ptr(ops),
+                                           _ptr(par_off), _ptr(params) if len(params) else None, _ptr(m), _ptr(w),
+                                           len(m), _ptr(E), _ptr(G) if grads else None), "qse_batch_run_small")
+        if not grads:
+            return E
+        return E, [G[par_off[i]:par_off[i + 1]].copy() for i in range(len(circuits))]
     def adjoint_expect(self, nqubits, precision, ops, params, obs_masks, obs_weights, ckpt_every=0):
         a = as_ops(ops)
         P = np.ascontiguousarray(params, dtype=np.float64)

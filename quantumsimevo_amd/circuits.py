@@ -240,6 +240,34 @@ def ansatz(n: int, depth: int) -> GateList:
     return g
 
 
+def pack_batch(circuits):
+    """Flatten [(n, ops, params), ...] into the arrays of qse_batch_run_small:
+    nq (int32), op_off (int64, ncirc+1), ops, par_off (int64, ncirc+1), params."""
+    nq = np.array([c[0] for c in circuits], dtype=np.int32)
+    op_off = np.zeros(len(circuits) + 1, dtype=np.int64)
+    par_off = np.zeros(len(circuits) + 1, dtype=np.int64)
+    for i, (_, ops, par) in enumerate(circuits):
+        op_off[i + 1] = op_off[i] + len(ops)
+        par_off[i + 1] = par_off[i] + len(par)
+    ops = np.concatenate([np.asarray(c[1]) for c in circuits]) if circuits else np.zeros(0)
+    params = np.concatenate([np.asarray(c[2], dtype=np.float64) for c in circuits]) if circuits else np.zeros(0)
+    return nq, op_off, ops, par_off, params
+
+
+def uneven_ansatz_batch(count: int, seed: int, n_range=(10, 22), depth_range=(4, 12)):
+    """`count` ansatz circuits (see ansatz()): qubit counts uniform in n_range,
+    depths uniform in depth_range, angles uniform in [0, 2pi); returns
+    [(n, ops, params), ...] for pack_batch."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        depth = int(rng.integers(depth_range[0], depth_range[1] + 1))
+        ops = ansatz(n, depth).ops()
+        out.append((n, ops, rng.random(2 * n * depth) * 2.0 * math.pi))
+    return out
+
+
 def amp_updates(ops: np.ndarray, n: int) -> float:
     """Algorithmic amplitude updates sum_g T(g): N for H, X, Y, RX, RY, RZ,
     generic 2x2; N/2 for Z, phase, CNOT, SWAP; N/4 for CPhase; N/2^m for

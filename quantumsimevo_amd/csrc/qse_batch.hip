@@ -1,0 +1,505 @@
+// qse_batch.hip — batched small-circuit engine (qse_batch_run_small): many
+// circuits of uneven qubit counts, forward run, observable and adjoint
+// gradients, in one persistent launch.
+//
+// Circuits are taken from device work queues (in decreasing order of estimated
+// work) and each circuit's whole step list (qse_batch.hpp) is executed by one
+// team: fused passes over the tiles of the circuit's state in HBM (the
+// LDS-tiled pass of qse_fused.hip, in place), the observable, and FP64
+// reductions for the energy and the generator terms. Circuits of 18 qubits or
+// more are taken by teams of 16 workgroups that split every pass's tiles and
+// every reduction's indices and meet at team barriers (global counters); the
+// others by single workgroups, whose own barriers are the only
+// synchronisation. Reductions sum per-workgroup partials in rank order
+// (deterministic). Tiles of k < KMAX qubits use the first 2^(k-4) threads; the
+// others wait at the barriers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "qse_batch.hpp"
+#include "qse_device.hpp"
+#include "qse_fused_dev.hpp"
+#include "qse_internal.hpp"
+
+namespace qse {
+
+namespace {
+
+// Tile qubits of the engine's largest tile (both precisions): 32 or 64 KiB of
+// LDS per workgroup, two workgroups per CU.
+constexpr int BATCH_KMAX = 12;
+constexpr int BATCH_THREADS = 1 << (BATCH_KMAX - 4);
+
+struct BArgs {
+    const FPass* __restrict__ passes;
+    const FStage* __restrict__ stages;
+    const FOp* __restrict__ ops;
+    const FTerm* __restrict__ terms;
+    const FSlot* __restrict__ slots;
+    const FExtTerm* __restrict__ eterms;
+    const double* __restrict__ alphas;
+};
+
+// One in-place pass of tile size K over the 2^n amplitudes at s: tiles r, r + T, ...
+// (this workgroup's share when T workgroups work on the circuit).
+template <class R, int K>
+__device__ void batch_pass(cx<R>* s, int n, const FPass& P, const BArgs& A, cx<R>* tile, SlotVal* slotv, int r,
+                           int T) {
+    constexpr int NT = 1 << (K - 4);
+    constexpr int TB = K - 4;
+    const int tid = threadIdx.x;
+    const bool act = tid < NT;
+    uint64_t gt = 0, ot = 0;
+    uint32_t st_t = 0;
+    if (act) {
+#pragma unroll
+        for (int b = 0; b < TB; ++b)
+            if ((tid >> b) & 1) {
+                gt |= 1ULL << P.tq[b];
+                ot |= 1ULL << P.so_q[b];
+                st_t |= 1u << P.so_l[b];
+            }
+    }
+    const uint32_t swz_tid = swz((uint32_t)tid), swz_st = swz(st_t);
+    const uint64_t ntiles = 1ULL << (n - K);
+    for (uint64_t ti = (uint64_t)r; ti < ntiles; ti += (uint64_t)T) {
+        uint64_t base = 0, obase = 0;
+        {
+            uint64_t t = ti;
+            for (int e = 0; e < P.next; ++e) {
+                base |= (t & 1ULL) << P.epos[e];
+                obase |= (t & 1ULL) << P.eout[e];
+                t >>= 1;
+            }
+        }
+        cx<R> ld[16];
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                uint64_t g = base | gt;
+                if (i & 1) g |= 1ULL << P.tq[TB + 0];
+                if (i & 2) g |= 1ULL << P.tq[TB + 1];
+                if (i & 4) g |= 1ULL << P.tq[TB + 2];
+                if (i & 8) g |= 1ULL << P.tq[TB + 3];
+                ld[i] = s[g];
+            }
+        }
+        // per-tile external factors: angle sum of each slot's terms whose qubits are set
+        for (int sl = tid; sl < P.nslots; sl += blockDim.x) {
+            const FSlot S = A.slots[P.slot_begin + sl];
+            double a = 0.0;
+            for (uint32_t e = S.begin; e < S.end; ++e)
+                if ((base & A.eterms[e].emask) == A.eterms[e].emask) a += A.eterms[e].ang;
+            double sn, cs;
+            sincos(a, &sn, &cs);
+            slotv[sl] = {a, cs, sn, 0.0};
+        }
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) tile[swz_tid ^ swz((uint32_t)(i * NT))] = ld[i];
+        }
+        __syncthreads();
+        for (int sg = 0; sg < P.nstages; ++sg) {
+            const FStage& S = A.stages[P.stage_begin + sg];
+            if (act) {
+                uint32_t toff = 0;
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+                    if ((tid >> t) & 1) toff |= 1u << S.tmap[t];
+                const uint32_t r0 = swz(1u << S.rmap[0]), r1 = swz(1u << S.rmap[1]), r2 = swz(1u << S.rmap[2]),
+                               r3 = swz(1u << S.rmap[3]);
+                const uint32_t st_off = swz(toff);
+                cx<R> a[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    a[j] = tile[st_off ^ ((j & 1) ? r0 : 0u) ^ ((j & 2) ? r1 : 0u) ^ ((j & 4) ? r2 : 0u) ^
+                                ((j & 8) ? r3 : 0u)];
+                for (uint32_t oi = S.op_begin; oi < S.op_end; ++oi) {
+                    const FOp& op = A.ops[oi];
+                    if (op.type == FOP_MONO) {
+                        double gr, gi;
+                        mono_factor<TB>(op, A.terms, A.alphas, tid, toff, base, slotv, gr, gi);
+                        const cx<R> g = {(R)gr, (R)gi};
+                        const uint32_t rctl = op.rctl;
+#pragma unroll
+                        for (int j = 0; j < 16; ++j)
+                            if (((uint32_t)j & rctl) == rctl) a[j] = cmul(g, a[j]);
+                    } else if (op.type == FOP_H) {
+                        switch (op.p) {
+                            case 0: hadamard_p<R, 0>(a); break;
+                            case 1: hadamard_p<R, 1>(a); break;
+                            case 2: hadamard_p<R, 2>(a); break;
+                            default: hadamard_p<R, 3>(a); break;
+                        }
+                    } else if (op.type == FOP_REAL) {
+                        switch (op.p) {
+                            case 0: real_p<R, 0>(a, op); break;
+                            case 1: real_p<R, 1>(a, op); break;
+                            case 2: real_p<R, 2>(a, op); break;
+                            default: real_p<R, 3>(a, op); break;
+                        }
+                    } else if (op.type == FOP_RXLIKE) {
+                        switch (op.p) {
+                            case 0: rxlike_p<R, 0>(a, op); break;
+                            case 1: rxlike_p<R, 1>(a, op); break;
+                            case 2: rxlike_p<R, 2>(a, op); break;
+                            default: rxlike_p<R, 3>(a, op); break;
+                        }
+                    } else if (op.type == FOP_XSWAP && op.rctl == 0 && op.tctl == 0 && op.ectl == 0) {
+                        switch (op.p) {
+                            case 0: xswap_p<R, 0>(a); break;
+                            case 1: xswap_p<R, 1>(a); break;
+                            case 2: xswap_p<R, 2>(a); break;
+                            default: xswap_p<R, 3>(a); break;
+                        }
+                    } else if (((toff & op.tctl) == op.tctl) && ((base & op.ectl) == op.ectl)) {
+                        switch (op.p) {
+                            case 0: dense_p<R, 0>(a, op); break;
+                            case 1: dense_p<R, 1>(a, op); break;
+                            case 2: dense_p<R, 2>(a, op); break;
+                            default: dense_p<R, 3>(a, op); break;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    tile[st_off ^ ((j & 1) ? r0 : 0u) ^ ((j & 2) ? r1 : 0u) ^ ((j & 4) ? r2 : 0u) ^
+                         ((j & 8) ? r3 : 0u)] = a[j];
+            }
+            __syncthreads();
+        }
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                uint64_t g = obase | ot;
+                uint32_t ls = swz_st;
+                if (i & 1) { g |= 1ULL << P.so_q[TB + 0]; ls ^= swz(1u << P.so_l[TB + 0]); }
+                if (i & 2) { g |= 1ULL << P.so_q[TB + 1]; ls ^= swz(1u << P.so_l[TB + 1]); }
+                if (i & 4) { g |= 1ULL << P.so_q[TB + 2]; ls ^= swz(1u << P.so_l[TB + 2]); }
+                if (i & 8) { g |= 1ULL << P.so_q[TB + 3]; ls ^= swz(1u << P.so_l[TB + 3]); }
+                s[g] = tile[ls];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <class R>
+__device__ void batch_pass_k(int k, cx<R>* s, int n, const FPass& P, const BArgs& A, cx<R>* tile, SlotVal* slotv,
+                             int r, int T) {
+    switch (k) {
+        case 8: batch_pass<R, 8>(s, n, P, A, tile, slotv, r, T); break;
+        case 9: batch_pass<R, 9>(s, n, P, A, tile, slotv, r, T); break;
+        case 10: batch_pass<R, 10>(s, n, P, A, tile, slotv, r, T); break;
+        case 11: batch_pass<R, 11>(s, n, P, A, tile, slotv, r, T); break;
+        default: batch_pass<R, 12>(s, n, P, A, tile, slotv, r, T); break;
+    }
+}
+
+// Sum of v over the workgroup (every thread calls; thread 0 gets the result).
+__device__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    __syncthreads();
+    return t;
+}
+
+__device__ double obs_factor(uint64_t i, uint64_t keep, const uint64_t* __restrict__ m, const double* __restrict__ w,
+                             int nterms) {
+    double f = 0.0;
+    for (int t = 0; t < nterms; ++t)
+        if ((keep >> t) & 1) f += (__popcll(i & m[t]) & 1) ? -w[t] : w[t];
+    return f;
+}
+
+// Bound on the iterations of one barrier wait (each sleeps ~128 cycles): about
+// a second, far beyond any legitimate wait for a teammate's share of a step.
+constexpr uint32_t TEAM_SPIN_LIMIT = 1u << 23;
+
+// T workgroups working on one circuit (T = 1: a single workgroup).
+struct Team {
+    int id, r, T;  // team index, rank of this workgroup in it, team size
+    int* bar;      // the team's barrier counter (global memory)
+    int gen;       // barriers passed (thread 0's copy is the one used)
+    int* err;      // set when a wait times out
+};
+
+// Barrier over the team's workgroups (just a workgroup barrier when T = 1):
+// this workgroup's global writes are released before arriving and the
+// teammates' acquired after. Returns false when the wait timed out or another
+// workgroup reported a failure; the caller then leaves the kernel.
+__device__ bool team_sync(Team& tm, int* s_flag) {
+    __syncthreads();
+    if (tm.T == 1) return true;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(tm.bar, 1);
+        const int target = (++tm.gen) * tm.T;
+        int ok = 1;
+        for (uint32_t spin = 0; __hip_atomic_load(tm.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target;
+             ++spin) {
+            if (spin > TEAM_SPIN_LIMIT || __hip_atomic_load(tm.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                atomicExch(tm.err, 1);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence();
+        *s_flag = ok;
+    }
+    __syncthreads();
+    return *s_flag != 0;
+}
+
+struct BKernelArgs {
+    cx<double>* arena_d;  // reinterpret for the precision
+    const BCirc* circs;
+    const int32_t* order;
+    const BStep* steps;
+    BArgs prog;
+    const uint64_t* obs_m;
+    const double* obs_w;
+    int nterms;
+    double* tpart;  // per team: T x BATCH_DIAG partial sums
+    double* vals;
+};
+
+// One circuit's step list by a team (or a single workgroup). False: a team
+// barrier failed.
+template <class R>
+__device__ bool run_circuit(const BCirc& C, Team& tm, const BKernelArgs& K, cx<R>* tile, SlotVal* slotv,
+                            double* red, int* s_flag) {
+    const int tid = threadIdx.x, nth = blockDim.x;
+    cx<R>* arena = (cx<R>*)K.arena_d;
+    cx<R>* psi = arena + C.psi_off;
+    cx<R>* lam = arena + C.lam_off;
+    const uint64_t N = 1ULL << C.n;
+    const uint64_t first = (uint64_t)tm.r * nth + tid, stride = (uint64_t)tm.T * nth;
+    double* part = K.tpart + (size_t)tm.id * tm.T * BATCH_DIAG;
+    for (int si = C.step_begin; si < C.step_end; ++si) {
+        const BStep& S = K.steps[si];
+        const int kind = S.kind;
+        if (kind == BS_BASIS) {
+            for (uint64_t i = first; i < N; i += stride) psi[i] = {(R)(i == 0 ? 1 : 0), (R)0};
+        } else if (kind == BS_PROG) {
+            for (int p = S.pass_begin; p < S.pass_end; ++p) {
+                const FPass& P = K.prog.passes[p];
+                if (S.target & 1) {
+                    batch_pass_k<R>(C.k, psi, C.n, P, K.prog, tile, slotv, tm.r, tm.T);
+                    if (!team_sync(tm, s_flag)) return false;
+                }
+                if (S.target & 2) {
+                    batch_pass_k<R>(C.k, lam, C.n, P, K.prog, tile, slotv, tm.r, tm.T);
+                    if (!team_sync(tm, s_flag)) return false;
+                }
+            }
+            continue;
+        } else if (kind == BS_OBS) {
+            for (uint64_t i = first; i < N; i += stride) {
+                const double f = obs_factor(i, C.obs_keep, K.obs_m, K.obs_w, K.nterms);
+                const cx<R> a = psi[i];
+                lam[i] = {(R)(f * (double)a.re), (R)(f * (double)a.im)};
+            }
+        } else {  // BS_ENERGY, BS_GDIAG: partial sums of this workgroup, then the team's in rank order
+            const int nv = (kind == BS_ENERGY) ? 1 : S.nmask;
+            double acc[BATCH_DIAG];
+#pragma unroll
+            for (int d = 0; d < BATCH_DIAG; ++d) acc[d] = 0.0;
+            if (kind == BS_ENERGY) {
+                for (uint64_t i = first; i < N; i += stride)
+                    acc[0] += prob(psi[i]) * obs_factor(i, C.obs_keep, K.obs_m, K.obs_w, K.nterms);
+            } else {
+                for (uint64_t i = first; i < N; i += stride) {
+                    const cx<R> p = psi[i], l = lam[i];
+                    const double w = (double)l.re * (double)p.im - (double)l.im * (double)p.re;
+#pragma unroll
+                    for (int d = 0; d < BATCH_DIAG; ++d) acc[d] += (__popcll(i & S.mask[d]) & 1) ? -w : w;
+                }
+            }
+            const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+            for (int d = 0; d < BATCH_DIAG; ++d) {
+                const double v = wave_sum(acc[d]);
+                if (lane == 0) red[wave * BATCH_DIAG + d] = v;
+            }
+            __syncthreads();
+            if (tid < nv) {
+                double v = 0.0;
+                for (int w = 0; w < nth / 64; ++w) v += red[w * BATCH_DIAG + tid];
+                if (tm.T == 1) K.vals[S.out + tid] = v;
+                else part[tm.r * BATCH_DIAG + tid] = v;
+            }
+            if (tm.T > 1) {
+                if (!team_sync(tm, s_flag)) return false;
+                if (tm.r == 0 && tid < nv) {
+                    double v = 0.0;
+                    for (int q = 0; q < tm.T; ++q) v += part[q * BATCH_DIAG + tid];
+                    K.vals[S.out + tid] = v;
+                }
+            }
+        }
+        // the step's writes (and, for reductions, the leader's reads of the partials) before the next step
+        if (!team_sync(tm, s_flag)) return false;
+    }
+    return true;
+}
+
+// Persistent engine. Phase 1: teams of T workgroups take the first nbig
+// circuits of `order` from queue qctr[0]; phase 2: every workgroup alone takes
+// the remaining circuits from queue qctr[1]. Launched cooperatively, so every
+// workgroup of the grid is resident and team barriers cannot wait on a
+// workgroup that never runs; every wait is bounded besides.
+template <class R>
+__global__ __launch_bounds__(BATCH_THREADS) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_batch(
+    BKernelArgs K, int ncirc, int nbig, int T, int* qctr, int* team_bar, int* team_cur, int* err) {
+    __shared__ cx<R> tile[1 << BATCH_KMAX];
+    __shared__ SlotVal slotv[FUSE_MAX_SLOTS];
+    __shared__ double red[BATCH_THREADS / 64 * BATCH_DIAG];
+    __shared__ int s_next, s_flag;
+    const int tid = threadIdx.x;
+    if (T > 1 && nbig > 0) {
+        Team tm{(int)blockIdx.x / T, (int)blockIdx.x % T, T, team_bar + blockIdx.x / T, 0, err};
+        for (;;) {
+            if (tm.r == 0 && tid == 0) team_cur[tm.id] = atomicAdd(&qctr[0], 1);
+            if (!team_sync(tm, &s_flag)) return;
+            if (tid == 0) s_next = __hip_atomic_load(&team_cur[tm.id], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __syncthreads();
+            const int qi = s_next;
+            if (qi >= nbig) break;
+            if (!run_circuit<R>(K.circs[K.order[qi]], tm, K, tile, slotv, red, &s_flag)) return;
+        }
+    }
+    Team solo{0, 0, 1, nullptr, 0, err};
+    const int first = (T > 1) ? nbig : 0;
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_next = first + atomicAdd(&qctr[1], 1);
+        __syncthreads();
+        const int qi = s_next;
+        if (qi >= ncirc) break;
+        run_circuit<R>(K.circs[K.order[qi]], solo, K, tile, slotv, red, &s_flag);
+    }
+}
+
+}  // namespace
+
+}  // namespace qse
+
+using namespace qse;
+
+extern "C" int qse_batch_run_small(qse_ctx* ctx, int precision, int ncirc, const int* nq, const int64_t* op_off,
+                                   const qse_op* ops, const int64_t* par_off, const double* params,
+                                   const uint64_t* obs_masks, const double* obs_weights, int nterms,
+                                   double* energies, double* grads) {
+    if (!ctx || (precision != QSE_C128 && precision != QSE_C64) || ncirc < 1 || !nq || !op_off ||
+        (op_off[ncirc] > op_off[0] && !ops) || nterms < 0 || (nterms > 0 && (!obs_masks || !obs_weights)) ||
+        !energies || (grads && !par_off))
+        return set_error(QSE_ERR_INVALID_ARG, "bad argument");
+    QSE_HIP(hipSetDevice(ctx->device));
+    BatchPlan B;
+    int rc = build_batch_plan(precision, ncirc, nq, op_off, ops, par_off, params, obs_masks, nterms, grads != nullptr,
+                              BATCH_KMAX, precision == QSE_C128 ? 3 : 4, B);
+    if (rc) return rc;
+    // circuits of BATCH_TEAM_MIN_QUBITS or more go to teams, largest first
+    constexpr int BATCH_TEAM_MIN_QUBITS = 18;
+    std::stable_partition(B.order.begin(), B.order.end(),
+                          [&](int c) { return B.circs[c].n >= BATCH_TEAM_MIN_QUBITS; });
+    int nbig = 0;
+    for (int c : B.order) nbig += B.circs[c].n >= BATCH_TEAM_MIN_QUBITS ? 1 : 0;
+    // grid: every workgroup resident (cooperative launch), a whole number of teams
+    int per_cu = 0;
+    QSE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, precision == QSE_C128 ? (const void*)k_batch<double> : (const void*)k_batch<float>, BATCH_THREADS,
+        0));
+    int grid = std::max(1, per_cu) * ctx->num_cus;
+    int T = 16;
+    if (nbig == 0 || grid < 2 * T) T = 1;
+    if (T > 1) grid = grid / T * T;
+    grid = std::min(grid, std::max(ncirc, T > 1 ? grid : 1));
+    const int nteams = (T > 1) ? grid / T : 1;
+    const size_t ab = amp_bytes(precision);
+    size_t free_b = 0, total_b = 0;
+    QSE_HIP(hipMemGetInfo(&free_b, &total_b));
+    const double avail = (double)free_b + (double)ctx->scratch[4].bytes;  // the buffer of an earlier batch is reused
+    if ((double)B.arena_amps * (double)ab > 0.9 * avail)
+        return set_error(QSE_ERR_OUT_OF_MEMORY, "batch: the circuits' states do not fit the free device memory");
+    // one device buffer: arena | program | circuits | order | steps | observable | counters | partials | values
+    Serialized S;
+    serialize(B.prog, S);
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    size_t off = 0;
+    const size_t o_arena = off; off = al(off + B.arena_amps * ab);
+    const size_t o_prog = off; off = al(off + S.bytes.size());
+    const size_t o_circ = off; off = al(off + sizeof(BCirc) * B.circs.size());
+    const size_t o_order = off; off = al(off + sizeof(int32_t) * B.order.size());
+    const size_t o_steps = off; off = al(off + sizeof(BStep) * B.steps.size());
+    const size_t o_obsm = off; off = al(off + sizeof(uint64_t) * std::max(nterms, 1));
+    const size_t o_obsw = off; off = al(off + sizeof(double) * std::max(nterms, 1));
+    const size_t o_ctl = off; off = al(off + sizeof(int) * (3 + 2 * (size_t)nteams));  // qctr[2], err, bar, cur
+    const size_t o_part = off; off = al(off + sizeof(double) * (size_t)nteams * T * BATCH_DIAG);
+    const size_t o_vals = off; off = al(off + sizeof(double) * std::max(B.nvals, 1));
+    // context scratch (kept between calls: a batch of this size is typically run again)
+    void* dptr = nullptr;
+    rc = scratch(ctx, 4, off, &dptr);
+    if (rc) return rc;
+    char* dev = (char*)dptr;
+    auto fail = [&](int code) {
+        hipStreamSynchronize(ctx->stream);
+        return code;
+    };
+    auto h2d = [&](size_t o, const void* src, size_t bytes) {
+        return bytes ? hipMemcpyAsync(dev + o, src, bytes, hipMemcpyHostToDevice, ctx->stream) : hipSuccess;
+    };
+    hipError_t e = h2d(o_prog, S.bytes.data(), S.bytes.size());
+    if (e == hipSuccess) e = h2d(o_circ, B.circs.data(), sizeof(BCirc) * B.circs.size());
+    if (e == hipSuccess) e = h2d(o_order, B.order.data(), sizeof(int32_t) * B.order.size());
+    if (e == hipSuccess) e = h2d(o_steps, B.steps.data(), sizeof(BStep) * B.steps.size());
+    if (e == hipSuccess && nterms > 0) e = h2d(o_obsm, obs_masks, sizeof(uint64_t) * nterms);
+    if (e == hipSuccess && nterms > 0) e = h2d(o_obsw, obs_weights, sizeof(double) * nterms);
+    if (e == hipSuccess) e = hipMemsetAsync(dev + o_ctl, 0, sizeof(int) * (3 + 2 * (size_t)nteams), ctx->stream);
+    if (e != hipSuccess) return fail(hip_error(e, "batch upload"));
+    char* pg = dev + o_prog;
+    BKernelArgs K;
+    K.arena_d = (cx<double>*)(dev + o_arena);
+    K.circs = (const BCirc*)(dev + o_circ);
+    K.order = (const int32_t*)(dev + o_order);
+    K.steps = (const BStep*)(dev + o_steps);
+    K.prog = BArgs{(const FPass*)(pg + S.off[0]), (const FStage*)(pg + S.off[1]), (const FOp*)(pg + S.off[2]),
+                   (const FTerm*)(pg + S.off[3]),  (const FSlot*)(pg + S.off[4]),  (const FExtTerm*)(pg + S.off[5]),
+                   (const double*)(pg + S.off[6])};
+    K.obs_m = (const uint64_t*)(dev + o_obsm);
+    K.obs_w = (const double*)(dev + o_obsw);
+    K.nterms = nterms;
+    K.tpart = (double*)(dev + o_part);
+    K.vals = (double*)(dev + o_vals);
+    int* ctl = (int*)(dev + o_ctl);
+    int* qctr = ctl;
+    int* err = ctl + 2;
+    int* team_bar = ctl + 3;
+    int* team_cur = ctl + 3 + nteams;
+    void* args[] = {&K, &ncirc, &nbig, &T, &qctr, &team_bar, &team_cur, &err};
+    TimedLaunch t;
+    launch_begin(ctx, FAM_BATCH, &t);
+    e = hipLaunchCooperativeKernel(precision == QSE_C128 ? (const void*)k_batch<double> : (const void*)k_batch<float>,
+                                   dim3(grid), dim3(BATCH_THREADS), args, 0, ctx->stream);
+    if (e != hipSuccess) return fail(hip_error(e, "k_batch cooperative launch"));
+    launch_end(ctx, &t, B.est_bytes * (double)ab);
+    std::vector<double> vals(std::max(B.nvals, 1));
+    int herr = 0;
+    e = hipMemcpyAsync(vals.data(), dev + o_vals, sizeof(double) * vals.size(), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(hip_error(e, "batch readback"));
+    if (herr) return set_error(QSE_ERR_HIP, "batch: a team barrier timed out");
+    batch_results(B, vals.data(), par_off, energies, grads);
+    return QSE_OK;
+}

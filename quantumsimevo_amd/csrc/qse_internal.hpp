@@ -13,7 +13,7 @@
 namespace qse {
 
 // Kernel families for the timing report (qse_ctx_timing).
-enum Family { FAM_SWEEP = 0, FAM_FUSED = 1, FAM_READOUT = 2, FAM_ADJOINT = 3, FAM_COUNT = 4 };
+enum Family { FAM_SWEEP = 0, FAM_FUSED = 1, FAM_READOUT = 2, FAM_ADJOINT = 3, FAM_BATCH = 4, FAM_COUNT = 5 };
 
 struct TimedLaunch {
     int family;
@@ -40,6 +40,15 @@ struct ProgEntry {
     hipModule_t mod = nullptr;
     std::vector<hipFunction_t> fns;
 };
+
+// A host plan flattened into one buffer: the seven program arrays, each
+// 256-byte aligned, at off[0..6] (passes, stages, ops, terms, slots, eterms,
+// alphas).
+struct Serialized {
+    std::vector<char> bytes;
+    size_t off[7];
+};
+void serialize(const HostPlan& P, Serialized& S);
 
 // A tuned plan for one exact input (qubit count, precision, gate list,
 // parameter values, shape settings), so repeated runs skip planning.
@@ -79,9 +88,9 @@ struct qse_ctx {
     bool timing = false;
     std::vector<qse::TimedLaunch> pending;
     std::vector<hipEvent_t> event_pool;
-    int64_t launches[qse::FAM_COUNT] = {0, 0, 0, 0};
-    double total_ms[qse::FAM_COUNT] = {0, 0, 0, 0};
-    double total_bytes[qse::FAM_COUNT] = {0, 0, 0, 0};
+    int64_t launches[qse::FAM_COUNT] = {};
+    double total_ms[qse::FAM_COUNT] = {};
+    double total_bytes[qse::FAM_COUNT] = {};
 };
 
 struct qse_state {

@@ -11,38 +11,8 @@ namespace qse {
 int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int* passes_out,
               bool dry_run, bool allow_jit);
 
-double op_angle(const qse_op& op, const double* params) {
-    if (op.param >= 0 && params) return op.theta + op.scale * params[op.param];
-    return op.theta;
-}
-
-int validate_ops(int n, const qse_op* ops, int nops, const double* params) {
-    for (int k = 0; k < nops; ++k) {
-        const qse_op& o = ops[k];
-        auto badq = [&](int q) { return q < 0 || q >= n; };
-        switch (o.kind) {
-            case QSE_GATE_IDENTITY: break;
-            case QSE_GATE_X: case QSE_GATE_Y: case QSE_GATE_Z: case QSE_GATE_H: case QSE_GATE_PHASE:
-            case QSE_GATE_RX: case QSE_GATE_RY: case QSE_GATE_RZ: case QSE_GATE_MATRIX:
-                if (badq(o.target)) return set_error(QSE_ERR_INVALID_ARG, "op " + std::to_string(k) + ": bad target");
-                break;
-            case QSE_GATE_CNOT: case QSE_GATE_CPHASE: case QSE_GATE_SWAP:
-                if (badq(o.target) || badq(o.control))
-                    return set_error(QSE_ERR_INVALID_ARG, "op " + std::to_string(k) + ": bad qubit");
-                break;
-            case QSE_GATE_MCPHASE:
-                if (badq(o.target) || (n < 64 && (o.controls >> n)))
-                    return set_error(QSE_ERR_INVALID_ARG, "op " + std::to_string(k) + ": bad qubit");
-                break;
-            default:
-                return set_error(QSE_ERR_UNSUPPORTED, "op " + std::to_string(k) + ": unsupported gate kind " +
-                                                          std::to_string(o.kind));
-        }
-        if (o.param >= 0 && !params)
-            return set_error(QSE_ERR_INVALID_ARG, "op " + std::to_string(k) + " is parameterised but params is NULL");
-    }
-    return QSE_OK;
-}
+double op_angle(const qse_op& op, const double* params);
+int validate_ops(int n, const qse_op* ops, int nops, const double* params);
 
 // One op through the single-gate sweeps.
 int apply_one(qse_ctx* ctx, qse_state* st, const qse_op& o, const double* params) {

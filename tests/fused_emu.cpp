@@ -13,16 +13,13 @@
 #include <utility>
 #include <vector>
 
+#include "qse_batch.hpp"
 #include "qse_fused.hpp"
 #include "qse_hip.h"
 
 namespace qse {
 // The planner reports errors through this hook (libqse_hip keeps a message).
 int set_error(int status, const std::string&) { return status; }
-double op_angle(const qse_op& op, const double* params) {
-    if (op.param >= 0 && params) return op.theta + op.scale * params[op.param];
-    return op.theta;
-}
 int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P, int flags);
 int choose_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P,
                 bool allow_oop);
@@ -85,6 +82,70 @@ static void apply_op(const HostPlan& P, const FOp& op, cd* a, int tid, int TB, u
     }
 }
 
+// One pass of plan P over the 2^n amplitudes of s, written to dst (dst may be
+// s: each tile writes the addresses it read).
+static void emu_pass(const HostPlan& P, const FPass& fp, int n, const cd* s, cd* dst) {
+    const int k = fp.k;
+    const int NT = 1 << (k - 4), TB = k - 4;
+    std::vector<cd> tile(1u << k), slotv(FUSE_MAX_SLOTS);
+    const uint64_t ntiles = 1ULL << (n - k);
+    for (uint64_t ti = 0; ti < ntiles; ++ti) {
+        uint64_t base = 0, obase = 0, t = ti;
+        for (int e = 0; e < fp.next; ++e) {
+            base |= (t & 1ULL) << fp.epos[e];
+            obase |= (t & 1ULL) << fp.eout[e];
+            t >>= 1;
+        }
+        for (int sl = 0; sl < fp.nslots; ++sl) {
+            const FSlot& S = P.slots[fp.slot_begin + sl];
+            double a = 0.0;
+            for (uint32_t e = S.begin; e < S.end; ++e)
+                if ((base & P.eterms[e].emask) == P.eterms[e].emask) a += P.eterms[e].ang;
+            slotv[sl] = cd(a, 0.0);  // the slot's angle, carried in the real part
+        }
+        for (int tid = 0; tid < NT; ++tid)
+            for (int i = 0; i < 16; ++i) {
+                const uint32_t li = (uint32_t)(i * NT + tid);
+                uint64_t g = base;
+                for (int b = 0; b < k; ++b)
+                    if ((li >> b) & 1) g |= 1ULL << fp.tq[b];
+                tile[swz(li)] = s[g];
+            }
+        for (int sg = 0; sg < fp.nstages; ++sg) {
+            const FStage& S = P.stages[fp.stage_begin + sg];
+            for (int tid = 0; tid < NT; ++tid) {
+                uint32_t toff = 0;
+                for (int tb = 0; tb < TB; ++tb)
+                    if ((tid >> tb) & 1) toff |= 1u << S.tmap[tb];
+                uint32_t li[16];
+                cd a[16];
+                for (int j = 0; j < 16; ++j) {
+                    uint32_t l = toff;
+                    for (int p = 0; p < 4; ++p)
+                        if ((j >> p) & 1) l |= 1u << S.rmap[p];
+                    li[j] = swz(l);
+                    a[j] = tile[li[j]];
+                }
+                for (uint32_t oi = S.op_begin; oi < S.op_end; ++oi)
+                    apply_op(P, P.ops[oi], a, tid, TB, toff, base, slotv.data());
+                for (int j = 0; j < 16; ++j) tile[li[j]] = a[j];
+            }
+        }
+        for (int tid = 0; tid < NT; ++tid)
+            for (int i = 0; i < 16; ++i) {
+                const uint32_t go = (uint32_t)(i * NT + tid);
+                uint64_t g = obase;
+                uint32_t ls = 0;
+                for (int b = 0; b < k; ++b)
+                    if ((go >> b) & 1) {
+                        g |= 1ULL << fp.so_q[b];
+                        ls |= 1u << fp.so_l[b];
+                    }
+                dst[g] = tile[swz(ls)];
+            }
+    }
+}
+
 extern "C" int emu_run_fused(int n, int k, int L, const qse_op* ops, int nops, const double* params, double* psi,
                              int* npasses, int relabel) {
     HostPlan P;
@@ -95,67 +156,8 @@ extern "C" int emu_run_fused(int n, int k, int L, const qse_op* ops, int nops, c
     std::vector<cd> bufA(reinterpret_cast<cd*>(psi), reinterpret_cast<cd*>(psi) + N), bufB(N);
     cd* s = bufA.data();
     cd* shadow = bufB.data();
-    const int NT = 1 << (k - 4), TB = k - 4;
-    std::vector<cd> tile(1u << k), slotv(FUSE_MAX_SLOTS);
     for (const FPass& fp : P.passes) {
-        // every local bit must be covered exactly once by the tile positions
-        const uint64_t ntiles = 1ULL << (n - k);
-        cd* dst = fp.out_of_place ? shadow : s;
-        for (uint64_t ti = 0; ti < ntiles; ++ti) {
-            uint64_t base = 0, obase = 0, t = ti;
-            for (int e = 0; e < fp.next; ++e) {
-                base |= (t & 1ULL) << fp.epos[e];
-                obase |= (t & 1ULL) << fp.eout[e];
-                t >>= 1;
-            }
-            for (int sl = 0; sl < fp.nslots; ++sl) {
-                const FSlot& S = P.slots[fp.slot_begin + sl];
-                double a = 0.0;
-                for (uint32_t e = S.begin; e < S.end; ++e)
-                    if ((base & P.eterms[e].emask) == P.eterms[e].emask) a += P.eterms[e].ang;
-                slotv[sl] = cd(a, 0.0);  // the slot's angle, carried in the real part
-            }
-            for (int tid = 0; tid < NT; ++tid)
-                for (int i = 0; i < 16; ++i) {
-                    const uint32_t li = (uint32_t)(i * NT + tid);
-                    uint64_t g = base;
-                    for (int b = 0; b < k; ++b)
-                        if ((li >> b) & 1) g |= 1ULL << fp.tq[b];
-                    tile[swz(li)] = s[g];
-                }
-            for (int sg = 0; sg < fp.nstages; ++sg) {
-                const FStage& S = P.stages[fp.stage_begin + sg];
-                for (int tid = 0; tid < NT; ++tid) {
-                    uint32_t toff = 0;
-                    for (int tb = 0; tb < TB; ++tb)
-                        if ((tid >> tb) & 1) toff |= 1u << S.tmap[tb];
-                    uint32_t li[16];
-                    cd a[16];
-                    for (int j = 0; j < 16; ++j) {
-                        uint32_t l = toff;
-                        for (int p = 0; p < 4; ++p)
-                            if ((j >> p) & 1) l |= 1u << S.rmap[p];
-                        li[j] = swz(l);
-                        a[j] = tile[li[j]];
-                    }
-                    for (uint32_t oi = S.op_begin; oi < S.op_end; ++oi)
-                        apply_op(P, P.ops[oi], a, tid, TB, toff, base, slotv.data());
-                    for (int j = 0; j < 16; ++j) tile[li[j]] = a[j];
-                }
-            }
-            for (int tid = 0; tid < NT; ++tid)
-                for (int i = 0; i < 16; ++i) {
-                    const uint32_t go = (uint32_t)(i * NT + tid);
-                    uint64_t g = obase;
-                    uint32_t ls = 0;
-                    for (int b = 0; b < k; ++b)
-                        if ((go >> b) & 1) {
-                            g |= 1ULL << fp.so_q[b];
-                            ls |= 1u << fp.so_l[b];
-                        }
-                    dst[g] = tile[swz(ls)];
-                }
-        }
+        emu_pass(P, fp, n, s, fp.out_of_place ? shadow : s);
         if (fp.out_of_place) std::swap(s, shadow);
     }
     std::memcpy(psi, s, sizeof(cd) * N);
@@ -320,4 +322,91 @@ extern "C" int emu_jit_pass_qubits(int pass, uint32_t* tq) {
     const FPass& fp = g_jit_plan.passes[pass];
     for (int b = 0; b < fp.k; ++b) tq[b] = fp.tq[b];
     return fp.k;
+}
+
+// ---- batched small-circuit engine (qse_batch_plan.cpp) ----
+// Executes a batch plan's step lists in FP64 on the host with the kernel's
+// step semantics (qse_batch.hip) and returns energies and gradients through
+// the library's own result mapping. Returns the plan's status.
+extern "C" int emu_batch(int ncirc, const int* nq, const int64_t* op_off, const qse_op* ops, const int64_t* par_off,
+                         const double* params, const uint64_t* obs_m, const double* obs_w, int nterms, int with_grads,
+                         int kmax, int L, double* energies, double* grads) {
+    BatchPlan B;
+    const int rc = build_batch_plan(0, ncirc, nq, op_off, ops, par_off, params, obs_m, nterms, with_grads != 0, kmax,
+                                    L, B);
+    if (rc) return rc;
+    std::vector<double> vals(std::max(B.nvals, 1), 0.0);
+    for (int c = 0; c < ncirc; ++c) {
+        const BCirc& C = B.circs[c];
+        const uint64_t N = 1ULL << C.n;
+        std::vector<cd> psi(N), lam(N);
+        auto f = [&](uint64_t i) {
+            double v = 0.0;
+            for (int t = 0; t < nterms; ++t)
+                if ((C.obs_keep >> t) & 1) v += (__builtin_popcountll(i & obs_m[t]) & 1) ? -obs_w[t] : obs_w[t];
+            return v;
+        };
+        for (int si = C.step_begin; si < C.step_end; ++si) {
+            const BStep& S = B.steps[si];
+            switch (S.kind) {
+                case BS_BASIS:
+                    std::fill(psi.begin(), psi.end(), cd(0.0, 0.0));
+                    psi[0] = 1.0;
+                    break;
+                case BS_PROG:
+                    for (int p = S.pass_begin; p < S.pass_end; ++p) {
+                        if (B.prog.passes[p].k != C.k) return 101;
+                        if (S.target & 1) emu_pass(B.prog, B.prog.passes[p], C.n, psi.data(), psi.data());
+                        if (S.target & 2) emu_pass(B.prog, B.prog.passes[p], C.n, lam.data(), lam.data());
+                    }
+                    break;
+                case BS_OBS:
+                    for (uint64_t i = 0; i < N; ++i) lam[i] = f(i) * psi[i];
+                    break;
+                case BS_ENERGY: {
+                    double acc = 0.0;
+                    for (uint64_t i = 0; i < N; ++i) acc += std::norm(psi[i]) * f(i);
+                    vals[S.out] = acc;
+                    break;
+                }
+                case BS_GDIAG:
+                    for (int d = 0; d < S.nmask; ++d) {
+                        double acc = 0.0;
+                        for (uint64_t i = 0; i < N; ++i) {
+                            const double w = (std::conj(lam[i]) * psi[i]).imag();
+                            acc += (__builtin_popcountll(i & S.mask[d]) & 1) ? -w : w;
+                        }
+                        vals[S.out + d] = acc;
+                    }
+                    break;
+                default: return 102;
+            }
+        }
+    }
+    batch_results(B, vals.data(), par_off, energies, with_grads ? grads : nullptr);
+    return 0;
+}
+
+// Batch plan summary: out[0] passes, out[1] steps, out[2] reduction steps,
+// out[3] program steps, out[4] value slots; returns the plan's status.
+extern "C" int emu_batch_stats(int ncirc, const int* nq, const int64_t* op_off, const qse_op* ops,
+                               const int64_t* par_off, const double* params, const uint64_t* obs_m, int nterms,
+                               int kmax, int L, int64_t* out) {
+    BatchPlan B;
+    const int rc = build_batch_plan(0, ncirc, nq, op_off, ops, par_off, params, obs_m, nterms, true, kmax, L, B);
+    if (rc) return rc;
+    int64_t red = 0, progs = 0, passes = 0;
+    for (const BStep& s : B.steps) {
+        if (s.kind == BS_GDIAG || s.kind == BS_ENERGY) ++red;
+        if (s.kind == BS_PROG) {
+            ++progs;
+            passes += (int64_t)(s.pass_end - s.pass_begin) * ((s.target == 3) ? 2 : 1);
+        }
+    }
+    out[0] = passes;
+    out[1] = (int64_t)B.steps.size();
+    out[2] = red;
+    out[3] = progs;
+    out[4] = B.nvals;
+    return 0;
 }
