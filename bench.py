@@ -120,6 +120,67 @@ def qaoa_leg(ctx, batch: int, shift_sets: int) -> dict:
             "shift_evaluations_per_grad": int(2 * len(occ)), "mean_energy": float(np.mean(E))}
 
 
+def noise_leg(ctx) -> dict:
+    """Batched Pauli-noise trajectories: 20-qubit random layered circuit (one of
+    H/RX/RY/RZ/phase per qubit per layer, CNOT brickwork), depth 30, complex64;
+    512 trajectories whose X/Z frames come from the Noise draw process (seeds
+    0..511, bit- and phase-flip probability 1e-3 per eligible gate); 10 000
+    shots per trajectory sampled from the FP64 CDF of the one executed state
+    (frames permute outcomes, Noise.cpp:34-44)."""
+    from quantumsimevo_amd import _quantumsimevo as Q
+    n, depth, ntraj, shots = 20, 30, 512, 10000
+    ops = C.random_layered(n, depth, seed=2024).ops()
+    rng = np.random.default_rng(11)
+    u = rng.random((ntraj, shots))
+    st = ctx.state(n, capi.C64)
+    for _ in range(2):  # warm-up: the second run compiles the circuit's passes (run-time specialisation)
+        st.set_basis(0)
+        st.run(ops)
+    st.sample_frames(np.zeros(1, dtype=np.uint64), u[:1, :100])
+    ctx.sync()
+    t0 = time.perf_counter()
+    x, _z = Q.noise_frames(n, ops["kind"], ops["target"], np.arange(ntraj, dtype=np.uint32), 1e-3, 1.0)
+    t1 = time.perf_counter()
+    st.set_basis(0)
+    st.run(ops)
+    out = st.sample_frames(x, u)
+    t2 = time.perf_counter()
+    st.close()
+    return {"workload": f"{ntraj} noise trajectories x {shots} shots, {n}-qubit random layered circuit depth {depth}, "
+                        "complex64 state, FP64 CDF", "seconds": t2 - t0, "frames_seconds": t1 - t0,
+            "trajectories_per_s": ntraj / (t2 - t0), "shots_per_s": ntraj * shots / (t2 - t0),
+            "distinct_frames": int(len(np.unique(x))), "outcomes_checksum": int(np.bitwise_xor.reduce(out.ravel()))}
+
+
+def uneven_batch_leg(ctx) -> dict:
+    """1024 ansatz circuits (RY+RZ per qubit per layer, linear CNOT ladder),
+    qubits uniform in 10..22, depth uniform in 4..12, angles uniform in
+    [0, 2pi), observable sum_i Z_i: energies and adjoint gradients in one
+    persistent launch, complex64 with FP64 accumulation; the same batch in
+    complex128 gives the FP64 values the errors are stated against."""
+    circs = C.uneven_ansatz_batch(1024, seed=2024, n_range=(10, 22), depth_range=(4, 12))
+    masks = [1 << q for q in range(22)]
+    w = [1.0] * 22
+    small = circs[:8]
+    ctx.batch_run_small(capi.C64, small, masks, w)  # warm-up (device buffer, code objects)
+    ctx.reset_timing()
+    ctx.enable_timing(True)
+    t0 = time.perf_counter()
+    E64, G64 = ctx.batch_run_small(capi.C64, circs, masks, w)
+    t1 = time.perf_counter()
+    ctx.enable_timing(False)
+    kern = ctx.timing(capi.FAM_BATCH)
+    E, Gr = ctx.batch_run_small(capi.C128, circs, masks, w)
+    t2 = time.perf_counter()
+    nparams = sum(len(c[2]) for c in circs)
+    return {"workload": "1024 ansatz circuits, 10..22 qubits, depth 4..12, sum of Z, energies + adjoint gradients, "
+                        "complex64 / FP64 accumulation, one persistent launch",
+            "seconds": t1 - t0, "kernel_seconds": kern["ms"] / 1e3, "gradient_vectors_per_s": len(circs) / (t1 - t0),
+            "parameters": nparams, "complex128_seconds": t2 - t1,
+            "max_abs_energy_err_vs_fp64": float(np.max(np.abs(E64 - E))),
+            "max_abs_grad_err_vs_fp64": float(max(np.max(np.abs(a - b)) for a, b in zip(G64, Gr)))}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +196,7 @@ def main() -> int:
     ap.add_argument("--no-qaoa", action="store_true")
     ap.add_argument("--qaoa-batch", type=int, default=64)
     ap.add_argument("--qaoa-shift-sets", type=int, default=1)
+    ap.add_argument("--no-extra", action="store_true", help="skip the noise-trajectory and uneven-batch legs")
     ap.add_argument("--verify", type=int, default=16, metavar="K",
                     help="after timing, rerun one QFT from the seed and compare K output amplitudes with the FP64 "
                          "DFT closed form evaluated on the GPU (qse_qft_probe)")
@@ -259,6 +321,11 @@ def main() -> int:
 
     if not args.no_qaoa:
         result["qaoa"] = qaoa_leg(ctx, args.qaoa_batch, args.qaoa_shift_sets)
+    if not args.no_extra:
+        result["noise_trajectories"] = noise_leg(ctx)
+        log(f"noise trajectories: {result['noise_trajectories']['seconds']:.3f} s")
+        result["uneven_batch"] = uneven_batch_leg(ctx)
+        log(f"uneven batch: {result['uneven_batch']['seconds']:.3f} s")
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(n, ops, args.cpu_prefix)

@@ -10,6 +10,7 @@
 #include "quantumsimevo/Circuit.h"
 #include "quantumsimevo/CircuitUnitaryOperationHIP.h"
 #include "quantumsimevo/HardwareCheck.h"
+#include "quantumsimevo/Noise.h"
 
 namespace py = pybind11;
 
@@ -95,4 +96,39 @@ PYBIND11_MODULE(_quantumsimevo, m) {
           },
           py::arg("device") = 0);
     m.def("print_report", &HardwareCheck::printReport, py::arg("device") = 0);
+    // Pauli frames of many noise trajectories: per seed, the Noise draw process
+    // (Noise.cpp:21-32, two draws per eligible gate in replay order) over the
+    // gate list (GateType kinds, targets); rate >= 0 replaces every eligible
+    // type's rate. Returns (bit-flip masks, phase-flip masks), one per seed.
+    m.def("noise_frames",
+          [](size_t n, py::array_t<int, py::array::c_style | py::array::forcecast> kinds,
+             py::array_t<int, py::array::c_style | py::array::forcecast> targets,
+             py::array_t<unsigned int, py::array::c_style | py::array::forcecast> seeds, double rate,
+             double multiplier) {
+              if (kinds.size() != targets.size()) throw std::invalid_argument("kinds and targets differ in length");
+              if (n < 1 || n > 64) throw std::invalid_argument("1..64 qubits");
+              const int* k = kinds.data();
+              const int* t = targets.data();
+              const unsigned int* sd = seeds.data();
+              const ssize_t ng = kinds.size(), nt = seeds.size();
+              py::array_t<uint64_t> xs(nt), zs(nt);
+              uint64_t* x = xs.mutable_data();
+              uint64_t* z = zs.mutable_data();
+              {
+                  py::gil_scoped_release nogil;
+                  for (ssize_t j = 0; j < nt; ++j) {
+                      Noise nz(n, sd[j], multiplier);
+                      if (rate >= 0.0)
+                          for (GateType g : {GateType::X, GateType::Y, GateType::Z, GateType::H, GateType::CNOT,
+                                             GateType::PHASE, GateType::RX, GateType::RY, GateType::RZ})
+                              nz.setRate(g, rate);
+                      for (ssize_t i = 0; i < ng; ++i) nz.accumulateError((GateType)k[i], (size_t)t[i]);
+                      x[j] = nz.bitFlipMask();
+                      z[j] = nz.phaseFlipMask();
+                  }
+              }
+              return py::make_tuple(xs, zs);
+          },
+          py::arg("n"), py::arg("kinds"), py::arg("targets"), py::arg("seeds"), py::arg("rate") = -1.0,
+          py::arg("multiplier") = 1.0);
 }

@@ -108,14 +108,24 @@ __global__ __launch_bounds__(RB_THREADS) void k_shots(const cx<R>* __restrict__ 
         double prev = P[b], cur = prev;
         uint64_t j = b * B;
         bool found = false;
-        for (uint64_t i = 0; i < B; ++i) {
-            cur = prev + prob(s[(b * B + i) ^ x]);
-            if (cur >= target) {
-                j = b * B + i;
-                found = true;
-                break;
+        // the walk's running sum, 16 amplitudes per round: the loads of a round
+        // are issued together, the additions stay in index order (same rounding)
+        for (uint64_t i0 = 0; i0 < B && !found; i0 += 16) {
+            double pr[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) pr[c] = (i0 + c < B) ? prob(s[(b * B + i0 + c) ^ x]) : 0.0;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                if (!found && i0 + c < B) {
+                    cur = prev + pr[c];
+                    if (cur >= target) {
+                        j = b * B + i0 + c;
+                        found = true;
+                    } else {
+                        prev = cur;
+                    }
+                }
             }
-            prev = cur;
         }
         const double depth = (double)(B + nb + 64);
         const double delta = ((double)N + depth) * UR;

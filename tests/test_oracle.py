@@ -284,3 +284,15 @@ def test_noise_frames_match_reference_noise_class():
                 want.append(-(q + 1))
         assert order == want
     assert any(int(r["x"]) or int(r["z"]) for r in GOLD["noise_frames"])
+
+
+def test_product_noise_frames_match_reference_draws():
+    """The Python module's trajectory frames (host Noise class) against the
+    oracle's restatement of Noise.cpp (pinned to the reference Noise elsewhere)."""
+    from quantumsimevo_amd import _quantumsimevo as Q
+    n = 12
+    ops = C.random_layered(n, 10, seed=3).ops()
+    for rate, mult in ((-1.0, 1.0), (1e-2, 1.0), (-1.0, 25.0)):
+        x, z = Q.noise_frames(n, ops["kind"], ops["target"], np.arange(40, dtype=np.uint32), rate, mult)
+        xo, zo = O.noise_trajectories(n, ops, 0, 40, rate, mult)
+        assert np.array_equal(x, xo) and np.array_equal(z, zo)
