@@ -226,6 +226,14 @@ def main() -> int:
     ctx.set_jit({"auto": capi.JIT_AUTO, "off": capi.JIT_OFF, "always": capi.JIT_ALWAYS}[args.jit])
     info = capi.device_info(0)
     log(f"device: {info['name']} CUs={info['compute_units']} HBM={info['total_bytes'] / 2**30:.1f} GiB")
+    plan0 = ctx.plan_info(n, prec, ops) if not args.unfused else {"low_bits": 0}
+    # practical copy ceiling of this device for the pass's access pattern (tiled out-of-place copy of the
+    # state's size; complex64 runs of 2^L amplitudes are 2^(L-1) 16-byte elements)
+    log2_el = n if prec == capi.C128 else n - 1
+    run_el = max(plan0["low_bits"] - (0 if prec == capi.C128 else 1), 0)
+    copy_contig = ctx.copy_bandwidth(log2_el, 12)
+    copy_runs = ctx.copy_bandwidth(log2_el, run_el) if run_el < 12 else copy_contig
+    log(f"copy ceiling: contiguous {copy_contig:.0f} GB/s, runs of 2^{run_el} x 16 B {copy_runs:.0f} GB/s")
     st = ctx.state(n, prec)
     st.set_random(12345)
     plan = ctx.plan_info(n, prec, ops) if not args.unfused else {"passes": len(ops), "tile_bits": 0, "low_bits": 0,
@@ -298,7 +306,9 @@ def main() -> int:
                    "jit": args.jit, "jit_timed": jit_timed, "jit_compile_ms": jit1["compile_ms"]},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
-                     "algorithmic_bytes_per_launch": bytes_per_launch, "mean_launch_ms": mean_ms},
+                     "algorithmic_bytes_per_launch": bytes_per_launch, "mean_launch_ms": mean_ms,
+                     "copy_ceiling_gbs": {"contiguous_64KiB_tiles": copy_contig, "runs_of_the_plan": copy_runs},
+                     "frac_of_copy_ceiling": achieved / 1e9 / copy_runs if copy_runs > 0 else None},
         "pct_hbm_step": (bytes_per_launch * dom["launches"] / args.steps) / (ms_step * 1e-3) / HBM_PEAK,
     }
     if args.verify > 0:

@@ -211,6 +211,12 @@ int qse_plan_info(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops, i
  * vector, so it can check runs whose input no longer fits beside the output.
  * count <= 1024; synchronises. */
 int qse_qft_probe(qse_ctx* ctx, int nqubits, uint64_t seed, const uint64_t* indices, int count, double* out);
+/* Practical copy ceiling of the device (no counterpart in the reference): GB/s
+ * (read + write) of an out-of-place copy of 2^log2_elems 16-byte elements in
+ * 64 KiB tiles made of contiguous runs of 2^run_log2 elements, the access
+ * pattern of a fused pass with run_log2 contiguous low qubits (complex128).
+ * Allocates two buffers of 16 * 2^log2_elems bytes for the call; synchronises. */
+int qse_copy_bandwidth(qse_ctx* ctx, int log2_elems, int run_log2, double* gbs);
 
 /* ---- readout (StateVector.cpp:77-156), driven by host-supplied uniforms ----
  * u are canonical uniforms in [0,1) (std::generate_canonical<double,53>); the

@@ -39,7 +39,7 @@ EXPORTS = [
     "qse_state_download_range", "qse_state_set_basis", "qse_state_copy", "qse_state_set_random", "qse_apply_matrix",
     "qse_apply_hadamard", "qse_apply_x", "qse_apply_y", "qse_apply_z", "qse_apply_phase", "qse_apply_rx",
     "qse_apply_ry", "qse_apply_rz", "qse_apply_cnot", "qse_apply_cphase", "qse_apply_mcphase",
-    "qse_apply_swap", "qse_pauli_frame", "qse_run", "qse_run_sweeps", "qse_plan_passes", "qse_plan_info", "qse_qft_probe", "qse_sample",
+    "qse_apply_swap", "qse_pauli_frame", "qse_run", "qse_run_sweeps", "qse_plan_passes", "qse_plan_info", "qse_qft_probe", "qse_copy_bandwidth", "qse_sample",
     "qse_measure", "qse_norm2", "qse_prob_one", "qse_collapse_qubit", "qse_expect_zstrings",
     "qse_adjoint_expect", "qse_batch_expect", "qse_batch_run_small", "qse_sample_frames",
 ]
@@ -110,6 +110,7 @@ def _sigs(lib):
         "qse_plan_passes": ([_vp, _i, _i, _vp, _i, C.POINTER(_i)], _i),
         "qse_plan_info": ([_vp, _i, _i, _vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)], _i),
         "qse_qft_probe": ([_vp, _i, C.c_uint64, _vp, _i, _vp], _i),
+        "qse_copy_bandwidth": ([_vp, _i, _i, _pd], _i),
         "qse_sample": ([_vp, _vp, _vp, _i64, _vp], _i),
         "qse_measure": ([_vp, _vp, _d, _pu64], _i),
         "qse_norm2": ([_vp, _vp, _pd], _i),
@@ -250,6 +251,12 @@ class Context:
         out = np.empty(2 * len(idx))
         check(self.lib.qse_qft_probe(self.h, nqubits, seed, _ptr(idx), len(idx), _ptr(out)), "qse_qft_probe")
         return out.view(np.complex128)
+
+    def copy_bandwidth(self, log2_elems: int = 30, run_log2: int = 12) -> float:
+        """GB/s of a tiled out-of-place copy with contiguous runs of 2^run_log2 16-byte elements."""
+        g = C.c_double(0)
+        check(self.lib.qse_copy_bandwidth(self.h, log2_elems, run_log2, C.byref(g)), "qse_copy_bandwidth")
+        return g.value
 
     # ---- batched entry points ----
     def batch_run_small(self, precision, circuits, obs_masks, obs_weights, grads: bool = True):

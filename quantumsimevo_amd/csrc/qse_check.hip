@@ -85,6 +85,29 @@ __global__ __launch_bounds__(PROBE_BLOCK) void k_qft_probe(int n, uint64_t seed,
     }
 }
 
+// Out-of-place copy of 2^30-element-class buffers in tiles of 4096 16-byte
+// elements (64 KiB): thread t of 256 moves elements t + 256 j, j < 16, of its
+// tile. Element li of tile ti lives at ((li >> R) << S | ti) << R | (li & (2^R - 1)):
+// runs of 2^R contiguous elements, 2^(12-R) runs per tile spread 2^S runs apart,
+// the access pattern of a fused pass with R contiguous low qubits.
+__global__ __launch_bounds__(256) void k_copy_probe(const double2* __restrict__ s, double2* __restrict__ d,
+                                                    uint64_t ntiles, int R, int S) {
+    const uint64_t rmask = (1ULL << R) - 1;
+    for (uint64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        double2 v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint64_t li = (uint64_t)threadIdx.x + 256 * j;
+            v[j] = s[((((li >> R) << S) | ti) << R) | (li & rmask)];
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint64_t li = (uint64_t)threadIdx.x + 256 * j;
+            d[((((li >> R) << S) | ti) << R) | (li & rmask)] = v[j];
+        }
+    }
+}
+
 uint64_t bitrev(uint64_t v, int n) {
     uint64_t r = 0;
     for (int b = 0; b < n; ++b)
@@ -139,4 +162,46 @@ extern "C" int qse_qft_probe(qse_ctx* ctx, int nqubits, uint64_t seed, const uin
         for (int c = 0; c < 2 * K; ++c) out[2 * first + c] = acc[c] * f;
     }
     return QSE_OK;
+}
+
+extern "C" int qse_copy_bandwidth(qse_ctx* ctx, int log2_elems, int run_log2, double* gbs) {
+    if (!ctx || !gbs || log2_elems < 12 || log2_elems > 31 || run_log2 < 0 || run_log2 > 12)
+        return set_error(QSE_ERR_INVALID_ARG, "bad argument");
+    QSE_HIP(hipSetDevice(ctx->device));
+    const uint64_t n = 1ULL << log2_elems;  // 16-byte elements per buffer
+    const uint64_t ntiles = n >> 12;
+    const int S = log2_elems - 12;  // tile index bits between a tile's runs
+    // largest index: (((2^(12-R) - 1) << S | (ntiles - 1)) << R) | (2^R - 1) = n - 1
+    void* buf = nullptr;
+    if (hipMalloc(&buf, 2 * n * 16) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(QSE_ERR_OUT_OF_MEMORY, "copy probe: allocation failed");
+    }
+    double2* s = (double2*)buf;
+    double2* d = s + n;
+    int rc = QSE_OK;
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipMemsetAsync(buf, 0, 2 * n * 16, ctx->stream);
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)ctx->num_cus * 16);
+    if (e == hipSuccess) {
+        k_copy_probe<<<grid, 256, 0, ctx->stream>>>(s, d, ntiles, run_log2, S);  // warm-up
+        e = hipEventRecord(a, ctx->stream);
+    }
+    for (int r = 0; r < 5 && e == hipSuccess; ++r) {
+        k_copy_probe<<<grid, 256, 0, ctx->stream>>>(s, d, ntiles, run_log2, S);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    if (e != hipSuccess) rc = hip_error(e, "copy probe");
+    else *gbs = 2.0 * 16.0 * (double)n / (ms / 5.0 * 1e-3) / 1e9;
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    hipStreamSynchronize(ctx->stream);
+    hipFree(buf);
+    return rc;
 }
