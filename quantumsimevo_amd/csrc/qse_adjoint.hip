@@ -37,9 +37,12 @@ namespace qse {
 
 int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int* passes_out,
               bool dry_run, bool allow_jit);
+int run_plan(qse_ctx* ctx, qse_state* st, const HostPlan& plan, int k, bool allow_jit);
+int plan_in_place(qse_ctx* ctx, int n, int prec, const qse_op* ops, int nops, const double* params, HostPlan& P,
+                  int* k_out);
 int validate_ops(int n, const qse_op* ops, int nops, const double* params);
-int reduce(qse_ctx* ctx, const qse_state* st, int mode, int q, const uint64_t* masks, const double* w, int nterms,
-           double* out);
+int reduce_zstr_async(qse_ctx* ctx, const qse_state* st, const uint64_t* d_m, const double* d_w, int nterms,
+                      double* d_out);
 
 static constexpr int AD_THREADS = 256;
 static constexpr int DIAG_CHUNK = 16;
@@ -88,6 +91,97 @@ __global__ __launch_bounds__(AD_THREADS) void k_grad_diag(const cx<R>* __restric
     }
 }
 
+// Up to NM Z-strings in one pass: partial[b][d] = sum over the block's
+// amplitudes of (-1)^popcount(i & m_d) w_i, w_i = Im(conj(lambda_i) psi_i).
+// With S threads in the grid (a power of two), thread t takes the amplitudes
+// i = t + S (j + 32 k), j < 32, so the sign splits into
+//   parity(t & m) ^ parity(j & (m >> log2 S)) ^ parity(32 S k & m):
+// a per-thread constant, one entry of the 32-point Walsh-Hadamard transform
+// H[u] = sum_j (-1)^popcount(j & u) w_j of the thread's 32 values, and a
+// sign that is the same for the whole grid in iteration k. The transform is
+// kept in LDS (u-major), one read per mask per 32 amplitudes. Requires
+// 32 S <= N and N a multiple of 32 S.
+constexpr int WHT_J = 32;
+template <int NM>
+struct DiagMasksN {
+    uint64_t m[NM];
+};
+
+template <class R, int NM>
+__global__ __launch_bounds__(AD_THREADS) void k_grad_diag_wht(const cx<R>* __restrict__ psi,
+                                                              const cx<R>* __restrict__ lam, uint64_t N, int log2S,
+                                                              DiagMasksN<NM> M, double* __restrict__ part) {
+    __shared__ double hs[WHT_J][AD_THREADS];
+    __shared__ double red[AD_THREADS / 64][NM];
+    const uint64_t S = 1ULL << log2S;
+    const uint64_t t = (uint64_t)blockIdx.x * AD_THREADS + threadIdx.x;
+    uint64_t cpar = 0;  // bit d: parity(t & m_d)
+#pragma unroll
+    for (int d = 0; d < NM; ++d) cpar |= (uint64_t)(__popcll(t & M.m[d]) & 1) << d;
+    double acc[NM];
+#pragma unroll
+    for (int d = 0; d < NM; ++d) acc[d] = 0.0;
+    const uint64_t iters = N >> (log2S + 5);
+    for (uint64_t k = 0; k < iters; ++k) {
+        double w[WHT_J];
+#pragma unroll
+        for (int j = 0; j < WHT_J; ++j) {
+            const uint64_t i = t + S * ((uint64_t)j + WHT_J * k);
+            const cx<R> p = psi[i], l = lam[i];
+            w[j] = (double)l.re * (double)p.im - (double)l.im * (double)p.re;
+        }
+#pragma unroll
+        for (int h = 1; h < WHT_J; h <<= 1)
+#pragma unroll
+            for (int j = 0; j < WHT_J; ++j)
+                if (!(j & h)) {
+                    const double a = w[j], b = w[j + h];
+                    w[j] = a + b;
+                    w[j + h] = a - b;
+                }
+#pragma unroll
+        for (int u = 0; u < WHT_J; ++u) hs[u][threadIdx.x] = w[u];
+        // no barrier: each thread reads back only its own column
+        const uint64_t kb = (WHT_J * S) * k;
+#pragma unroll
+        for (int d = 0; d < NM; ++d) {
+            const int u = (int)((M.m[d] >> log2S) & (WHT_J - 1));
+            const double v = hs[u][threadIdx.x];
+            const bool neg = (((cpar >> d) ^ (uint64_t)__popcll(kb & M.m[d])) & 1) != 0;
+            acc[d] += neg ? -v : v;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < NM; ++d) {
+        const double v = wave_sum(acc[d]);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][d] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NM) {
+        double v = 0.0;
+        for (int w = 0; w < AD_THREADS / 64; ++w) v += red[w][threadIdx.x];
+        part[(uint64_t)blockIdx.x * NM + threadIdx.x] = v;
+    }
+}
+
+// out[d] = sum over rows r of part[r * stride + d], one workgroup per column
+// d < cols: thread t adds rows t, t + 256, ... in order, then a fixed tree
+// over the 256 threads (the same order on every run).
+__global__ __launch_bounds__(AD_THREADS) void k_sum_columns(const double* __restrict__ part, int rows, int stride,
+                                                            double* __restrict__ out) {
+    const int d = blockIdx.x;
+    double v = 0.0;
+    for (int r = threadIdx.x; r < rows; r += AD_THREADS) v += part[(size_t)r * stride + d];
+    __shared__ double sh[AD_THREADS];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = AD_THREADS / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[d] = sh[0];
+}
+
 namespace {
 
 struct Ws {
@@ -111,14 +205,51 @@ int launch_apply_obs(qse_ctx* ctx, qse_state* lam, const uint64_t* dm, const dou
     return QSE_OK;
 }
 
-template <class R>
-int grad_diag(qse_ctx* ctx, qse_state* psi, qse_state* lam, const std::vector<uint64_t>& masks, double* out) {
+template <class R, int NM>
+int launch_wht(qse_ctx* ctx, const qse_state* psi, const qse_state* lam, const uint64_t* masks, int cnt,
+               unsigned grid, int log2S, double* pbuf, double* d_out) {
+    DiagMasksN<NM> M{};
+    for (int d = 0; d < cnt; ++d) M.m[d] = masks[d];
     const uint64_t N = 1ULL << psi->n;
+    TimedLaunch t;
+    launch_begin(ctx, FAM_ADJOINT, &t);
+    k_grad_diag_wht<R, NM><<<grid, AD_THREADS, 0, ctx->stream>>>((const cx<R>*)psi->d, (const cx<R>*)lam->d, N, log2S,
+                                                                 M, pbuf);
+    QSE_HIP(hipGetLastError());
+    launch_end(ctx, &t, 2.0 * amp_bytes(psi->prec) * (double)N);
+    k_sum_columns<<<(unsigned)cnt, AD_THREADS, 0, ctx->stream>>>(pbuf, (int)grid, NM, d_out);
+    QSE_HIP(hipGetLastError());
+    return QSE_OK;
+}
+
+// Im<lambda|Z_m|psi> for every mask, written to d_out[0 .. masks.size()) on
+// the device (no host wait): one pass per 64 masks (per 16 below 13 qubits).
+template <class R>
+int grad_diag_async(qse_ctx* ctx, qse_state* psi, qse_state* lam, const std::vector<uint64_t>& masks,
+                    double* d_out) {
+    const uint64_t N = 1ULL << psi->n;
+    int rc;
+    if (psi->n >= 13) {
+        // grid: a power of two, at most 1024 workgroups, 32 amplitudes per thread and iteration
+        int log2S = psi->n - 5;
+        if (log2S > 18) log2S = 18;  // 1024 x 256 threads
+        const unsigned grid = 1u << (log2S - 8);
+        void* pbuf;
+        rc = scratch(ctx, 6, sizeof(double) * 64 * grid, &pbuf);
+        if (rc) return rc;
+        for (size_t c0 = 0; c0 < masks.size(); c0 += 64) {
+            const int cnt = (int)std::min<size_t>(64, masks.size() - c0);
+            if (cnt <= 16) rc = launch_wht<R, 16>(ctx, psi, lam, masks.data() + c0, cnt, grid, log2S, (double*)pbuf, d_out + c0);
+            else if (cnt <= 32) rc = launch_wht<R, 32>(ctx, psi, lam, masks.data() + c0, cnt, grid, log2S, (double*)pbuf, d_out + c0);
+            else rc = launch_wht<R, 64>(ctx, psi, lam, masks.data() + c0, cnt, grid, log2S, (double*)pbuf, d_out + c0);
+            if (rc) return rc;
+        }
+        return QSE_OK;
+    }
     const unsigned g = std::min<unsigned>(grid_for(ctx, N, AD_THREADS, 8), 1024u);
     void* pbuf;
-    int rc = scratch(ctx, 6, sizeof(double) * DIAG_CHUNK * g, &pbuf);
+    rc = scratch(ctx, 6, sizeof(double) * DIAG_CHUNK * g, &pbuf);
     if (rc) return rc;
-    std::vector<double> h((size_t)DIAG_CHUNK * g);
     for (size_t c0 = 0; c0 < masks.size(); c0 += DIAG_CHUNK) {
         DiagMasks M{};
         const size_t cnt = std::min<size_t>(DIAG_CHUNK, masks.size() - c0);
@@ -129,34 +260,63 @@ int grad_diag(qse_ctx* ctx, qse_state* psi, qse_state* lam, const std::vector<ui
                                                           (double*)pbuf);
         QSE_HIP(hipGetLastError());
         launch_end(ctx, &t, 2.0 * amp_bytes(psi->prec) * (double)N);
-        QSE_HIP(hipMemcpyAsync(h.data(), pbuf, sizeof(double) * DIAG_CHUNK * g, hipMemcpyDeviceToHost, ctx->stream));
-        QSE_HIP(hipStreamSynchronize(ctx->stream));
-        for (size_t d = 0; d < cnt; ++d) {
-            double v = 0.0;
-            for (unsigned b = 0; b < g; ++b) v += h[(size_t)b * DIAG_CHUNK + d];
-            out[c0 + d] = v;
-        }
+        k_sum_columns<<<(unsigned)cnt, AD_THREADS, 0, ctx->stream>>>((const double*)pbuf, (int)g, DIAG_CHUNK,
+                                                                    d_out + c0);
+        QSE_HIP(hipGetLastError());
     }
     return QSE_OK;
 }
 
+// Run ops on each of the given states (same qubit count and precision) with
+// one in-place plan.
+int run_on(qse_ctx* ctx, std::initializer_list<qse_state*> states, const std::vector<qse_op>& ops,
+           const double* params) {
+    if (ops.empty()) return QSE_OK;
+    qse_state* s0 = *states.begin();
+    int rc;
+    if (s0->n < FUSE_MIN_QUBITS) {
+        for (qse_state* st : states) {
+            rc = run_fused(ctx, st, ops.data(), (int)ops.size(), params, nullptr, false, false);
+            if (rc) return rc;
+        }
+        return QSE_OK;
+    }
+    HostPlan P;
+    int k = 0;
+    rc = plan_in_place(ctx, s0->n, s0->prec, ops.data(), (int)ops.size(), params, P, &k);
+    if (rc) return rc;
+    for (qse_state* st : states) {
+        rc = run_plan(ctx, st, P, k, false);
+        if (rc) return rc;
+    }
+    return QSE_OK;
+}
+
+// Where one parameter set's reduction values land in the call's device result
+// array: the energy, and per gradient term the value it scales.
+struct SetSlots {
+    int energy = -1;
+    std::vector<std::pair<int, GradTerm>> terms;
+};
+
+// One parameter set: forward sweep, then the reverse sweep of the segment
+// schedule. Every reduction writes its value to d_res[*next ...] on the device;
+// `slots` records which value is which.
 template <class R>
 int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double* params,
                 const std::vector<AdjSegment>& segs, const std::vector<qse_op>& prefix, const uint64_t* dm,
-                const double* dw, int nterms, const uint64_t* obs_m_h, const double* obs_w_h, int ckpt_every,
-                double* energy, double* grad) {
+                const double* dw, int nterms, int ckpt_every, double* d_res, int* next, SetSlots& slots) {
     int rc = qse_state_set_basis(ctx, ws.psi, 0);
     if (rc) return rc;
     const int S = (int)segs.size();
     // forward: whole list fused, or segment by segment storing checkpoints
     std::map<int, int> ckpt_at;  // segment -> checkpoint slot holding psi at its evaluation point
     if (ckpt_every > 0 && S > 1) {
-        rc = run_fused(ctx, ws.psi, prefix.data(), (int)prefix.size(), params, nullptr, false, false);
+        rc = run_on(ctx, {ws.psi}, prefix, params);
         if (rc) return rc;
         int slot = 0;
         for (int g = S - 2; g >= 0; --g) {  // forward order; the last segment's gates are in the prefix
-            rc = run_fused(ctx, ws.psi, segs[g].forward.data(), (int)segs[g].forward.size(), params, nullptr,
-                           false, false);
+            rc = run_on(ctx, {ws.psi}, segs[g].forward, params);
             if (rc) return rc;
             if (g > 0 && ((S - 1 - g) % ckpt_every) == 0 && slot < (int)ws.ckpt.size()) {
                 rc = qse_state_copy(ctx, ws.ckpt[slot], ws.psi);
@@ -165,22 +325,21 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
             }
         }
     } else {
-        rc = run_fused(ctx, ws.psi, ops, nops, params, nullptr, false, false);
+        rc = run_on(ctx, {ws.psi}, std::vector<qse_op>(ops, ops + nops), params);
         if (rc) return rc;
     }
-    rc = reduce(ctx, ws.psi, 2, 0, obs_m_h, obs_w_h, nterms, energy);
+    slots.energy = (*next)++;
+    rc = reduce_zstr_async(ctx, ws.psi, dm, dw, nterms, d_res + slots.energy);
     if (rc) return rc;
     rc = qse_state_copy(ctx, ws.lam, ws.psi);
     if (rc) return rc;
     rc = (ws.lam->prec == QSE_C128) ? launch_apply_obs<double>(ctx, ws.lam, dm, dw, nterms)
                                     : launch_apply_obs<float>(ctx, ws.lam, dm, dw, nterms);
     if (rc) return rc;
-    if (S > 0 && !segs[0].pre.empty()) {
-        // rotate into the first evaluation point's frame (X/Y generators become Z-strings)
-        for (qse_state* v : {ws.psi, ws.lam}) {
-            rc = run_fused(ctx, v, segs[0].pre.data(), (int)segs[0].pre.size(), nullptr, nullptr, false, false);
-            if (rc) return rc;
-        }
+    // rotate into the first evaluation point's frame (X/Y generators become Z-strings)
+    if (S > 0) {
+        rc = run_on(ctx, {ws.psi, ws.lam}, segs[0].pre, nullptr);
+        if (rc) return rc;
     }
     for (int g = 0; g < S; ++g) {
         const AdjSegment& seg = segs[g];
@@ -188,41 +347,44 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
         if (ck != ckpt_at.end()) {
             rc = qse_state_copy(ctx, ws.psi, ws.ckpt[ck->second]);
             if (rc) return rc;
-            if (!seg.pre.empty()) {
-                rc = run_fused(ctx, ws.psi, seg.pre.data(), (int)seg.pre.size(), nullptr, nullptr, false, false);
-                if (rc) return rc;
-            }
+            rc = run_on(ctx, {ws.psi}, seg.pre, nullptr);
+            if (rc) return rc;
         }
         // generator terms at this point: one pass for up to 16 Z-strings
         std::vector<uint64_t> masks;
-        std::vector<std::vector<GradTerm>> mterms;
+        std::vector<int> mslot;
         for (const auto& dt : seg.diag) {
-            auto pos = std::find(masks.begin(), masks.end(), dt.first);
-            if (pos == masks.end()) {
+            const size_t pos = std::find(masks.begin(), masks.end(), dt.first) - masks.begin();
+            if (pos == masks.size()) {
                 masks.push_back(dt.first);
-                mterms.push_back({dt.second});
-            } else {
-                mterms[pos - masks.begin()].push_back(dt.second);
+                mslot.push_back(*next + (int)pos);
             }
+            slots.terms.push_back({mslot[pos], dt.second});
         }
         if (!masks.empty()) {
-            std::vector<double> vals(masks.size());
-            rc = grad_diag<R>(ctx, ws.psi, ws.lam, masks, vals.data());
+            rc = grad_diag_async<R>(ctx, ws.psi, ws.lam, masks, d_res + *next);
             if (rc) return rc;
-            for (size_t i = 0; i < masks.size(); ++i)
-                for (const GradTerm& t : mterms[i]) grad[t.param] += t.coeff * vals[i];
+            *next += (int)masks.size();
         }
         if (g + 1 < S && !seg.unapply.empty()) {
-            if (ckpt_at.find(g + 1) == ckpt_at.end()) {
-                rc = run_fused(ctx, ws.psi, seg.unapply.data(), (int)seg.unapply.size(), nullptr, nullptr, false,
-                               false);
-                if (rc) return rc;
-            }
-            rc = run_fused(ctx, ws.lam, seg.unapply.data(), (int)seg.unapply.size(), nullptr, nullptr, false, false);
+            if (ckpt_at.find(g + 1) == ckpt_at.end()) rc = run_on(ctx, {ws.psi, ws.lam}, seg.unapply, nullptr);
+            else rc = run_on(ctx, {ws.lam}, seg.unapply, nullptr);
             if (rc) return rc;
         }
     }
     return QSE_OK;
+}
+
+// Number of reduction values adjoint_one writes for a schedule.
+int values_per_set(const std::vector<AdjSegment>& segs) {
+    int v = 1;
+    for (const AdjSegment& seg : segs) {
+        std::vector<uint64_t> masks;
+        for (const auto& dt : seg.diag)
+            if (std::find(masks.begin(), masks.end(), dt.first) == masks.end()) masks.push_back(dt.first);
+        v += (int)masks.size();
+    }
+    return v;
 }
 
 int check_common(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops, int nops, const double* params,
@@ -279,21 +441,40 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
         QSE_HIP(hipMemcpyAsync(dm, obs_masks, 8 * nterms, hipMemcpyHostToDevice, ctx->stream));
         QSE_HIP(hipMemcpyAsync(dw, obs_weights, 8 * nterms, hipMemcpyHostToDevice, ctx->stream));
     }
+    // every reduction value of the call lands in one device array, read once at the end
+    const int vps = values_per_set(segs0);
+    void* rbuf;
+    rc = scratch(ctx, 8, sizeof(double) * (size_t)vps * batch, &rbuf);
+    if (rc) return rc;
+    double* d_res = (double*)rbuf;
+    std::vector<SetSlots> slots(batch);
+    int next = 0;
     for (int b = 0; b < batch; ++b) {
         const double* pb = params ? params + (size_t)b * nparams : nullptr;
-        double* gb = grads ? grads + (size_t)b * nparams : nullptr;
-        if (gb) std::fill(gb, gb + nparams, 0.0);
-        std::vector<double> gtmp(std::max(nparams, 1), 0.0);
         std::vector<AdjSegment> segs;
         rc = adjoint_segments(ops, nops, pb, segs, prefix);
         if (rc) return rc;
+        if (values_per_set(segs) != vps)
+            return set_error(QSE_ERR_UNSUPPORTED, "adjoint: schedule changed between parameter sets");
         rc = (precision == QSE_C128)
-                 ? adjoint_one<double>(ctx, ws, ops, nops, pb, segs, prefix, dm, dw, nterms, obs_masks, obs_weights,
-                                       ckpt_every, &energies[b], gtmp.data())
-                 : adjoint_one<float>(ctx, ws, ops, nops, pb, segs, prefix, dm, dw, nterms, obs_masks, obs_weights,
-                                      ckpt_every, &energies[b], gtmp.data());
+                 ? adjoint_one<double>(ctx, ws, ops, nops, pb, segs, prefix, dm, dw, nterms, ckpt_every, d_res, &next,
+                                       slots[b])
+                 : adjoint_one<float>(ctx, ws, ops, nops, pb, segs, prefix, dm, dw, nterms, ckpt_every, d_res, &next,
+                                      slots[b]);
         if (rc) return rc;
-        if (gb) std::copy(gtmp.begin(), gtmp.begin() + nparams, gb);
+    }
+    void* pin;
+    rc = pinned(ctx, sizeof(double) * (size_t)std::max(next, 1), &pin);
+    if (rc) return rc;
+    QSE_HIP(hipMemcpyAsync(pin, d_res, sizeof(double) * (size_t)next, hipMemcpyDeviceToHost, ctx->stream));
+    QSE_HIP(hipStreamSynchronize(ctx->stream));
+    const double* vals = (const double*)pin;
+    for (int b = 0; b < batch; ++b) {
+        energies[b] = vals[slots[b].energy];
+        if (!grads) continue;
+        double* gb = grads + (size_t)b * nparams;
+        std::fill(gb, gb + nparams, 0.0);
+        for (const auto& st : slots[b].terms) gb[st.second.param] += st.second.coeff * vals[st.first];
     }
     return QSE_OK;
 }
@@ -304,17 +485,39 @@ int qse_batch_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops
     int rc = check_common(ctx, nqubits, precision, ops, nops, params, nparams, batch, obs_masks, obs_weights, nterms);
     if (rc) return rc;
     if (!energies) return set_error(QSE_ERR_INVALID_ARG, "null output");
-    qse_state* st = nullptr;
-    rc = qse_state_create(ctx, nqubits, precision, &st);
+    Ws ws;
+    ws.ctx = ctx;
+    rc = qse_state_create(ctx, nqubits, precision, &ws.psi);
     if (rc) return rc;
-    for (int b = 0; b < batch && rc == QSE_OK; ++b) {
-        const double* pb = params ? params + (size_t)b * nparams : nullptr;
-        rc = qse_state_set_basis(ctx, st, 0);
-        if (!rc) rc = run_fused(ctx, st, ops, nops, pb, nullptr, false, false);
-        if (!rc) rc = reduce(ctx, st, 2, 0, obs_masks, obs_weights, nterms, &energies[b]);
+    // observable and the per-set energies on the device, read once at the end
+    void* obuf;
+    const size_t mpad = ((size_t)std::max(nterms, 1) + 31) / 32 * 32;
+    rc = scratch(ctx, 7, 16 * mpad + 8 * (size_t)batch + 256, &obuf);
+    if (rc) return rc;
+    uint64_t* dm = (uint64_t*)obuf;
+    double* dw = (double*)((char*)obuf + 8 * mpad);
+    double* d_res = dw + mpad;
+    if (nterms > 0) {
+        QSE_HIP(hipMemcpyAsync(dm, obs_masks, 8 * nterms, hipMemcpyHostToDevice, ctx->stream));
+        QSE_HIP(hipMemcpyAsync(dw, obs_weights, 8 * nterms, hipMemcpyHostToDevice, ctx->stream));
     }
-    qse_state_destroy(ctx, st);
-    return rc;
+    const std::vector<qse_op> list(ops, ops + nops);
+    for (int b = 0; b < batch; ++b) {
+        const double* pb = params ? params + (size_t)b * nparams : nullptr;
+        rc = qse_state_set_basis(ctx, ws.psi, 0);
+        if (rc) return rc;
+        rc = run_on(ctx, {ws.psi}, list, pb);
+        if (rc) return rc;
+        rc = reduce_zstr_async(ctx, ws.psi, dm, dw, nterms, d_res + b);
+        if (rc) return rc;
+    }
+    void* pin;
+    rc = pinned(ctx, sizeof(double) * (size_t)batch, &pin);
+    if (rc) return rc;
+    QSE_HIP(hipMemcpyAsync(pin, d_res, sizeof(double) * (size_t)batch, hipMemcpyDeviceToHost, ctx->stream));
+    QSE_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(energies, pin, sizeof(double) * (size_t)batch);
+    return QSE_OK;
 }
 
 }  // extern "C"

@@ -224,10 +224,12 @@ int qse_ctx_destroy(qse_ctx* ctx) {
     for (auto e : ctx->event_pool) hipEventDestroy(e);
     for (auto& s : ctx->scratch)
         if (s.ptr) hipFree(s.ptr);
-    for (auto& e : ctx->prog_cache) {
+    for (auto& e : ctx->prog_cache)
         if (e.mod) hipModuleUnload(e.mod);
-        hipFree(e.dev);
-    }
+    for (auto& c : ctx->ring.inflight) hipEventDestroy(c.done);
+    for (auto e : ctx->ring.spare) hipEventDestroy(e);
+    if (ctx->ring.dev) hipFree(ctx->ring.dev);
+    if (ctx->ring.host) hipHostFree(ctx->ring.host);
     if (ctx->pinned) hipHostFree(ctx->pinned);
     hipStreamDestroy(ctx->stream);
     delete ctx;

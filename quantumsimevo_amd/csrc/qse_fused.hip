@@ -242,33 +242,100 @@ void serialize(const HostPlan& P, Serialized& S) {
 }
 
 // Device copy of a serialized program, reused when the same program runs again.
+namespace {
+
+size_t ring_len(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+
+// Copy `bytes` into a fresh place of the program ring (growing the ring when
+// the program does not fit) and return its offset. Cache entries whose device
+// copy that place overwrites lose it.
+int ring_upload(qse_ctx* ctx, const std::vector<char>& bytes, size_t* off) {
+    ProgRing& R = ctx->ring;
+    const size_t need = ring_len(bytes.size());
+    if (need > R.cap) {
+        QSE_HIP(hipStreamSynchronize(ctx->stream));
+        for (auto& c : R.inflight) R.spare.push_back(c.done);
+        R.inflight.clear();
+        if (R.dev) QSE_HIP(hipFree(R.dev));
+        if (R.host) QSE_HIP(hipHostFree(R.host));
+        R.dev = nullptr;
+        R.host = nullptr;
+        R.cap = 0;
+        const size_t cap = std::max<size_t>(2 * need, (size_t)16 << 20);
+        QSE_HIP(hipMalloc((void**)&R.dev, cap));
+        QSE_HIP(hipHostMalloc((void**)&R.host, cap, hipHostMallocDefault));
+        R.cap = cap;
+        R.head = 0;
+        for (auto& e : ctx->prog_cache) e.dev = nullptr;
+    }
+    if (R.head + need > R.cap) R.head = 0;
+    const size_t b = R.head, e = b + need;
+    // Copies complete in stream order: waiting for the last one that read
+    // [b, e) retires it and every older one; then retire finished ones.
+    int last = -1;
+    for (int i = 0; i < (int)R.inflight.size(); ++i)
+        if (R.inflight[i].begin < e && b < R.inflight[i].end) last = i;
+    if (last >= 0) QSE_HIP(hipEventSynchronize(R.inflight[last].done));
+    size_t done = (size_t)(last + 1);
+    while (done < R.inflight.size() && hipEventQuery(R.inflight[done].done) == hipSuccess) ++done;
+    for (size_t i = 0; i < done; ++i) R.spare.push_back(R.inflight[i].done);
+    R.inflight.erase(R.inflight.begin(), R.inflight.begin() + done);
+    for (auto& en : ctx->prog_cache)
+        if (en.dev && en.ring_off < e && b < en.ring_off + ring_len(en.bytes.size())) en.dev = nullptr;
+    std::memcpy(R.host + b, bytes.data(), bytes.size());
+    QSE_HIP(hipMemcpyAsync(R.dev + b, R.host + b, bytes.size(), hipMemcpyHostToDevice, ctx->stream));
+    hipEvent_t ev;
+    if (!R.spare.empty()) {
+        ev = R.spare.back();
+        R.spare.pop_back();
+    } else {
+        QSE_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+    QSE_HIP(hipEventRecord(ev, ctx->stream));
+    R.inflight.push_back({b, e, ev});
+    R.head = e;
+    *off = b;
+    return QSE_OK;
+}
+
+}  // namespace
+
 int program_on_device(qse_ctx* ctx, const Serialized& S, int prec, ProgEntry** entry) {
+    ProgEntry* hit = nullptr;
     for (auto& e : ctx->prog_cache) {
         if (e.prec == prec && e.bytes.size() == S.bytes.size() &&
             std::memcmp(e.bytes.data(), S.bytes.data(), S.bytes.size()) == 0) {
-            e.stamp = ++ctx->prog_clock;
-            *entry = &e;
-            return QSE_OK;
+            hit = &e;
+            break;
         }
     }
-    if (ctx->prog_cache.size() >= 32) {
-        size_t oldest = 0;
-        for (size_t i = 1; i < ctx->prog_cache.size(); ++i)
-            if (ctx->prog_cache[i].stamp < ctx->prog_cache[oldest].stamp) oldest = i;
-        QSE_HIP(hipStreamSynchronize(ctx->stream));
-        if (ctx->prog_cache[oldest].mod) QSE_HIP(hipModuleUnload(ctx->prog_cache[oldest].mod));
-        QSE_HIP(hipFree(ctx->prog_cache[oldest].dev));
-        ctx->prog_cache.erase(ctx->prog_cache.begin() + oldest);
+    if (!hit) {
+        if (ctx->prog_cache.size() >= 32) {
+            size_t oldest = 0;
+            for (size_t i = 1; i < ctx->prog_cache.size(); ++i)
+                if (ctx->prog_cache[i].stamp < ctx->prog_cache[oldest].stamp) oldest = i;
+            if (ctx->prog_cache[oldest].mod) {
+                // its kernels may still be queued
+                QSE_HIP(hipStreamSynchronize(ctx->stream));
+                QSE_HIP(hipModuleUnload(ctx->prog_cache[oldest].mod));
+            }
+            ctx->prog_cache.erase(ctx->prog_cache.begin() + oldest);
+        }
+        ProgEntry e;
+        e.bytes = S.bytes;
+        e.prec = prec;
+        ctx->prog_cache.push_back(std::move(e));
+        hit = &ctx->prog_cache.back();
     }
-    ProgEntry e;
-    e.bytes = S.bytes;
-    e.prec = prec;
-    QSE_HIP(hipMalloc(&e.dev, S.bytes.size()));
-    // pageable source: the copy is staged before this call returns
-    QSE_HIP(hipMemcpyAsync(e.dev, S.bytes.data(), S.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
-    e.stamp = ++ctx->prog_clock;
-    ctx->prog_cache.push_back(std::move(e));
-    *entry = &ctx->prog_cache.back();
+    if (!hit->dev) {
+        size_t off = 0;
+        const int rc = ring_upload(ctx, hit->bytes, &off);
+        if (rc) return rc;
+        hit->dev = ctx->ring.dev + off;
+        hit->ring_off = off;
+    }
+    hit->stamp = ++ctx->prog_clock;
+    *entry = hit;
     return QSE_OK;
 }
 
@@ -324,8 +391,6 @@ static int launch_pass(qse_ctx* ctx, qse_state* st, int K, const FProgram& prog,
     return QSE_OK;
 }
 
-// Smallest state that goes through the fused path (below: one sweep per gate).
-constexpr int FUSE_MIN_QUBITS = 10;
 
 // Key of a qse_run input for the plan cache: everything the plan depends on.
 static std::vector<char> plan_key(const qse_ctx* ctx, const qse_state* st, const qse_op* ops, int nops,
@@ -341,6 +406,75 @@ static std::vector<char> plan_key(const qse_ctx* ctx, const qse_state* st, const
     w += sizeof(qse_op) * (size_t)nops;
     if (maxp >= 0 && params) std::memcpy(w, params, sizeof(double) * (size_t)(maxp + 1));
     return key;
+}
+
+// Launch the passes of `plan` (tile qubits k) on st: program upload (ring),
+// run-time specialised kernels when allowed and available, per-pass slot
+// tables, and the pointer exchange after each out-of-place pass.
+int run_plan(qse_ctx* ctx, qse_state* st, const HostPlan& plan, int k, bool allow_jit) {
+    int rc;
+    if (!plan.passes.empty() && plan.passes.back().out_of_place && !st->shadow)
+        return set_error(QSE_ERR_UNSUPPORTED, "out-of-place plan without a second buffer");
+    Serialized S;
+    serialize(plan, S);
+    ProgEntry* entry = nullptr;
+    rc = program_on_device(ctx, S, st->prec, &entry);
+    if (rc) return rc;
+    bool jit = false;
+    if (allow_jit) rc = jit_for(ctx, *entry, plan, st->prec, &jit);
+    if (rc) return rc;
+    char* dev = (char*)entry->dev;
+    FProgram prog;
+    prog.passes = (const FPass*)(dev + S.off[0]);
+    prog.stages = (const FStage*)(dev + S.off[1]);
+    prog.ops = (const FOp*)(dev + S.off[2]);
+    prog.terms = (const FTerm*)(dev + S.off[3]);
+    prog.slots = (const FSlot*)(dev + S.off[4]);
+    prog.eterms = (const FExtTerm*)(dev + S.off[5]);
+    prog.alphas = (const double*)(dev + S.off[6]);
+    const double bytes = 2.0 * (double)amp_bytes(st->prec) * (double)(1ULL << st->n);
+    const uint64_t ntiles = 1ULL << (st->n - k);
+    int max_slots = 0;
+    for (const FPass& fp : plan.passes) max_slots = std::max(max_slots, fp.nslots);
+    SlotVal* table = nullptr;
+    if (max_slots > 0) {
+        void* tb;
+        rc = scratch(ctx, 5, sizeof(SlotVal) * ntiles * (uint64_t)max_slots, &tb);
+        if (rc) return rc;
+        table = (SlotVal*)tb;
+    }
+    for (int p = 0; p < (int)plan.passes.size(); ++p) {
+        if (plan.passes[p].nslots > 0) {
+            const uint64_t work = ntiles * (uint64_t)plan.passes[p].nslots;
+            k_slot_table<<<grid_for(ctx, work, 256, 8), 256, 0, ctx->stream>>>(prog.passes, prog.slots, prog.eterms,
+                                                                              p, ntiles, table);
+            QSE_HIP(hipGetLastError());
+        }
+        TimedLaunch t;
+        launch_begin(ctx, FAM_FUSED, &t);
+        const bool oop = plan.passes[p].out_of_place != 0;
+        if (jit) {
+            rc = jit_launch(ctx, entry->fns[p], st->d, oop ? st->shadow : st->d, table, ntiles, k);
+            ++ctx->jit_launches;
+        } else {
+            rc = (st->prec == QSE_C128) ? launch_pass<double>(ctx, st, k, prog, table, p, oop)
+                                        : launch_pass<float>(ctx, st, k, prog, table, p, oop);
+        }
+        if (rc) return rc;
+        launch_end(ctx, &t, bytes);
+        if (oop) std::swap(st->d, st->shadow);
+    }
+    return QSE_OK;
+}
+
+// In-place plan of ops for an n-qubit state with the context's default shape
+// (the adjoint's internal programs: planned once, run on psi and lambda).
+int plan_in_place(qse_ctx* ctx, int n, int prec, const qse_op* ops, int nops, const double* params, HostPlan& P,
+                  int* k_out) {
+    int k, L;
+    fusion_shape(ctx, n, prec, &k, &L);
+    *k_out = k;
+    return choose_plan(n, k, L, ops, nops, params, P, false);
 }
 
 int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int* passes_out,
@@ -397,56 +531,7 @@ int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const do
             plan = &P;
         }
     }
-    Serialized S;
-    serialize(*plan, S);
-    ProgEntry* entry = nullptr;
-    rc = program_on_device(ctx, S, st->prec, &entry);
-    if (rc) return rc;
-    bool jit = false;
-    if (allow_jit) rc = jit_for(ctx, *entry, *plan, st->prec, &jit);
-    if (rc) return rc;
-    char* dev = (char*)entry->dev;
-    FProgram prog;
-    prog.passes = (const FPass*)(dev + S.off[0]);
-    prog.stages = (const FStage*)(dev + S.off[1]);
-    prog.ops = (const FOp*)(dev + S.off[2]);
-    prog.terms = (const FTerm*)(dev + S.off[3]);
-    prog.slots = (const FSlot*)(dev + S.off[4]);
-    prog.eterms = (const FExtTerm*)(dev + S.off[5]);
-    prog.alphas = (const double*)(dev + S.off[6]);
-    const double bytes = 2.0 * (double)amp_bytes(st->prec) * (double)(1ULL << st->n);
-    const uint64_t ntiles = 1ULL << (st->n - k);
-    int max_slots = 0;
-    for (const FPass& fp : plan->passes) max_slots = std::max(max_slots, fp.nslots);
-    SlotVal* table = nullptr;
-    if (max_slots > 0) {
-        void* tb;
-        rc = scratch(ctx, 5, sizeof(SlotVal) * ntiles * (uint64_t)max_slots, &tb);
-        if (rc) return rc;
-        table = (SlotVal*)tb;
-    }
-    for (int p = 0; p < (int)plan->passes.size(); ++p) {
-        if (plan->passes[p].nslots > 0) {
-            const uint64_t work = ntiles * (uint64_t)plan->passes[p].nslots;
-            k_slot_table<<<grid_for(ctx, work, 256, 8), 256, 0, ctx->stream>>>(prog.passes, prog.slots, prog.eterms,
-                                                                              p, ntiles, table);
-            QSE_HIP(hipGetLastError());
-        }
-        TimedLaunch t;
-        launch_begin(ctx, FAM_FUSED, &t);
-        const bool oop = plan->passes[p].out_of_place != 0;
-        if (jit) {
-            rc = jit_launch(ctx, entry->fns[p], st->d, oop ? st->shadow : st->d, table, ntiles, k);
-            ++ctx->jit_launches;
-        } else {
-            rc = (st->prec == QSE_C128) ? launch_pass<double>(ctx, st, k, prog, table, p, oop)
-                                        : launch_pass<float>(ctx, st, k, prog, table, p, oop);
-        }
-        if (rc) return rc;
-        launch_end(ctx, &t, bytes);
-        if (oop) std::swap(st->d, st->shadow);
-    }
-    return QSE_OK;
+    return run_plan(ctx, st, *plan, k, allow_jit);
 }
 
 }  // namespace qse

@@ -26,19 +26,38 @@ struct Scratch {
     size_t bytes = 0;
 };
 
-// A fused-pass program resident on the device, keyed by its serialized bytes
-// and the state precision (the same plan serves both precisions, its compiled
-// kernels do not), with its run-time compiled kernels once it has them (one
-// per pass).
+// A fused-pass program, keyed by its serialized bytes and the state precision
+// (the same plan serves both precisions, its compiled kernels do not): its
+// place in the device program ring while it is there (dev, ring_off), and its
+// run-time compiled kernels once it has them (one per pass).
 struct ProgEntry {
     std::vector<char> bytes;
     int prec = 0;
-    void* dev = nullptr;
+    void* dev = nullptr;  // null once the ring has reused its place
+    size_t ring_off = 0;
     uint64_t stamp = 0;
     int runs = 0;
     bool jit_failed = false;
     hipModule_t mod = nullptr;
     std::vector<hipFunction_t> fns;
+};
+
+// Device program ring: programs are copied to consecutive places of one device
+// buffer from the same places of a pinned host buffer, in stream order, so an
+// upload neither allocates nor waits for the device. The device copy of a
+// place is overwritten only by a later upload on the same stream, after every
+// kernel launched before it; the host copy of a place is rewritten once the
+// copy that read it has completed (its event).
+struct ProgRing {
+    char* dev = nullptr;
+    char* host = nullptr;
+    size_t cap = 0, head = 0;
+    struct Copy {
+        size_t begin, end;
+        hipEvent_t done;
+    };
+    std::vector<Copy> inflight;  // oldest first
+    std::vector<hipEvent_t> spare;
 };
 
 // A host plan flattened into one buffer: the seven program arrays, each
@@ -67,9 +86,10 @@ struct qse_ctx {
     size_t lds_per_block = 65536;
     hipStream_t stream = nullptr;
     // Device scratch, grown on demand (partials, flags, cumulative arrays).
-    qse::Scratch scratch[8];
-    // Device copies of recently run fused programs.
+    qse::Scratch scratch[10];
+    // Recently run fused programs, their device copies in the program ring.
     std::vector<qse::ProgEntry> prog_cache;
+    qse::ProgRing ring;
     uint64_t prog_clock = 0;
     // Tuned plans of recent qse_run inputs.
     std::vector<qse::PlanEntry> plan_cache;
@@ -129,6 +149,9 @@ int fold_timing(qse_ctx* ctx);
 unsigned grid_for(const qse_ctx* ctx, uint64_t work, int block, int waves_per_cu = 32);
 
 inline size_t amp_bytes(int prec) { return prec == QSE_C128 ? 16 : 8; }
+
+// Smallest state that goes through the fused path (below: one sweep per gate).
+constexpr int FUSE_MIN_QUBITS = 10;
 
 // ---- per-gate sweeps (qse_gates.hip) ----
 // Pair kernel: for k < nk, base = deposit(k); (i0, i1) = (base|off0, base|off1).

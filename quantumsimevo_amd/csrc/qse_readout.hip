@@ -399,6 +399,31 @@ int reduce(qse_ctx* ctx, const qse_state* st, int mode, int q, const uint64_t* m
     return reduce_t<float>(ctx, st, mode, q, masks, w, nterms, out);
 }
 
+// sum_k w_k <Z_{m_k}> with masks and weights already on the device, written to
+// d_out on the device (stream-ordered, no host wait): the same partial sums
+// and final order as reduce().
+int reduce_zstr_async(qse_ctx* ctx, const qse_state* st, const uint64_t* d_m, const double* d_w, int nterms,
+                      double* d_out) {
+    const uint64_t N = 1ULL << st->n;
+    const unsigned g = std::min<unsigned>(grid_for(ctx, N, RB_THREADS, 8), 2048u);
+    void* part;
+    int rc = scratch(ctx, 9, sizeof(double) * g, &part);
+    if (rc) return rc;
+    TimedLaunch t;
+    launch_begin(ctx, FAM_READOUT, &t);
+    if (st->prec == QSE_C128)
+        k_reduce<double, RED_ZSTR><<<g, RB_THREADS, 0, ctx->stream>>>((const cx<double>*)st->d, N, 0, d_m, d_w,
+                                                                       nterms, (double*)part);
+    else
+        k_reduce<float, RED_ZSTR><<<g, RB_THREADS, 0, ctx->stream>>>((const cx<float>*)st->d, N, 0, d_m, d_w,
+                                                                      nterms, (double*)part);
+    QSE_HIP(hipGetLastError());
+    launch_end(ctx, &t, (double)amp_bytes(st->prec) * N);
+    k_sum_partials<<<1, 64, 0, ctx->stream>>>((const double*)part, (int)g, d_out);
+    QSE_HIP(hipGetLastError());
+    return QSE_OK;
+}
+
 // Single-qubit collapse (StateVector.cpp:158-221): keep the `outcome` half
 // scaled by 1/nf; with reset, move it to the |0> half and zero the |1> half.
 template <class R>

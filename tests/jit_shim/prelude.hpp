@@ -20,3 +20,8 @@ void shim_syncthreads();
 #define __shared__ static
 #define __launch_bounds__(x)
 #define __syncthreads() shim_syncthreads()
+// Non-temporal state accesses become plain ones; the two-component vector type
+// of the generated code becomes a GCC vector (same [i] subscripts).
+#define ext_vector_type(n) vector_size((n) * sizeof(R))
+#define __builtin_nontemporal_load(p) (*(p))
+#define __builtin_nontemporal_store(v, p) (*(p) = (v))

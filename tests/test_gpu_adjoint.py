@@ -119,3 +119,35 @@ def test_torch_autograd_function(ctx, qaoa_small):
     # parameter shift on the GPU agrees with the adjoint
     gs = eng.parameter_shift(params[:1])
     assert np.max(np.abs(gs - Gr[:1])) < 1e-10
+
+
+def test_wide_zstring_reductions_qaoa16(ctx):
+    """From 13 qubits the generator terms of a block are reduced 64 Z-strings
+    per pass (Walsh-Hadamard reduction); checked against the oracle's
+    parameter shift at 16 qubits."""
+    n, p = 16, 2
+    edges = G.random_regular_graph(n, 3, seed=9)
+    ops = G.qaoa_maxcut(n, edges, p).ops()
+    masks, w = G.maxcut_observable(edges)
+    params = np.random.default_rng(5).random((2, 2 * p)) * math.pi
+    E, Gr = ctx.adjoint_expect(n, capi.C128, ops, params, masks, w)
+    for b in range(len(params)):
+        assert abs(E[b] - oracle_energy(n, ops, params[b], masks, w)) < 1e-11
+        assert np.max(np.abs(Gr[b] - oracle_shift(n, ops, params[b], masks, w))) < 1e-10
+
+
+def test_more_than_64_terms_in_one_block(ctx):
+    """A cost layer on the complete graph K_14 puts 91 Z_a Z_b generators in
+    one block: two reduction passes (64 + 27 masks)."""
+    n = 14
+    edges = [(a, b) for a in range(n) for b in range(a + 1, n)]
+    ops = G.qaoa_maxcut(n, edges, 1).ops()
+    masks, w = G.maxcut_observable(edges)
+    params = np.array([0.37, 1.21])
+    E, Gr = ctx.adjoint_expect(n, capi.C128, ops, params, masks, w)
+    assert abs(E[0] - oracle_energy(n, ops, params, masks, w)) < 1e-11 * max(1.0, abs(E[0]))
+    gs = oracle_shift(n, ops, params, masks, w)
+    # gradients reach ~1e2 here: the bar is relative to their size
+    assert np.max(np.abs(Gr[0] - gs)) < 1e-10 * max(1.0, float(np.max(np.abs(gs))))
+    E32, G32 = ctx.adjoint_expect(n, capi.C64, ops, params, masks, w)
+    assert abs(E32[0] - E[0]) < 1e-4 and np.max(np.abs(G32 - Gr)) < 1e-3
