@@ -132,12 +132,20 @@ int qse_ctx_get_fusion(qse_ctx* ctx, int precision, int* tile_bits, int* low_bit
  * QSE_JIT_AUTO (default) when the same program (gates and angles) runs a
  * second time, QSE_JIT_ALWAYS on first use, QSE_JIT_OFF never (the generic
  * program-interpreting kernel). Results agree with the generic kernel to
- * rounding. */
+ * rounding. The internal runs of qse_adjoint_expect and qse_batch_expect,
+ * whose coefficients change with the parameters, use kernels keyed by the
+ * plan's structure instead, which read the coefficients from the program
+ * buffer: QSE_JIT_AUTO compiles a structure on a worker thread from its
+ * second run on (the generic kernel runs until the module is ready),
+ * QSE_JIT_ALWAYS compiles it on first use and waits. */
 enum qse_jit_mode { QSE_JIT_OFF = 0, QSE_JIT_AUTO = 1, QSE_JIT_ALWAYS = 2 };
 int qse_ctx_set_jit(qse_ctx* ctx, int mode);
-/* Programs compiled, passes launched through compiled kernels, and total
- * compile time in milliseconds since the context was created. */
+/* Programs and structures compiled, passes launched through compiled kernels,
+ * and total compile time in milliseconds since the context was created. */
 int qse_ctx_jit_stats(qse_ctx* ctx, int64_t* compiled, int64_t* jit_launches, double* compile_ms);
+/* Wait until every compilation started on worker threads has finished (no
+ * counterpart in the reference). */
+int qse_ctx_jit_wait(qse_ctx* ctx);
 /* Kernel timing: when enabled, every kernel launched by the context records HIP
  * events; qse_ctx_timing() returns the count and summed milliseconds per kernel
  * family. Families: 0 pair sweep, 1 fused tile pass, 2 readout, 3 adjoint, 4 batched circuits. */

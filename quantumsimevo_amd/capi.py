@@ -34,7 +34,7 @@ JIT_OFF, JIT_AUTO, JIT_ALWAYS = range(3)
 EXPORTS = [
     "qse_abi_version", "qse_status_string", "qse_last_error", "qse_device_count", "qse_device_mem_info",
     "qse_device_info", "qse_ctx_create", "qse_ctx_destroy", "qse_ctx_sync", "qse_ctx_set_fusion",
-    "qse_ctx_get_fusion", "qse_ctx_set_jit", "qse_ctx_jit_stats", "qse_ctx_enable_timing", "qse_ctx_timing", "qse_ctx_reset_timing",
+    "qse_ctx_get_fusion", "qse_ctx_set_jit", "qse_ctx_jit_stats", "qse_ctx_jit_wait", "qse_ctx_enable_timing", "qse_ctx_timing", "qse_ctx_reset_timing",
     "qse_state_create", "qse_state_destroy", "qse_state_info", "qse_state_upload", "qse_state_download",
     "qse_state_download_range", "qse_state_set_basis", "qse_state_copy", "qse_state_set_random", "qse_apply_matrix",
     "qse_apply_hadamard", "qse_apply_x", "qse_apply_y", "qse_apply_z", "qse_apply_phase", "qse_apply_rx",
@@ -79,6 +79,7 @@ def _sigs(lib):
         "qse_ctx_get_fusion": ([_vp, _i, C.POINTER(_i), C.POINTER(_i)], _i),
         "qse_ctx_set_jit": ([_vp, _i], _i),
         "qse_ctx_jit_stats": ([_vp, C.POINTER(_i64), C.POINTER(_i64), _pd], _i),
+        "qse_ctx_jit_wait": ([_vp], _i),
         "qse_ctx_enable_timing": ([_vp, _i], _i),
         "qse_ctx_timing": ([_vp, _i, C.POINTER(_i64), _pd, _pd], _i),
         "qse_ctx_reset_timing": ([_vp], _i),
@@ -213,6 +214,10 @@ class Context:
         c, l, ms = C.c_int64(0), C.c_int64(0), C.c_double(0)
         check(self.lib.qse_ctx_jit_stats(self.h, C.byref(c), C.byref(l), C.byref(ms)), "qse_ctx_jit_stats")
         return {"compiled": c.value, "launches": l.value, "compile_ms": ms.value}
+
+    def jit_wait(self):
+        """Wait for the structure compilations running on worker threads."""
+        check(self.lib.qse_ctx_jit_wait(self.h), "qse_ctx_jit_wait")
 
     def enable_timing(self, on: bool = True):
         check(self.lib.qse_ctx_enable_timing(self.h, int(on)), "qse_ctx_enable_timing")

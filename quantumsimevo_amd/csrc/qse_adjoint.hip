@@ -37,7 +37,7 @@ namespace qse {
 
 int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int* passes_out,
               bool dry_run, bool allow_jit);
-int run_plan(qse_ctx* ctx, qse_state* st, const HostPlan& plan, int k, bool allow_jit);
+int run_plan(qse_ctx* ctx, qse_state* st, const HostPlan& plan, int k, int jit_kind);
 int plan_in_place(qse_ctx* ctx, int n, int prec, const qse_op* ops, int nops, const double* params, HostPlan& P,
                   int* k_out);
 int validate_ops(int n, const qse_op* ops, int nops, const double* params);
@@ -286,7 +286,7 @@ int run_on(qse_ctx* ctx, std::initializer_list<qse_state*> states, const std::ve
     rc = plan_in_place(ctx, s0->n, s0->prec, ops.data(), (int)ops.size(), params, P, &k);
     if (rc) return rc;
     for (qse_state* st : states) {
-        rc = run_plan(ctx, st, P, k, false);
+        rc = run_plan(ctx, st, P, k, JIT_BY_STRUCTURE);
         if (rc) return rc;
     }
     return QSE_OK;

@@ -19,6 +19,7 @@ int sample_frames(qse_ctx* ctx, qse_state* st, const uint64_t* xmask, int ntraj,
 int reduce(qse_ctx* ctx, const qse_state* st, int mode, int q, const uint64_t* masks, const double* w, int nterms,
            double* out);
 int launch_collapse(qse_ctx* ctx, qse_state* st, int q, int outcome, int reset, double nf);
+void jit_release_all(qse_ctx* ctx);
 
 static thread_local std::string g_last_error;
 
@@ -217,6 +218,7 @@ int qse_ctx_destroy(qse_ctx* ctx) {
     if (!ctx) return QSE_OK;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    qse::jit_release_all(ctx);
     for (auto& t : ctx->pending) {
         hipEventDestroy(t.a);
         hipEventDestroy(t.b);

@@ -36,9 +36,12 @@ int choose_plan(int n, int k, int L, const qse_op* ops, int nops, const double* 
 int tuned_plan(const qse_ctx* ctx, int n, int prec, const qse_op* ops, int nops, const double* params, HostPlan& P,
                bool allow_oop, int* k_out, int* L_out);
 int run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params);
-std::string jit_source(const HostPlan& P, int prec);
+std::string jit_source(const HostPlan& P, int prec, const size_t* table_off);
 int jit_compile(const std::string& src, int npasses, hipModule_t* mod, std::vector<hipFunction_t>& fns);
-int jit_launch(qse_ctx* ctx, hipFunction_t fn, const void* src, void* dst, const void* table, uint64_t ntiles, int K);
+int jit_launch(qse_ctx* ctx, hipFunction_t fn, const void* src, void* dst, const void* table, uint64_t ntiles, int K,
+               const void* prog);
+int jit_for_structure(qse_ctx* ctx, const HostPlan& P, const Serialized& S, int prec,
+                      const std::vector<hipFunction_t>** fns);
 
 // Per-tile external factors of one pass, for every tile at once: entry
 // (tile, slot) = angle sum of the slot's external terms whose qubits are all
@@ -222,25 +225,6 @@ __global__ __launch_bounds__(1 << (K - 4)) __attribute__((amdgpu_waves_per_eu(2,
     }
 }
 
-static size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
-
-void serialize(const HostPlan& P, Serialized& S) {
-    const size_t sz[7] = {sizeof(FPass) * P.passes.size(), sizeof(FStage) * P.stages.size(),
-                          sizeof(FOp) * P.ops.size(),       sizeof(FTerm) * P.terms.size(),
-                          sizeof(FSlot) * P.slots.size(),   sizeof(FExtTerm) * P.eterms.size(),
-                          sizeof(double) * P.alphas.size()};
-    const void* src[7] = {P.passes.data(), P.stages.data(), P.ops.data(),   P.terms.data(),
-                          P.slots.data(),  P.eterms.data(), P.alphas.data()};
-    size_t tot = 0;
-    for (int i = 0; i < 7; ++i) {
-        S.off[i] = tot;
-        tot += al256(std::max<size_t>(sz[i], 1));
-    }
-    S.bytes.assign(tot, 0);
-    for (int i = 0; i < 7; ++i)
-        if (sz[i]) std::memcpy(S.bytes.data() + S.off[i], src[i], sz[i]);
-}
-
 // Device copy of a serialized program, reused when the same program runs again.
 namespace {
 
@@ -350,7 +334,7 @@ int jit_for(qse_ctx* ctx, ProgEntry& e, const HostPlan& P, int prec, bool* use) 
     if (ctx->jit_mode == QSE_JIT_AUTO && e.runs < 2) return QSE_OK;
     if (!e.mod) {
         const auto t0 = std::chrono::steady_clock::now();
-        const int rc = jit_compile(jit_source(P, prec), (int)P.passes.size(), &e.mod, e.fns);
+        const int rc = jit_compile(jit_source(P, prec, nullptr), (int)P.passes.size(), &e.mod, e.fns);
         ctx->jit_compile_ms +=
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         if (rc) {
@@ -409,9 +393,11 @@ static std::vector<char> plan_key(const qse_ctx* ctx, const qse_state* st, const
 }
 
 // Launch the passes of `plan` (tile qubits k) on st: program upload (ring),
-// run-time specialised kernels when allowed and available, per-pass slot
-// tables, and the pointer exchange after each out-of-place pass.
-int run_plan(qse_ctx* ctx, qse_state* st, const HostPlan& plan, int k, bool allow_jit) {
+// run-time specialised kernels when allowed and available (jit_kind: keyed by
+// the exact program, literal coefficients; or keyed by the plan structure,
+// coefficients read from the program), per-pass slot tables, and the pointer
+// exchange after each out-of-place pass.
+int run_plan(qse_ctx* ctx, qse_state* st, const HostPlan& plan, int k, int jit_kind) {
     int rc;
     if (!plan.passes.empty() && plan.passes.back().out_of_place && !st->shadow)
         return set_error(QSE_ERR_UNSUPPORTED, "out-of-place plan without a second buffer");
@@ -420,8 +406,14 @@ int run_plan(qse_ctx* ctx, qse_state* st, const HostPlan& plan, int k, bool allo
     ProgEntry* entry = nullptr;
     rc = program_on_device(ctx, S, st->prec, &entry);
     if (rc) return rc;
-    bool jit = false;
-    if (allow_jit) rc = jit_for(ctx, *entry, plan, st->prec, &jit);
+    const std::vector<hipFunction_t>* fns = nullptr;
+    if (jit_kind == JIT_BY_PROGRAM) {
+        bool jit = false;
+        rc = jit_for(ctx, *entry, plan, st->prec, &jit);
+        if (jit) fns = &entry->fns;
+    } else if (jit_kind == JIT_BY_STRUCTURE) {
+        rc = jit_for_structure(ctx, plan, S, st->prec, &fns);
+    }
     if (rc) return rc;
     char* dev = (char*)entry->dev;
     FProgram prog;
@@ -453,8 +445,8 @@ int run_plan(qse_ctx* ctx, qse_state* st, const HostPlan& plan, int k, bool allo
         TimedLaunch t;
         launch_begin(ctx, FAM_FUSED, &t);
         const bool oop = plan.passes[p].out_of_place != 0;
-        if (jit) {
-            rc = jit_launch(ctx, entry->fns[p], st->d, oop ? st->shadow : st->d, table, ntiles, k);
+        if (fns) {
+            rc = jit_launch(ctx, (*fns)[p], st->d, oop ? st->shadow : st->d, table, ntiles, k, entry->dev);
             ++ctx->jit_launches;
         } else {
             rc = (st->prec == QSE_C128) ? launch_pass<double>(ctx, st, k, prog, table, p, oop)
@@ -531,7 +523,7 @@ int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const do
             plan = &P;
         }
     }
-    return run_plan(ctx, st, *plan, k, allow_jit);
+    return run_plan(ctx, st, *plan, k, allow_jit ? JIT_BY_PROGRAM : JIT_NONE);
 }
 
 }  // namespace qse

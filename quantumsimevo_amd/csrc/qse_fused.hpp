@@ -121,4 +121,18 @@ struct HostPlan {
     std::vector<double> alphas;
 };
 
+// A host plan flattened into one buffer: the seven program arrays, each
+// 256-byte aligned, at off[0..6] (passes, stages, ops, terms, slots, eterms,
+// alphas).
+struct Serialized {
+    std::vector<char> bytes;
+    size_t off[7];
+};
+void serialize(const HostPlan& P, Serialized& S);
+
+// The serialized plan with every value field zeroed (gate coefficients, phase
+// factors and angles, thread-bit factors): plans that differ only in their
+// values (the same circuit at other parameters) share it.
+std::vector<char> structure_key(const HostPlan& P);
+
 }  // namespace qse

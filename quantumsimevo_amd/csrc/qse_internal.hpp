@@ -4,7 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+#include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/qse_hip.h"
@@ -60,14 +63,24 @@ struct ProgRing {
     std::vector<hipEvent_t> spare;
 };
 
-// A host plan flattened into one buffer: the seven program arrays, each
-// 256-byte aligned, at off[0..6] (passes, stages, ops, terms, slots, eterms,
-// alphas).
-struct Serialized {
-    std::vector<char> bytes;
-    size_t off[7];
+// Compiled kernels of one plan structure (coefficients read from the program
+// buffer at run time), built on a worker thread: state 0 while compiling,
+// 1 ready, 2 failed.
+struct JitJob {
+    std::atomic<int> state{0};
+    hipModule_t mod = nullptr;
+    std::vector<hipFunction_t> fns;
+    double ms = 0.0;
+    std::thread worker;
 };
-void serialize(const HostPlan& P, Serialized& S);
+struct StructEntry {
+    std::vector<char> key;  // structure_key of the plan
+    int prec = 0;
+    int seen = 0;
+    bool counted = false;
+    uint64_t stamp = 0;
+    std::shared_ptr<JitJob> job;
+};
 
 // A tuned plan for one exact input (qubit count, precision, gate list,
 // parameter values, shape settings), so repeated runs skip planning.
@@ -91,6 +104,8 @@ struct qse_ctx {
     std::vector<qse::ProgEntry> prog_cache;
     qse::ProgRing ring;
     uint64_t prog_clock = 0;
+    // Kernels by plan structure (the adjoint's and batched expectations' runs).
+    std::vector<qse::StructEntry> struct_cache;
     // Tuned plans of recent qse_run inputs.
     std::vector<qse::PlanEntry> plan_cache;
     // Pinned host staging for small readbacks.
@@ -149,6 +164,11 @@ int fold_timing(qse_ctx* ctx);
 unsigned grid_for(const qse_ctx* ctx, uint64_t work, int block, int waves_per_cu = 32);
 
 inline size_t amp_bytes(int prec) { return prec == QSE_C128 ? 16 : 8; }
+
+// Run-time specialised kernels for run_plan: none, keyed by the exact program
+// (qse_run inputs), or keyed by the plan structure (internal runs whose
+// coefficients change from call to call).
+enum JitKind { JIT_NONE = 0, JIT_BY_PROGRAM = 1, JIT_BY_STRUCTURE = 2 };
 
 // Smallest state that goes through the fused path (below: one sweep per gate).
 constexpr int FUSE_MIN_QUBITS = 10;

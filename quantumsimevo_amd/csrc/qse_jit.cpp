@@ -12,6 +12,8 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -21,6 +23,7 @@
 namespace qse {
 
 std::string jit_kernel_name(int pass);
+std::string jit_source(const HostPlan& P, int prec, const size_t* table_off);
 
 // Compile `src` for gfx950 and load it; one function per pass.
 int jit_compile(const std::string& src, int npasses, hipModule_t* mod, std::vector<hipFunction_t>& fns) {
@@ -57,15 +60,129 @@ int jit_compile(const std::string& src, int npasses, hipModule_t* mod, std::vect
     return QSE_OK;
 }
 
-int jit_launch(qse_ctx* ctx, hipFunction_t fn, const void* src, void* dst, const void* table, uint64_t ntiles, int K) {
+int jit_launch(qse_ctx* ctx, hipFunction_t fn, const void* src, void* dst, const void* table, uint64_t ntiles, int K,
+               const void* prog) {
     const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)ctx->num_cus * 16);
     unsigned long long nt = ntiles;
-    void* args[] = {(void*)&src, (void*)&dst, (void*)&table, (void*)&nt};
+    void* args[] = {(void*)&src, (void*)&dst, (void*)&table, (void*)&nt, (void*)&prog};
     QSE_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, 1u << (K - 4), 1, 1, 0, ctx->stream, args, nullptr));
     return QSE_OK;
 }
 
+namespace {
+
+constexpr size_t STRUCT_CACHE_MAX = 64;
+constexpr int MAX_COMPILE_WORKERS = 4;
+
+void start_compile(qse_ctx* ctx, StructEntry& e, const HostPlan& P, const Serialized& S, int prec) {
+    auto job = std::make_shared<JitJob>();
+    std::string src = jit_source(P, prec, S.off);
+    const int np = (int)P.passes.size(), dev = ctx->device;
+    JitJob* j = job.get();
+    j->worker = std::thread([j, src = std::move(src), np, dev] {
+        hipSetDevice(dev);
+        const auto t0 = std::chrono::steady_clock::now();
+        const int rc = jit_compile(src, np, &j->mod, j->fns);
+        j->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        j->state.store(rc ? 2 : 1, std::memory_order_release);
+    });
+    e.job = std::move(job);
+}
+
+int running_compiles(const qse_ctx* ctx) {
+    int r = 0;
+    for (const auto& e : ctx->struct_cache)
+        if (e.job && e.job->state.load(std::memory_order_acquire) == 0) ++r;
+    return r;
+}
+
+// Wait for the entry's worker and release its module (kernels from it may be queued).
+int drop_entry(qse_ctx* ctx, StructEntry& e) {
+    if (!e.job) return QSE_OK;
+    if (e.job->worker.joinable()) e.job->worker.join();
+    if (e.job->mod) {
+        QSE_HIP(hipStreamSynchronize(ctx->stream));
+        QSE_HIP(hipModuleUnload(e.job->mod));
+        e.job->mod = nullptr;
+    }
+    return QSE_OK;
+}
+
+}  // namespace
+
+int jit_for_structure(qse_ctx* ctx, const HostPlan& P, const Serialized& S, int prec,
+                      const std::vector<hipFunction_t>** fns) {
+    *fns = nullptr;
+    if (ctx->jit_mode == QSE_JIT_OFF || P.passes.empty()) return QSE_OK;
+    std::vector<char> key = structure_key(P);
+    StructEntry* hit = nullptr;
+    for (auto& e : ctx->struct_cache)
+        if (e.prec == prec && e.key.size() == key.size() && std::memcmp(e.key.data(), key.data(), key.size()) == 0) {
+            hit = &e;
+            break;
+        }
+    if (!hit) {
+        if (ctx->struct_cache.size() >= STRUCT_CACHE_MAX) {
+            size_t oldest = 0;
+            for (size_t i = 1; i < ctx->struct_cache.size(); ++i)
+                if (ctx->struct_cache[i].stamp < ctx->struct_cache[oldest].stamp) oldest = i;
+            const int rc = drop_entry(ctx, ctx->struct_cache[oldest]);
+            if (rc) return rc;
+            ctx->struct_cache.erase(ctx->struct_cache.begin() + oldest);
+        }
+        StructEntry e;
+        e.key = std::move(key);
+        e.prec = prec;
+        ctx->struct_cache.push_back(std::move(e));
+        hit = &ctx->struct_cache.back();
+    }
+    hit->stamp = ++ctx->prog_clock;
+    ++hit->seen;
+    if (!hit->job) {
+        if (ctx->jit_mode == QSE_JIT_ALWAYS) {
+            start_compile(ctx, *hit, P, S, prec);
+            hit->job->worker.join();
+            if (hit->job->state.load() != 1)
+                return set_error(QSE_ERR_HIP, "run-time compilation of a pass structure failed");
+        } else if (hit->seen >= 2 && running_compiles(ctx) < MAX_COMPILE_WORKERS) {
+            start_compile(ctx, *hit, P, S, prec);
+        }
+    }
+    if (hit->job && hit->job->state.load(std::memory_order_acquire) == 1) {
+        if (!hit->counted) {
+            hit->counted = true;
+            ++ctx->jit_compiled;
+            ctx->jit_compile_ms += hit->job->ms;
+        }
+        *fns = &hit->job->fns;
+    }
+    return QSE_OK;
+}
+
+// Wait for every background compilation of the context.
+int jit_wait_all(qse_ctx* ctx) {
+    for (auto& e : ctx->struct_cache)
+        if (e.job && e.job->worker.joinable()) e.job->worker.join();
+    return QSE_OK;
+}
+
+// Release every structure-keyed module (context teardown).
+void jit_release_all(qse_ctx* ctx) {
+    for (auto& e : ctx->struct_cache) {
+        if (!e.job) continue;
+        if (e.job->worker.joinable()) e.job->worker.join();
+        if (e.job->mod) hipModuleUnload(e.job->mod);
+        e.job->mod = nullptr;
+    }
+    ctx->struct_cache.clear();
+}
+
 }  // namespace qse
+
+extern "C" int qse_ctx_jit_wait(qse_ctx* ctx) {
+    if (!ctx) return qse::set_error(QSE_ERR_INVALID_ARG, "null context");
+    return qse::jit_wait_all(ctx);
+}
 
 extern "C" int qse_ctx_set_jit(qse_ctx* ctx, int mode) {
     if (!ctx || mode < QSE_JIT_OFF || mode > QSE_JIT_ALWAYS)

@@ -11,6 +11,7 @@
 // agree to rounding (only floating-point contraction may differ). This file
 // uses no HIP runtime call: the CPU tests compile its output with a host shim.
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <sstream>
 #include <string>
@@ -69,12 +70,30 @@ void emit_mul_g(std::ostringstream& o, const std::string& dr, const std::string&
 }
 
 // One pass as kernel `name`.
-void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::string& name, bool prefetch) {
+// With `toff` (the serialized program's array offsets), every coefficient is
+// read from the program on the device (pv, its base) instead of being a
+// literal, so one compiled module serves all plans of the same structure.
+void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::string& name, bool prefetch,
+               const size_t* toff) {
     const FPass& fp = P.passes[pi];
+    // coefficient expressions (double): literal, or a load from the device program
+    auto opv = [&](uint32_t oi, int j) -> std::string {
+        if (!toff) return lit(P.ops[oi].m[j]);
+        return "pv[" + std::to_string((toff[2] + oi * sizeof(FOp) + offsetof(FOp, m)) / 8 + j) + "]";
+    };
+    auto termv = [&](uint32_t ti, int c) -> std::string {
+        if (!toff) return lit(P.terms[ti].d[c]);
+        return "pv[" + std::to_string((toff[3] + ti * sizeof(FTerm) + offsetof(FTerm, d)) / 8 + c) + "]";
+    };
+    auto alphav = [&](uint32_t ai) -> std::string {
+        if (!toff) return lit(P.alphas[ai]);
+        return "pv[" + std::to_string(toff[6] / 8 + ai) + "]";
+    };
     const int K = fp.k, TB = K - 4, NT = 1 << TB;
     const int nslots = fp.nslots;
     o << "extern \"C\" __global__ __launch_bounds__(" << NT << ") __attribute__((amdgpu_waves_per_eu(2, 8)))\n"
-      << "void " << name << "(const cx* src, cx* dst, const SlotVal* __restrict__ slot_table, unsigned long long ntiles) {\n"
+      << "void " << name << "(const cx* src, cx* dst, const SlotVal* __restrict__ slot_table, unsigned long long ntiles, "
+      << "const double* __restrict__ pv) {\n"
       << "  __shared__ cx tile[" << (1 << K) << "];\n"
       << "  __shared__ SlotVal slotv[" << (nslots > 0 ? nslots : 1) << "];\n"
       << "  const unsigned tid = threadIdx.x;\n"
@@ -153,11 +172,11 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
             const FOp& op = P.ops[oi];
             if (op.type == FOP_MONO) {
                 // constant x thread-bit factors (ascending bit) x terms, as in the generic kernel
-                o << "      {\n      double gr = " << lit(op.m[0]) << ", gi = " << lit(op.m[1]) << ";\n";
+                o << "      {\n      double gr = " << opv(oi, 0) << ", gi = " << opv(oi, 1) << ";\n";
                 for (int t = 0; t < TB; ++t)
                     if ((op.tctl >> t) & 1) {
                         o << "      if (tid & " << (1u << t) << "u)\n";
-                        emit_mul_g(o, lit(P.alphas[op.alpha_begin + 2 * t]), lit(P.alphas[op.alpha_begin + 2 * t + 1]));
+                        emit_mul_g(o, alphav(op.alpha_begin + 2 * t), alphav(op.alpha_begin + 2 * t + 1));
                     }
                 for (uint32_t ti = op.term_begin; ti < op.term_end; ++ti) {
                     const FTerm& T = P.terms[ti];
@@ -168,7 +187,7 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
                         const std::string sv = "slotv[" + std::to_string(T.slot) + "]";
                         emit_mul_g(o, sv + ".c", sv + ".s");
                     } else {
-                        emit_mul_g(o, lit(T.d[0]), lit(T.d[1]));
+                        emit_mul_g(o, termv(ti, 0), termv(ti, 1));
                     }
                 }
                 o << "      const R g_r = (R)gr, g_i = (R)gi;\n";
@@ -208,7 +227,29 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
                 continue;
             }
             o << (guard ? "      if (" + cond + ") {\n" : std::string("      {\n"));
+            if (toff && op.type != FOP_H) {
+                o << "      const R";
+                for (int j = 0; j < 8; ++j) o << (j ? ", " : " ") << "c" << j << " = (R)" << opv(oi, j);
+                o << ";\n";
+            }
             for (auto& pr : prs) {
+                if (toff && op.type != FOP_H) {
+                    // the same products and sums as below, with every coefficient from the table
+                    const std::string a = std::to_string(nm[pr.first]), b = std::to_string(nm[pr.second]);
+                    o << "      { const R x0 = ar" << a << ", y0 = ai" << a << ", x1 = ar" << b << ", y1 = ai" << b << ";\n";
+                    if (op.type == FOP_REAL)
+                        o << "        ar" << a << " = c0 * x0 + c2 * x1; ai" << a << " = c0 * y0 + c2 * y1;\n        ar"
+                          << b << " = c4 * x0 + c6 * x1; ai" << b << " = c4 * y0 + c6 * y1; }\n";
+                    else if (op.type == FOP_RXLIKE)
+                        o << "        ar" << a << " = c0 * x0 - c3 * y1; ai" << a << " = c0 * y0 + c3 * x1;\n        ar"
+                          << b << " = c6 * x1 - c5 * y0; ai" << b << " = c6 * y1 + c5 * x0; }\n";
+                    else
+                        o << "        ar" << a << " = (c0 * x0 - c1 * y0) + (c2 * x1 - c3 * y1); ai" << a
+                          << " = (c0 * y0 + c1 * x0) + (c2 * y1 + c3 * x1);\n        ar" << b
+                          << " = (c4 * x0 - c5 * y0) + (c6 * x1 - c7 * y1); ai" << b
+                          << " = (c4 * y0 + c5 * x0) + (c6 * y1 + c7 * x1); }\n";
+                    continue;
+                }
                 const std::string a = std::to_string(nm[pr.first]), b = std::to_string(nm[pr.second]);
                 o << "      { const R x0 = ar" << a << ", y0 = ai" << a << ", x1 = ar" << b << ", y1 = ai" << b << ";\n";
                 switch (op.type) {
@@ -261,7 +302,9 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
 std::string jit_kernel_name(int pass) { return "qse_jit_pass" + std::to_string(pass); }
 
 // HIP source of every pass of plan P (kernel qse_jit_pass<i> for pass i).
-std::string jit_source(const HostPlan& P, int prec) {
+// With table_off (the offsets of P's serialized program) the coefficients are
+// read from the program buffer passed as the kernels' last argument.
+std::string jit_source(const HostPlan& P, int prec, const size_t* table_off) {
     std::ostringstream o;
     o << "typedef " << (prec == QSE_C128 ? "double" : "float") << " R;\n"
       << "struct __attribute__((aligned(2 * sizeof(R)))) cx { R re, im; };\n"
@@ -285,7 +328,8 @@ std::string jit_source(const HostPlan& P, int prec) {
     else
         o << "__device__ __forceinline__ cx ldnt(const cx* p) { return *p; }\n"
              "__device__ __forceinline__ void stnt(cx* p, cx a) { *p = a; }\n\n";
-    for (int p = 0; p < (int)P.passes.size(); ++p) emit_pass(o, P, p, jit_kernel_name(p), prec == QSE_C64);
+    for (int p = 0; p < (int)P.passes.size(); ++p)
+        emit_pass(o, P, p, jit_kernel_name(p), prec == QSE_C64, table_off);
     return o.str();
 }
 

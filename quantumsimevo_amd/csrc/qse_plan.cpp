@@ -25,6 +25,7 @@
 // per-tile factors computed once per tile.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <complex>
 #include <map>
 #include <set>
@@ -755,6 +756,36 @@ int tuned_plan(const qse_ctx* ctx, int n, int prec, const qse_op* ops, int nops,
     *k_out = k0;
     *L_out = L0;
     return choose_plan(n, k0, L0, ops, nops, params, P, allow_oop);
+}
+
+static size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+void serialize(const HostPlan& P, Serialized& S) {
+    const size_t sz[7] = {sizeof(FPass) * P.passes.size(), sizeof(FStage) * P.stages.size(),
+                          sizeof(FOp) * P.ops.size(),       sizeof(FTerm) * P.terms.size(),
+                          sizeof(FSlot) * P.slots.size(),   sizeof(FExtTerm) * P.eterms.size(),
+                          sizeof(double) * P.alphas.size()};
+    const void* src[7] = {P.passes.data(), P.stages.data(), P.ops.data(),   P.terms.data(),
+                          P.slots.data(),  P.eterms.data(), P.alphas.data()};
+    size_t tot = 0;
+    for (int i = 0; i < 7; ++i) {
+        S.off[i] = tot;
+        tot += al256(std::max<size_t>(sz[i], 1));
+    }
+    S.bytes.assign(tot, 0);
+    for (int i = 0; i < 7; ++i)
+        if (sz[i]) std::memcpy(S.bytes.data() + S.off[i], src[i], sz[i]);
+}
+
+std::vector<char> structure_key(const HostPlan& P) {
+    HostPlan Z = P;
+    for (FOp& o : Z.ops) std::fill(o.m, o.m + 8, 0.0);
+    for (FTerm& t : Z.terms) t.d[0] = t.d[1] = t.ang = 0.0;
+    for (FExtTerm& e : Z.eterms) e.ang = 0.0;
+    std::fill(Z.alphas.begin(), Z.alphas.end(), 0.0);
+    Serialized S;
+    serialize(Z, S);
+    return std::move(S.bytes);
 }
 
 }  // namespace qse

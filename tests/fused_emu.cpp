@@ -265,20 +265,44 @@ extern "C" int emu_plan_stats(int n, int k, int L, const qse_op* ops, int nops, 
 // it is not realisable), keeps the plan for the calls below and returns the
 // length of the generated HIP source, copied into `out` when `cap` allows.
 namespace qse {
-std::string jit_source(const HostPlan& P, int prec);
+std::string jit_source(const HostPlan& P, int prec, const size_t* table_off);
 }
 static HostPlan g_jit_plan;
+static Serialized g_jit_prog;
 static int g_jit_n = 0;
 
-extern "C" long emu_jit_prepare(int n, int k, int L, const qse_op* ops, int nops, const double* params, int relabel,
-                                int prec, char* out, long cap) {
+// table != 0: coefficients read from the serialized program (emu_jit_program)
+// instead of literals.
+extern "C" long emu_jit_prepare2(int n, int k, int L, const qse_op* ops, int nops, const double* params, int relabel,
+                                 int prec, int table, char* out, long cap) {
     g_jit_plan = HostPlan();
     int rc = plan_for(n, k, L, ops, nops, params, relabel, g_jit_plan);
     if (rc) return -rc;
     g_jit_n = n;
-    const std::string src = jit_source(g_jit_plan, prec);
+    serialize(g_jit_plan, g_jit_prog);
+    const std::string src = jit_source(g_jit_plan, prec, table ? g_jit_prog.off : nullptr);
     if (out && cap > (long)src.size()) std::memcpy(out, src.c_str(), src.size() + 1);
     return (long)src.size();
+}
+
+extern "C" long emu_jit_prepare(int n, int k, int L, const qse_op* ops, int nops, const double* params, int relabel,
+                                int prec, char* out, long cap) {
+    return emu_jit_prepare2(n, k, L, ops, nops, params, relabel, prec, 0, out, cap);
+}
+
+// The prepared plan's serialized program (what the device program buffer
+// holds), its size in bytes when out is null.
+extern "C" long emu_jit_program(char* out) {
+    if (out) std::memcpy(out, g_jit_prog.bytes.data(), g_jit_prog.bytes.size());
+    return (long)g_jit_prog.bytes.size();
+}
+
+// Structure keys of two plans compare equal (1) or not (0).
+extern "C" int emu_same_structure(int n, int k, int L, const qse_op* ops_a, const double* params_a,
+                                  const qse_op* ops_b, const double* params_b, int nops) {
+    HostPlan A, B;
+    if (plan_for(n, k, L, ops_a, nops, params_a, -1, A) || plan_for(n, k, L, ops_b, nops, params_b, -1, B)) return -1;
+    return structure_key(A) == structure_key(B) ? 1 : 0;
 }
 
 // Pass count of the prepared plan, and per pass: tile qubits, out-of-place flag, slots.

@@ -5,11 +5,11 @@ shim_dim3 blockIdx, gridDim;
 static pthread_barrier_t shim_bar;
 void shim_syncthreads() { pthread_barrier_wait(&shim_bar); }
 
-typedef void (*shim_kernel)(const cx*, cx*, const SlotVal*, unsigned long long);
+typedef void (*shim_kernel)(const cx*, cx*, const SlotVal*, unsigned long long, const double*);
 extern shim_kernel shim_kernels[];
 
 extern "C" int shim_launch(int pass, int nthreads, const void* src, void* dst, const void* table,
-                           unsigned long long ntiles) {
+                           unsigned long long ntiles, const void* prog) {
     pthread_barrier_init(&shim_bar, nullptr, (unsigned)nthreads);
     blockIdx = {0, 0, 0};
     gridDim = {1, 1, 1};
@@ -18,7 +18,7 @@ extern "C" int shim_launch(int pass, int nthreads, const void* src, void* dst, c
     for (int t = 0; t < nthreads; ++t)
         th.emplace_back([=] {
             threadIdx = {(unsigned)t, 0, 0};
-            shim_kernels[pass]((const cx*)src, (cx*)dst, (const SlotVal*)table, ntiles);
+            shim_kernels[pass]((const cx*)src, (cx*)dst, (const SlotVal*)table, ntiles, (const double*)prog);
         });
     for (auto& x : th) x.join();
     pthread_barrier_destroy(&shim_bar);

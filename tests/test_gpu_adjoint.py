@@ -151,3 +151,45 @@ def test_more_than_64_terms_in_one_block(ctx):
     assert np.max(np.abs(Gr[0] - gs)) < 1e-10 * max(1.0, float(np.max(np.abs(gs))))
     E32, G32 = ctx.adjoint_expect(n, capi.C64, ops, params, masks, w)
     assert abs(E32[0] - E[0]) < 1e-4 and np.max(np.abs(G32 - Gr)) < 1e-3
+
+
+def test_structure_keyed_kernels():
+    """The adjoint's internal runs through compiled kernels keyed by the plan
+    structure (coefficients read from the program buffer): compiled on first
+    use (JIT_ALWAYS) or on worker threads from the second sighting (JIT_AUTO);
+    both agree with the generic kernel and with the oracle."""
+    n, p = 14, 2
+    edges = G.random_regular_graph(n, 3, seed=11)
+    ops = G.qaoa_maxcut(n, edges, p).ops()
+    masks, w = G.maxcut_observable(edges)
+    params = np.random.default_rng(8).random((4, 2 * p)) * math.pi
+    c = capi.Context(0)
+    try:
+        c.set_jit(capi.JIT_OFF)
+        E0, G0 = c.adjoint_expect(n, capi.C128, ops, params, masks, w)
+        assert c.jit_stats()["launches"] == 0
+        c.set_jit(capi.JIT_ALWAYS)
+        E1, G1 = c.adjoint_expect(n, capi.C128, ops, params, masks, w)
+        st = c.jit_stats()
+        assert st["compiled"] > 0 and st["launches"] > 0
+        tol = 1e-10 * max(1.0, float(np.max(np.abs(G0))))
+        assert np.max(np.abs(E1 - E0)) < 1e-11 and np.max(np.abs(G1 - G0)) < tol
+        assert np.max(np.abs(G1[0] - oracle_shift(n, ops, params[0], masks, w))) < tol
+        E32, G32 = c.adjoint_expect(n, capi.C64, ops, params, masks, w)
+        assert np.max(np.abs(E32 - E0)) < 1e-4 and np.max(np.abs(G32 - G0)) < 1e-3
+        Eb = c.batch_expect(n, capi.C128, ops, params, masks, w)
+        assert np.max(np.abs(Eb - E0)) < 1e-11
+    finally:
+        c.close()
+    c = capi.Context(0)
+    try:
+        c.set_jit(capi.JIT_AUTO)
+        E2, G2 = c.adjoint_expect(n, capi.C128, ops, params, masks, w)  # starts the compilations
+        c.jit_wait()
+        launches = c.jit_stats()["launches"]
+        E3, G3 = c.adjoint_expect(n, capi.C128, ops, params, masks, w)
+        assert c.jit_stats()["launches"] > launches
+        for E, Gr in ((E2, G2), (E3, G3)):
+            assert np.max(np.abs(E - E0)) < 1e-11 and np.max(np.abs(Gr - G0)) < tol
+    finally:
+        c.close()

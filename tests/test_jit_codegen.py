@@ -22,17 +22,28 @@ from test_fused_plan import emu
 SHIM = O.ROOT / "tests" / "jit_shim"
 
 
-def _prepare(n, k, L, ops, params, relabel, prec):
+def _prepare(n, k, L, ops, params, relabel, prec, table=0):
     lib = emu()
-    lib.emu_jit_prepare.restype = C.c_long
-    lib.emu_jit_prepare.argtypes = [C.c_int] * 3 + [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_char_p,
-                                                     C.c_long]
+    lib.emu_jit_prepare2.restype = C.c_long
+    lib.emu_jit_prepare2.argtypes = [C.c_int] * 3 + [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                                      C.c_char_p, C.c_long]
     pp = None if params is None else np.ascontiguousarray(params, np.float64).ctypes.data
-    need = lib.emu_jit_prepare(n, k, L, ops.ctypes.data, len(ops), pp, relabel, prec, None, 0)
+    need = lib.emu_jit_prepare2(n, k, L, ops.ctypes.data, len(ops), pp, relabel, prec, table, None, 0)
     assert need > 0, f"planner status {-need}"
     buf = C.create_string_buffer(need + 1)
-    assert lib.emu_jit_prepare(n, k, L, ops.ctypes.data, len(ops), pp, relabel, prec, buf, need + 1) == need
+    assert lib.emu_jit_prepare2(n, k, L, ops.ctypes.data, len(ops), pp, relabel, prec, table, buf, need + 1) == need
     return buf.value.decode()
+
+
+def _program():
+    """Serialized program of the prepared plan (the device program buffer), 8-byte aligned."""
+    lib = emu()
+    lib.emu_jit_program.restype = C.c_long
+    lib.emu_jit_program.argtypes = [C.c_void_p]
+    size = lib.emu_jit_program(None)
+    buf = np.zeros((size + 7) // 8, np.float64)
+    lib.emu_jit_program(buf.ctypes.data)
+    return buf
 
 
 def _passes():
@@ -54,16 +65,18 @@ def _compile_host(src, npasses, tmp):
                         "-ffp-contract=off", "-o", str(so), str(cpp)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-4000:]
     lib = C.CDLL(str(so))
-    lib.shim_launch.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_ulonglong]
+    lib.shim_launch.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_ulonglong, C.c_void_p]
     return lib
 
 
-def run_jit(n, k, L, ops, psi, params=None, relabel=-1, prec=0):
-    """Generated program executed on the host; returns the final state (complex128)."""
+def run_jit(n, k, L, ops, psi, params=None, relabel=-1, prec=0, table=0):
+    """Generated program executed on the host; returns the final state (complex128).
+    table=1: the structure-keyed form, coefficients read from the serialized program."""
     pp = None if params is None else np.ascontiguousarray(params, np.float64)
     assert emu().emu_check_plan(n, k, L, ops.ctypes.data, len(ops), None if pp is None else pp.ctypes.data,
                                 relabel) == 0
-    src = _prepare(n, k, L, ops, params, relabel, prec)
+    src = _prepare(n, k, L, ops, params, relabel, prec, table)
+    prog = _program()
     passes = _passes()
     dt = np.complex128 if prec == 0 else np.complex64
     with tempfile.TemporaryDirectory() as tmp:
@@ -77,18 +90,18 @@ def run_jit(n, k, L, ops, psi, params=None, relabel=-1, prec=0):
                 emu().emu_jit_slot_table(p, table.ctypes.data_as(C.c_void_p))
             dst = other if oop else cur
             assert lib.shim_launch(p, 1 << (kp - 4), cur.ctypes.data, dst.ctypes.data, table.ctypes.data,
-                                   ntiles) == 0
+                                   ntiles, prog.ctypes.data) == 0
             if oop:
                 cur, other = other, cur
     return cur.astype(np.complex128), src, passes
 
 
-def check(n, k, L, ops, params=None, prec=0, seed=3, relabels=(-1, 0)):
+def check(n, k, L, ops, params=None, prec=0, seed=3, relabels=(-1, 0), table=0):
     psi = G.random_state(n, seed)
     want = O.run_ops(psi, n, ops, params)
     tol = 1e-12 if prec == 0 else 2e-5
     for rl in relabels:
-        got, _, passes = run_jit(n, k, L, ops, psi, params, rl, prec)
+        got, _, passes = run_jit(n, k, L, ops, psi, params, rl, prec, table)
         err = float(np.max(np.abs(got - want)))
         assert err < tol, (n, k, L, rl, prec, err, passes)
 
@@ -136,8 +149,8 @@ def test_hiprtc_compiles_for_gfx950():
     if not os.path.exists(rtc_path):
         pytest.skip("hiprtc not installed")
     rtc = C.CDLL(rtc_path)
-    for prec in (0, 1):
-        src = _prepare(14, 12, 6, G.qft(14).ops(), None, -1, prec)
+    for prec, table in ((0, 0), (1, 0), (0, 1), (1, 1)):
+        src = _prepare(14, 12, 6, G.qft(14).ops(), None, -1, prec, table)
         prog = C.c_void_p()
         assert rtc.hiprtcCreateProgram(C.byref(prog), src.encode(), b"t.hip", 0, None, None) == 0
         opts = [b"--offload-arch=gfx950", b"-O3", b"-Xclang", b"-target-feature", b"-Xclang", b"-packed-fp32-ops"]
@@ -152,3 +165,32 @@ def test_hiprtc_compiles_for_gfx950():
         size = C.c_size_t()
         assert rtc.hiprtcGetCodeSize(prog, C.byref(size)) == 0 and size.value > 0
         rtc.hiprtcDestroyProgram(C.byref(prog))
+
+
+def test_structure_keyed_form():
+    """Coefficients read from the program buffer (the kernels the adjoint's
+    internal runs use): every gate kind, slots and terms, both precisions."""
+    n = 12
+    ops = G.random_mixed(n, 300, 5).ops()
+    check(n, 8, 4, ops, seed=6, relabels=(-1,), table=1)
+    edges = G.random_regular_graph(n, 3, seed=1)
+    q = G.qaoa_maxcut(n, edges, 3).ops()
+    params = np.random.default_rng(2).random(6) * math.pi
+    check(n, 8, 2, q, params, relabels=(-1,), table=1)
+    check(n, 9, 3, G.random_layered(n, 5, seed=9).ops(), prec=1, relabels=(-1,), table=1)
+
+
+def test_structure_key_ignores_values():
+    lib = emu()
+    lib.emu_same_structure.argtypes = [C.c_int] * 3 + [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    n = 12
+    edges = G.random_regular_graph(n, 3, seed=1)
+    q = G.qaoa_maxcut(n, edges, 2).ops()
+    p1 = np.random.default_rng(3).random(4) * math.pi
+    p2 = np.random.default_rng(4).random(4) * math.pi
+    assert lib.emu_same_structure(n, 9, 3, q.ctypes.data, p1.ctypes.data, q.ctypes.data, p2.ctypes.data, len(q)) == 1
+    # another graph (same gate count) is another structure
+    other = G.qaoa_maxcut(n, G.random_regular_graph(n, 3, seed=2), 2).ops()
+    assert len(other) == len(q)
+    assert lib.emu_same_structure(n, 9, 3, q.ctypes.data, p1.ctypes.data, other.ctypes.data, p1.ctypes.data,
+                                  len(q)) == 0
