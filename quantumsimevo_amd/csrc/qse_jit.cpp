@@ -22,11 +22,11 @@
 
 namespace qse {
 
-std::string jit_kernel_name(int pass);
+std::string jit_kernel_name(int pass, bool table);
 std::string jit_source(const HostPlan& P, int prec, const size_t* table_off);
 
-// Compile `src` for gfx950 and load it; one function per pass.
-int jit_compile(const std::string& src, int npasses, hipModule_t* mod, std::vector<hipFunction_t>& fns) {
+// Compile `src` for gfx950 and load it; one function per pass (names per jit_kernel_name).
+int jit_compile(const std::string& src, int npasses, bool table, hipModule_t* mod, std::vector<hipFunction_t>& fns) {
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "qse_jit_program.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
         return set_error(QSE_ERR_HIP, "hiprtcCreateProgram failed");
@@ -49,7 +49,7 @@ int jit_compile(const std::string& src, int npasses, hipModule_t* mod, std::vect
     QSE_HIP(hipModuleLoadData(mod, code.data()));
     fns.assign(npasses, nullptr);
     for (int p = 0; p < npasses; ++p) {
-        const hipError_t e = hipModuleGetFunction(&fns[p], *mod, jit_kernel_name(p).c_str());
+        const hipError_t e = hipModuleGetFunction(&fns[p], *mod, jit_kernel_name(p, table).c_str());
         if (e != hipSuccess) {
             hipModuleUnload(*mod);
             *mod = nullptr;
@@ -82,7 +82,7 @@ void start_compile(qse_ctx* ctx, StructEntry& e, const HostPlan& P, const Serial
     j->worker = std::thread([j, src = std::move(src), np, dev] {
         hipSetDevice(dev);
         const auto t0 = std::chrono::steady_clock::now();
-        const int rc = jit_compile(src, np, &j->mod, j->fns);
+        const int rc = jit_compile(src, np, true, &j->mod, j->fns);
         j->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         j->state.store(rc ? 2 : 1, std::memory_order_release);
     });

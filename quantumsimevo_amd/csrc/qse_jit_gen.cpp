@@ -299,7 +299,12 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
 
 }  // namespace
 
-std::string jit_kernel_name(int pass) { return "qse_jit_pass" + std::to_string(pass); }
+// Kernel name of pass `pass`: qse_jit_pass<i> (literal coefficients) or
+// qse_jit_tpass<i> (coefficients from the program buffer), so profiles tell
+// the two apart.
+std::string jit_kernel_name(int pass, bool table) {
+    return (table ? "qse_jit_tpass" : "qse_jit_pass") + std::to_string(pass);
+}
 
 // HIP source of every pass of plan P (kernel qse_jit_pass<i> for pass i).
 // With table_off (the offsets of P's serialized program) the coefficients are
@@ -329,7 +334,7 @@ std::string jit_source(const HostPlan& P, int prec, const size_t* table_off) {
         o << "__device__ __forceinline__ cx ldnt(const cx* p) { return *p; }\n"
              "__device__ __forceinline__ void stnt(cx* p, cx a) { *p = a; }\n\n";
     for (int p = 0; p < (int)P.passes.size(); ++p)
-        emit_pass(o, P, p, jit_kernel_name(p), prec == QSE_C64, table_off);
+        emit_pass(o, P, p, jit_kernel_name(p, table_off != nullptr), prec == QSE_C64, table_off);
     return o.str();
 }
 

@@ -56,8 +56,9 @@ def _passes():
     return out
 
 
-def _compile_host(src, npasses, tmp):
-    table = "shim_kernel shim_kernels[] = {" + ", ".join(f"qse_jit_pass{p}" for p in range(npasses)) + "};\n"
+def _compile_host(src, npasses, tmp, table_form=0):
+    name = "qse_jit_tpass" if table_form else "qse_jit_pass"
+    table = "shim_kernel shim_kernels[] = {" + ", ".join(f"{name}{p}" for p in range(npasses)) + "};\n"
     cpp = Path(tmp) / "jit_host.cpp"
     cpp.write_text(f'#include "{SHIM / "prelude.hpp"}"\n' + src + f'#include "{SHIM / "epilogue.hpp"}"\n' + table)
     so = Path(tmp) / "libjit_host.so"
@@ -80,7 +81,7 @@ def run_jit(n, k, L, ops, psi, params=None, relabel=-1, prec=0, table=0):
     passes = _passes()
     dt = np.complex128 if prec == 0 else np.complex64
     with tempfile.TemporaryDirectory() as tmp:
-        lib = _compile_host(src, len(passes), tmp)
+        lib = _compile_host(src, len(passes), tmp, table)
         cur = np.ascontiguousarray(psi, dtype=dt).copy()
         other = np.zeros_like(cur)
         for p, (kp, oop, ns) in enumerate(passes):
