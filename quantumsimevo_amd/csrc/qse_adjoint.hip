@@ -107,7 +107,11 @@ struct DiagMasksN {
     uint64_t m[NM];
 };
 
-template <class R, int NM>
+// WHT_GRAD: w_i = Im(conj(lambda_i) psi_i) (generator terms); WHT_PROB:
+// w_i = |psi_i|^2 (expectation values; lam unused).
+enum WhtMode { WHT_GRAD = 0, WHT_PROB = 1 };
+
+template <class R, int NM, int MODE>
 __global__ __launch_bounds__(AD_THREADS) void k_grad_diag_wht(const cx<R>* __restrict__ psi,
                                                               const cx<R>* __restrict__ lam, uint64_t N, int log2S,
                                                               DiagMasksN<NM> M, double* __restrict__ part) {
@@ -127,8 +131,13 @@ __global__ __launch_bounds__(AD_THREADS) void k_grad_diag_wht(const cx<R>* __res
 #pragma unroll
         for (int j = 0; j < WHT_J; ++j) {
             const uint64_t i = t + S * ((uint64_t)j + WHT_J * k);
-            const cx<R> p = psi[i], l = lam[i];
-            w[j] = (double)l.re * (double)p.im - (double)l.im * (double)p.re;
+            const cx<R> p = psi[i];
+            if constexpr (MODE == WHT_PROB) {
+                w[j] = prob(p);
+            } else {
+                const cx<R> l = lam[i];
+                w[j] = (double)l.re * (double)p.im - (double)l.im * (double)p.re;
+            }
         }
 #pragma unroll
         for (int h = 1; h < WHT_J; h <<= 1)
@@ -205,19 +214,71 @@ int launch_apply_obs(qse_ctx* ctx, qse_state* lam, const uint64_t* dm, const dou
     return QSE_OK;
 }
 
-template <class R, int NM>
+template <class R, int NM, int MODE>
 int launch_wht(qse_ctx* ctx, const qse_state* psi, const qse_state* lam, const uint64_t* masks, int cnt,
                unsigned grid, int log2S, double* pbuf, double* d_out) {
     DiagMasksN<NM> M{};
     for (int d = 0; d < cnt; ++d) M.m[d] = masks[d];
     const uint64_t N = 1ULL << psi->n;
     TimedLaunch t;
-    launch_begin(ctx, FAM_ADJOINT, &t);
-    k_grad_diag_wht<R, NM><<<grid, AD_THREADS, 0, ctx->stream>>>((const cx<R>*)psi->d, (const cx<R>*)lam->d, N, log2S,
-                                                                 M, pbuf);
+    launch_begin(ctx, MODE == WHT_PROB ? FAM_READOUT : FAM_ADJOINT, &t);
+    k_grad_diag_wht<R, NM, MODE><<<grid, AD_THREADS, 0, ctx->stream>>>(
+        (const cx<R>*)psi->d, (const cx<R>*)(lam ? lam->d : psi->d), N, log2S, M, pbuf);
     QSE_HIP(hipGetLastError());
-    launch_end(ctx, &t, 2.0 * amp_bytes(psi->prec) * (double)N);
+    launch_end(ctx, &t, (MODE == WHT_PROB ? 1.0 : 2.0) * amp_bytes(psi->prec) * (double)N);
     k_sum_columns<<<(unsigned)cnt, AD_THREADS, 0, ctx->stream>>>(pbuf, (int)grid, NM, d_out);
+    QSE_HIP(hipGetLastError());
+    return QSE_OK;
+}
+
+constexpr int WHT_MIN_QUBITS = 13;
+
+// Z-string sums of `count` masks, 64 per pass, written to d_out[0 .. count) on
+// the device: sum_i (-1)^popcount(i & m) w_i with w_i from MODE.
+template <class R, int MODE>
+int zstrings_wht(qse_ctx* ctx, const qse_state* psi, const qse_state* lam, const uint64_t* masks, int count,
+                 double* d_out) {
+    // grid: a power of two, at most 1024 workgroups, 32 amplitudes per thread and iteration
+    int log2S = psi->n - 5;
+    if (log2S > 18) log2S = 18;  // 1024 x 256 threads
+    const unsigned grid = 1u << (log2S - 8);
+    void* pbuf;
+    int rc = scratch(ctx, 6, sizeof(double) * 64 * grid, &pbuf);
+    if (rc) return rc;
+    for (int c0 = 0; c0 < count; c0 += 64) {
+        const int cnt = std::min(64, count - c0);
+        if (cnt <= 16) rc = launch_wht<R, 16, MODE>(ctx, psi, lam, masks + c0, cnt, grid, log2S, (double*)pbuf, d_out + c0);
+        else if (cnt <= 32) rc = launch_wht<R, 32, MODE>(ctx, psi, lam, masks + c0, cnt, grid, log2S, (double*)pbuf, d_out + c0);
+        else if (cnt <= 48) rc = launch_wht<R, 48, MODE>(ctx, psi, lam, masks + c0, cnt, grid, log2S, (double*)pbuf, d_out + c0);
+        else rc = launch_wht<R, 64, MODE>(ctx, psi, lam, masks + c0, cnt, grid, log2S, (double*)pbuf, d_out + c0);
+        if (rc) return rc;
+    }
+    return QSE_OK;
+}
+
+// *out = sum_k w[k] s[k] in order (one thread).
+__global__ void k_weighted_sum(const double* __restrict__ s, const double* __restrict__ w, int n,
+                               double* __restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double v = 0.0;
+        for (int k = 0; k < n; ++k) v += w[k] * s[k];
+        *out = v;
+    }
+}
+
+// sum_k w_k <psi|Z_{m_k}|psi> written to *d_out on the device: from 13 qubits
+// 64 Z-strings per pass (Walsh-Hadamard reduction), below one pass over all
+// terms per amplitude.
+template <class R>
+int energy_async(qse_ctx* ctx, const qse_state* psi, const uint64_t* h_m, const uint64_t* d_m, const double* d_w,
+                 int nterms, double* d_out) {
+    if (psi->n < WHT_MIN_QUBITS || nterms == 0) return reduce_zstr_async(ctx, psi, d_m, d_w, nterms, d_out);
+    void* sbuf;
+    int rc = scratch(ctx, 9, sizeof(double) * (size_t)nterms, &sbuf);
+    if (rc) return rc;
+    rc = zstrings_wht<R, WHT_PROB>(ctx, psi, nullptr, h_m, nterms, (double*)sbuf);
+    if (rc) return rc;
+    k_weighted_sum<<<1, 64, 0, ctx->stream>>>((const double*)sbuf, d_w, nterms, d_out);
     QSE_HIP(hipGetLastError());
     return QSE_OK;
 }
@@ -229,23 +290,7 @@ int grad_diag_async(qse_ctx* ctx, qse_state* psi, qse_state* lam, const std::vec
                     double* d_out) {
     const uint64_t N = 1ULL << psi->n;
     int rc;
-    if (psi->n >= 13) {
-        // grid: a power of two, at most 1024 workgroups, 32 amplitudes per thread and iteration
-        int log2S = psi->n - 5;
-        if (log2S > 18) log2S = 18;  // 1024 x 256 threads
-        const unsigned grid = 1u << (log2S - 8);
-        void* pbuf;
-        rc = scratch(ctx, 6, sizeof(double) * 64 * grid, &pbuf);
-        if (rc) return rc;
-        for (size_t c0 = 0; c0 < masks.size(); c0 += 64) {
-            const int cnt = (int)std::min<size_t>(64, masks.size() - c0);
-            if (cnt <= 16) rc = launch_wht<R, 16>(ctx, psi, lam, masks.data() + c0, cnt, grid, log2S, (double*)pbuf, d_out + c0);
-            else if (cnt <= 32) rc = launch_wht<R, 32>(ctx, psi, lam, masks.data() + c0, cnt, grid, log2S, (double*)pbuf, d_out + c0);
-            else rc = launch_wht<R, 64>(ctx, psi, lam, masks.data() + c0, cnt, grid, log2S, (double*)pbuf, d_out + c0);
-            if (rc) return rc;
-        }
-        return QSE_OK;
-    }
+    if (psi->n >= WHT_MIN_QUBITS) return zstrings_wht<R, WHT_GRAD>(ctx, psi, lam, masks.data(), (int)masks.size(), d_out);
     const unsigned g = std::min<unsigned>(grid_for(ctx, N, AD_THREADS, 8), 1024u);
     void* pbuf;
     rc = scratch(ctx, 6, sizeof(double) * DIAG_CHUNK * g, &pbuf);
@@ -304,8 +349,9 @@ struct SetSlots {
 // `slots` records which value is which.
 template <class R>
 int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double* params,
-                const std::vector<AdjSegment>& segs, const std::vector<qse_op>& prefix, const uint64_t* dm,
-                const double* dw, int nterms, int ckpt_every, double* d_res, int* next, SetSlots& slots) {
+                const std::vector<AdjSegment>& segs, const std::vector<qse_op>& prefix, const uint64_t* hm,
+                const uint64_t* dm, const double* dw, int nterms, int ckpt_every, double* d_res, int* next,
+                SetSlots& slots) {
     int rc = qse_state_set_basis(ctx, ws.psi, 0);
     if (rc) return rc;
     const int S = (int)segs.size();
@@ -329,7 +375,7 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
         if (rc) return rc;
     }
     slots.energy = (*next)++;
-    rc = reduce_zstr_async(ctx, ws.psi, dm, dw, nterms, d_res + slots.energy);
+    rc = energy_async<R>(ctx, ws.psi, hm, dm, dw, nterms, d_res + slots.energy);
     if (rc) return rc;
     rc = qse_state_copy(ctx, ws.lam, ws.psi);
     if (rc) return rc;
@@ -457,9 +503,9 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
         if (values_per_set(segs) != vps)
             return set_error(QSE_ERR_UNSUPPORTED, "adjoint: schedule changed between parameter sets");
         rc = (precision == QSE_C128)
-                 ? adjoint_one<double>(ctx, ws, ops, nops, pb, segs, prefix, dm, dw, nterms, ckpt_every, d_res, &next,
+                 ? adjoint_one<double>(ctx, ws, ops, nops, pb, segs, prefix, obs_masks, dm, dw, nterms, ckpt_every, d_res, &next,
                                        slots[b])
-                 : adjoint_one<float>(ctx, ws, ops, nops, pb, segs, prefix, dm, dw, nterms, ckpt_every, d_res, &next,
+                 : adjoint_one<float>(ctx, ws, ops, nops, pb, segs, prefix, obs_masks, dm, dw, nterms, ckpt_every, d_res, &next,
                                       slots[b]);
         if (rc) return rc;
     }
@@ -508,7 +554,8 @@ int qse_batch_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops
         if (rc) return rc;
         rc = run_on(ctx, {ws.psi}, list, pb);
         if (rc) return rc;
-        rc = reduce_zstr_async(ctx, ws.psi, dm, dw, nterms, d_res + b);
+        rc = (precision == QSE_C128) ? energy_async<double>(ctx, ws.psi, obs_masks, dm, dw, nterms, d_res + b)
+                                     : energy_async<float>(ctx, ws.psi, obs_masks, dm, dw, nterms, d_res + b);
         if (rc) return rc;
     }
     void* pin;
