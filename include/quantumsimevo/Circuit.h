@@ -17,6 +17,7 @@
 #include "Noise.h"
 #include "StateVector.h"
 #include "Structurs.h"
+#include "../qse_hip.h"
 
 class CircuitUnitaryOperationHIP;
 
@@ -54,6 +55,12 @@ public:
     int measureQubit(size_t q);                            // mid-circuit collapse
     void resetQubit(size_t q);
     const std::vector<GateCommand>& commands() const { return commandList; }
+    // Gate-list files (GateListFile.h): the queued gates as C-ABI records, and
+    // saving them / appending a file's gates (same qubit count; parameter
+    // bindings are for the batched and adjoint APIs and are rejected here).
+    std::vector<qse_op> gateList() const;
+    void saveGates(const std::string& path) const;
+    void loadGates(const std::string& path);
     size_t numQubits() const { return stateVector.qubits(); }
     Noise& noise() { return pauliNoise; }
     CircuitUnitaryOperationHIP& backend();

@@ -82,6 +82,56 @@ class GateList:
             a[i] = (k, t, c, p, cm, th, sc, m)
         return a
 
+    # gate-list files (text format of include/quantumsimevo/GateListFile.h)
+    def to_text(self) -> str:
+        return gates_to_text(self.n, self.ops())
+
+    def save(self, path) -> None:
+        save_gates(path, self.n, self.ops())
+
+    @classmethod
+    def from_ops(cls, n: int, ops: np.ndarray) -> "GateList":
+        g = cls(n)
+        for r in np.ascontiguousarray(ops, dtype=OP_DTYPE):
+            g._ops.append((int(r["kind"]), int(r["target"]), int(r["control"]), int(r["param"]),
+                           int(r["controls"]), float(r["theta"]), float(r["scale"]), tuple(float(x) for x in r["m"])))
+        return g
+
+    @classmethod
+    def load(cls, path) -> "GateList":
+        n, ops = load_gates(path)
+        return cls.from_ops(n, ops)
+
+
+# ---- gate-list files: one reader/writer, the C++ one (GateListFile.cpp) ----
+
+def _native():
+    from . import _quantumsimevo
+    return _quantumsimevo
+
+
+def gates_to_text(n: int, ops: np.ndarray) -> str:
+    """Text of a gate list (qse-gates format, version 1)."""
+    a = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    return _native().write_gate_list(int(n), a.tobytes())
+
+
+def gates_from_text(text: str):
+    """(n, ops) of a gate-list text; ValueError-like RuntimeError naming the line when malformed."""
+    n, raw = _native().read_gate_list(text)
+    return n, np.frombuffer(raw, dtype=OP_DTYPE).copy()
+
+
+def save_gates(path, n: int, ops: np.ndarray) -> None:
+    with open(path, "w", encoding="ascii", newline="\n") as f:
+        f.write(gates_to_text(n, ops))
+
+
+def load_gates(path):
+    """(n, ops) from a gate-list file; usable directly as a batch entry (n, ops, params)."""
+    with open(path, "r", encoding="ascii") as f:
+        return gates_from_text(f.read())
+
 
 # ---- circuit families ----
 

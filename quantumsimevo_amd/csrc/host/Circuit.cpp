@@ -7,6 +7,7 @@
 
 #include "quantumsimevo/CircuitUnitaryOperationFactory.h"
 #include "quantumsimevo/CircuitUnitaryOperationHIP.h"
+#include "quantumsimevo/GateListFile.h"
 
 Circuit::Circuit(size_t N, bool simulatePauliNoise, int precision)
     : stateVector(N), simulateNoise(simulatePauliNoise), pauliNoise(N, 42, 1) {
@@ -181,4 +182,98 @@ void Circuit::resetQubit(size_t q) {
     check_q(stateVector, q);
     std::uniform_real_distribution<double> dist(0.0, 1.0);
     backend().collapseQubit(stateVector, q, dist(readoutGen), true);
+}
+
+std::vector<qse_op> Circuit::gateList() const {
+    std::vector<qse_op> out;
+    out.reserve(commandList.size());
+    for (const GateCommand& c : commandList) {
+        qse_op g{};
+        g.target = (int32_t)c.target;
+        g.control = -1;
+        g.param = -1;
+        g.theta = c.theta;
+        switch (c.type) {
+            case GateType::IDENTITY:
+                if (c.controls_list.size() == 1 && c.controls_list[0] == SIZE_MAX) {
+                    g.kind = QSE_GATE_MATRIX;
+                    const Gate2x2& m = matrices[c.resultIdx];
+                    for (int r = 0; r < 2; ++r)
+                        for (int k = 0; k < 2; ++k) {
+                            g.m[4 * r + 2 * k] = m.data[r][k].real();
+                            g.m[4 * r + 2 * k + 1] = m.data[r][k].imag();
+                        }
+                } else {
+                    g.kind = QSE_GATE_IDENTITY;
+                }
+                g.theta = 0.0;
+                break;
+            case GateType::X: g.kind = QSE_GATE_X; break;
+            case GateType::Y: g.kind = QSE_GATE_Y; break;
+            case GateType::Z: g.kind = QSE_GATE_Z; break;
+            case GateType::H: g.kind = QSE_GATE_H; break;
+            case GateType::PHASE: g.kind = QSE_GATE_PHASE; break;
+            case GateType::RX: g.kind = QSE_GATE_RX; break;
+            case GateType::RY: g.kind = QSE_GATE_RY; break;
+            case GateType::RZ: g.kind = QSE_GATE_RZ; break;
+            case GateType::CNOT: g.kind = QSE_GATE_CNOT; g.control = (int32_t)c.control; break;
+            case GateType::CPHASE: g.kind = QSE_GATE_CPHASE; g.control = (int32_t)c.control; break;
+            case GateType::MCPHASE:
+                g.kind = QSE_GATE_MCPHASE;
+                for (size_t q : c.controls_list) g.controls |= 1ULL << q;
+                break;
+            case GateType::SWAP: g.kind = QSE_GATE_SWAP; g.control = (int32_t)c.control; break;
+            default: throw std::runtime_error("gate list: " + c.description + " has no gate-list form");
+        }
+        if (g.kind == QSE_GATE_X || g.kind == QSE_GATE_Y || g.kind == QSE_GATE_Z || g.kind == QSE_GATE_H ||
+            g.kind == QSE_GATE_CNOT || g.kind == QSE_GATE_SWAP)
+            g.theta = 0.0;
+        out.push_back(g);
+    }
+    return out;
+}
+
+void Circuit::saveGates(const std::string& path) const {
+    const std::vector<qse_op> ops = gateList();
+    qse_gates::save(path, (int)numQubits(), ops.data(), ops.size());
+}
+
+void Circuit::loadGates(const std::string& path) {
+    int n = 0;
+    std::vector<qse_op> ops;
+    qse_gates::load(path, &n, ops);
+    if ((size_t)n != numQubits())
+        throw std::runtime_error("gate list " + path + " is for " + std::to_string(n) + " qubits, the circuit has " +
+                                 std::to_string(numQubits()));
+    for (const qse_op& g : ops) {
+        if (g.param >= 0)
+            throw std::runtime_error("gate list " + path + ": parameter bindings need the batched or adjoint API");
+        switch (g.kind) {
+            case QSE_GATE_IDENTITY: break;
+            case QSE_GATE_X: pauliX(g.target); break;
+            case QSE_GATE_Y: pauliY(g.target); break;
+            case QSE_GATE_Z: pauliZ(g.target); break;
+            case QSE_GATE_H: hadamard(g.target); break;
+            case QSE_GATE_PHASE: phase(g.theta, g.target); break;
+            case QSE_GATE_RX: rotateX(g.theta, g.target); break;
+            case QSE_GATE_RY: rotateY(g.theta, g.target); break;
+            case QSE_GATE_RZ: rotateZ(g.theta, g.target); break;
+            case QSE_GATE_CNOT: cnot(g.control, g.target); break;
+            case QSE_GATE_CPHASE: cphase(g.theta, g.control, g.target); break;
+            case QSE_GATE_MCPHASE: {
+                std::vector<size_t> cs;
+                for (int q = 0; q < 64; ++q)
+                    if ((g.controls >> q) & 1) cs.push_back((size_t)q);
+                mcphase(g.theta, cs, g.target);
+                break;
+            }
+            case QSE_GATE_SWAP: swap(g.target, g.control); break;
+            case QSE_GATE_MATRIX:
+                gate(Gate2x2(Complex(g.m[0], g.m[1]), Complex(g.m[2], g.m[3]), Complex(g.m[4], g.m[5]),
+                             Complex(g.m[6], g.m[7])),
+                     g.target);
+                break;
+            default: throw std::runtime_error("gate list: unsupported gate kind");
+        }
+    }
 }

@@ -2,12 +2,14 @@
 // (QSEPythonModule/QuantumSimEvoPy.cpp:8-41: Circuit with snake_case gate
 // methods, execute/measure/sample/reset/print_state) on the HIP backend, plus
 // state I/O, expectation values and seeded readout.
+#include <cstring>
 #include <pybind11/complex.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
 #include "quantumsimevo/Circuit.h"
+#include "quantumsimevo/GateListFile.h"
 #include "quantumsimevo/CircuitUnitaryOperationHIP.h"
 #include "quantumsimevo/HardwareCheck.h"
 #include "quantumsimevo/Noise.h"
@@ -85,7 +87,14 @@ PYBIND11_MODULE(_quantumsimevo, m) {
                             py::arg("x_mask") = c.noise().bitFlipMask(),
                             py::arg("z_mask") = c.noise().phaseFlipMask());
         })
-        .def("set_fusion", [](Circuit& c, bool on) { c.backend().setFusion(on); }, py::arg("fused"));
+        .def("set_fusion", [](Circuit& c, bool on) { c.backend().setFusion(on); }, py::arg("fused"))
+        // gate-list files (GateListFile.h)
+        .def("save_gates", &Circuit::saveGates, py::arg("path"))
+        .def("load_gates", &Circuit::loadGates, py::arg("path"))
+        .def("gate_list", [](const Circuit& c) {
+            const std::vector<qse_op> ops = c.gateList();
+            return py::bytes(reinterpret_cast<const char*>(ops.data()), ops.size() * sizeof(qse_op));
+        });
 
     m.def("device_report", [](int device) {
               const DeviceReport r = HardwareCheck::report(device);
@@ -96,6 +105,24 @@ PYBIND11_MODULE(_quantumsimevo, m) {
           },
           py::arg("device") = 0);
     m.def("print_report", &HardwareCheck::printReport, py::arg("device") = 0);
+    // Gate-list text (GateListFile.h) from / to raw qse_op records (104 bytes each).
+    m.def("write_gate_list",
+          [](int n, py::bytes records) {
+              const std::string b = records;
+              if (b.size() % sizeof(qse_op)) throw std::invalid_argument("records are not whole qse_op structs");
+              std::vector<qse_op> ops(b.size() / sizeof(qse_op));
+              if (!ops.empty()) std::memcpy(ops.data(), b.data(), b.size());
+              return qse_gates::write(n, ops.data(), ops.size());
+          },
+          py::arg("n"), py::arg("records"));
+    m.def("read_gate_list",
+          [](const std::string& text) {
+              int n = 0;
+              std::vector<qse_op> ops;
+              qse_gates::read(text, &n, ops);
+              return py::make_tuple(n, py::bytes(reinterpret_cast<const char*>(ops.data()), ops.size() * sizeof(qse_op)));
+          },
+          py::arg("text"));
     // Pauli frames of many noise trajectories: per seed, the Noise draw process
     // (Noise.cpp:21-32, two draws per eligible gate in replay order) over the
     // gate list (GateType kinds, targets); rate >= 0 replaces every eligible

@@ -2,6 +2,7 @@
 known answers of QSETests.cpp, the flush/markHostDirty contract, noise draws and
 seeded device readout, each against the CPU oracle."""
 import math
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -144,3 +145,35 @@ def test_complex64_circuit(Circuit):
     build(qc, gl)
     qc.execute()
     assert np.max(np.abs(qc.state() - O.run_ops(O.basis(n), n, gl.ops()))) < 2e-5
+
+
+def test_gate_list_files_through_the_circuit(Circuit, tmp_path):
+    """Circuit.save_gates writes the same text as the Python writer for the
+    same gates, and a Circuit built from the file (load_gates) reaches the
+    same state, checked against the oracle (1e-12)."""
+    n = 12
+    gl = G.random_mixed(n, 300, 11)
+    ops = gl.ops()
+    ops = ops[ops["kind"] != capi.IDENTITY]  # the Circuit API has no identity gate
+    keep = G.GateList.from_ops(n, ops)
+    qc = Circuit(n)
+    for o in ops:
+        if int(o["kind"]) == capi.MATRIX:
+            m = np.array(o["m"]).reshape(2, 2, 2)
+            qc.gate(m[..., 0] + 1j * m[..., 1], int(o["target"]))
+        else:
+            build(qc, G.GateList.from_ops(n, o[None]))
+    path = tmp_path / "mixed.qsegates"
+    qc.save_gates(str(path))
+    assert path.read_text() == keep.to_text()
+    assert np.frombuffer(qc.gate_list(), dtype=capi.OP_DTYPE).tobytes() == ops.tobytes()
+    qc.execute()
+    qd = Circuit(n)
+    qd.load_gates(str(path))
+    qd.execute()
+    want = O.run_ops(O.basis(n), n, ops)
+    assert np.array_equal(qd.state(), qc.state())
+    assert np.max(np.abs(qd.state() - want)) < 1e-12
+    # the committed example file, through the Circuit (parameter bindings refused there)
+    with pytest.raises(RuntimeError, match="parameter bindings"):
+        Circuit(5).load_gates(str(Path(__file__).resolve().parent / "golden" / "gatelist_example.qsegates"))
