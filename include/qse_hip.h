@@ -200,6 +200,13 @@ int qse_pauli_frame(qse_ctx* ctx, qse_state* st, uint64_t xmask, uint64_t zmask)
  * diagonal gate fused into one HBM read+write of the state). params may be NULL
  * when no op has param >= 0. */
 int qse_run(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params);
+/* The gate list applied `reps` times in a row (repeated identical sweeps, e.g.
+ * Trotter steps; no counterpart in the reference, whose Circuit::execute
+ * replays its list once per call, Circuit.cpp:64-120). After two eager runs
+ * the launches of one run (two when a run exchanges the state's buffers) are
+ * captured in a HIP graph and replayed; results equal `reps` calls of qse_run.
+ * The capture records kernels only (no memset nodes). */
+int qse_run_repeat(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int reps);
 /* Same list, one single-gate sweep per op (no fusion): the drop-in per-call
  * path of CircuitUnitaryOperationHIP and the unfused baseline. */
 int qse_run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params);

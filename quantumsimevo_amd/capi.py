@@ -39,7 +39,7 @@ EXPORTS = [
     "qse_state_download_range", "qse_state_set_basis", "qse_state_copy", "qse_state_set_random", "qse_apply_matrix",
     "qse_apply_hadamard", "qse_apply_x", "qse_apply_y", "qse_apply_z", "qse_apply_phase", "qse_apply_rx",
     "qse_apply_ry", "qse_apply_rz", "qse_apply_cnot", "qse_apply_cphase", "qse_apply_mcphase",
-    "qse_apply_swap", "qse_pauli_frame", "qse_run", "qse_run_sweeps", "qse_plan_passes", "qse_plan_info", "qse_qft_probe", "qse_copy_bandwidth", "qse_sample",
+    "qse_apply_swap", "qse_pauli_frame", "qse_run", "qse_run_repeat", "qse_run_sweeps", "qse_plan_passes", "qse_plan_info", "qse_qft_probe", "qse_copy_bandwidth", "qse_sample",
     "qse_measure", "qse_norm2", "qse_prob_one", "qse_collapse_qubit", "qse_expect_zstrings",
     "qse_adjoint_expect", "qse_batch_expect", "qse_batch_run_small", "qse_sample_frames",
 ]
@@ -108,6 +108,7 @@ def _sigs(lib):
         "qse_pauli_frame": ([_vp, _vp, _u64, _u64], _i),
         "qse_run": ([_vp, _vp, _vp, _i, _vp], _i),
         "qse_run_sweeps": ([_vp, _vp, _vp, _i, _vp], _i),
+        "qse_run_repeat": ([_vp, _vp, _vp, _i, _vp, _i], _i),
         "qse_plan_passes": ([_vp, _i, _i, _vp, _i, C.POINTER(_i)], _i),
         "qse_plan_info": ([_vp, _i, _i, _vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)], _i),
         "qse_qft_probe": ([_vp, _i, C.c_uint64, _vp, _i, _vp], _i),
@@ -397,6 +398,13 @@ class State:
         f = self.lib.qse_run if fused else self.lib.qse_run_sweeps
         check(f(self.ctx.h, self.h, _ptr(a), len(a), None if p is None else _ptr(p)),
               "qse_run" if fused else "qse_run_sweeps")
+
+    def run_repeat(self, ops, reps: int, params=None):
+        """The gate list applied reps times (replayed from a HIP graph after two eager runs)."""
+        a = as_ops(ops)
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        check(self.lib.qse_run_repeat(self.ctx.h, self.h, _ptr(a), len(a), None if p is None else _ptr(p), int(reps)),
+              "qse_run_repeat")
 
     # readout
     def sample(self, u) -> np.ndarray:

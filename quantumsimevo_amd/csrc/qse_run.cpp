@@ -1,6 +1,7 @@
 // qse_run.cpp — gate-list execution entry points: resolves each qse_op into
 // the single-gate sweeps (the replay loop of Circuit::execute,
 // Circuit.cpp:73-112), or hands the list to the fusion planner.
+#include <algorithm>
 #include <cmath>
 #include <string>
 
@@ -62,6 +63,60 @@ int qse_run(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const doub
     if (rc) return rc;
     if (nops == 0) return QSE_OK;
     return run_fused(ctx, st, ops, nops, params, nullptr, false, true);
+}
+
+int qse_run_repeat(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int reps) {
+    if (!ctx || !st || !st->d) return set_error(QSE_ERR_INVALID_ARG, "null context or state");
+    if (nops < 0 || (nops > 0 && !ops) || reps < 0) return set_error(QSE_ERR_INVALID_ARG, "bad op list or count");
+    int rc = validate_ops(st->n, ops, nops, params);
+    if (rc) return rc;
+    if (nops == 0 || reps == 0) return QSE_OK;
+    // Eager runs first: they cache the plan, place the program on the device,
+    // size the scratch buffers and (under QSE_JIT_AUTO) compile the passes, so
+    // the captured run makes no allocation, upload, compile or host wait.
+    int done = 0;
+    for (; done < std::min(reps, 2); ++done) {
+        rc = run_fused(ctx, st, ops, nops, params, nullptr, false, true);
+        if (rc) return rc;
+    }
+    if (done == reps) return QSE_OK;
+    // Capture one run, or two when a run exchanges the state's buffers (an
+    // out-of-place last pass), so that the replay leaves the pointers as the
+    // capture found them.
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    void* const d0 = st->d;
+    hipGraph_t graph = nullptr;
+    QSE_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int per = 1;
+    rc = run_fused(ctx, st, ops, nops, params, nullptr, false, true);
+    if (rc == QSE_OK && st->d != d0) {
+        per = 2;
+        rc = run_fused(ctx, st, ops, nops, params, nullptr, false, true);
+    }
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+    ctx->timing = timing;
+    if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    if (ec != hipSuccess) return hip_error(ec, "hipStreamEndCapture");
+    if (st->d != d0) {
+        hipGraphDestroy(graph);
+        return set_error(QSE_ERR_UNSUPPORTED, "run_repeat: the captured runs did not restore the state buffers");
+    }
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_error(e, "hipGraphInstantiate");
+    for (; done + per <= reps && e == hipSuccess; done += per) e = hipGraphLaunch(exec, ctx->stream);
+    hipGraphExecDestroy(exec);
+    if (e != hipSuccess) return hip_error(e, "hipGraphLaunch");
+    for (; done < reps; ++done) {
+        rc = run_fused(ctx, st, ops, nops, params, nullptr, false, true);
+        if (rc) return rc;
+    }
+    return QSE_OK;
 }
 
 int qse_run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params) {
