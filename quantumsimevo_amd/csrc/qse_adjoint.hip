@@ -38,8 +38,9 @@ namespace qse {
 int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int* passes_out,
               bool dry_run, bool allow_jit);
 int run_plan(qse_ctx* ctx, qse_state* st, const HostPlan& plan, int k, int jit_kind);
-int plan_in_place(qse_ctx* ctx, int n, int prec, const qse_op* ops, int nops, const double* params, HostPlan& P,
-                  int* k_out);
+int choose_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P, bool allow_oop);
+int tuned_plan(const qse_ctx* ctx, int n, int prec, const qse_op* ops, int nops, const double* params, HostPlan& P,
+               bool allow_oop, int* k_out, int* L_out);
 int validate_ops(int n, const qse_op* ops, int nops, const double* params);
 int reduce_zstr_async(qse_ctx* ctx, const qse_state* st, const uint64_t* d_m, const double* d_w, int nterms,
                       double* d_out);
@@ -312,10 +313,18 @@ int grad_diag_async(qse_ctx* ctx, qse_state* psi, qse_state* lam, const std::vec
     return QSE_OK;
 }
 
+// Tile shapes the shape tuner chose for the programs of the first parameter
+// set of a call, reused in the same order for the other sets (their programs
+// differ only in angles), so only the first set pays for tuning.
+struct ShapeMemo {
+    std::vector<std::pair<int, int>> kl;
+    size_t next = 0;
+};
+
 // Run ops on each of the given states (same qubit count and precision) with
-// one in-place plan.
+// one in-place plan of the tuned shape.
 int run_on(qse_ctx* ctx, std::initializer_list<qse_state*> states, const std::vector<qse_op>& ops,
-           const double* params) {
+           const double* params, ShapeMemo* memo) {
     if (ops.empty()) return QSE_OK;
     qse_state* s0 = *states.begin();
     int rc;
@@ -327,8 +336,16 @@ int run_on(qse_ctx* ctx, std::initializer_list<qse_state*> states, const std::ve
         return QSE_OK;
     }
     HostPlan P;
-    int k = 0;
-    rc = plan_in_place(ctx, s0->n, s0->prec, ops.data(), (int)ops.size(), params, P, &k);
+    int k = 0, L = 0;
+    if (memo && memo->next < memo->kl.size()) {
+        k = memo->kl[memo->next].first;
+        L = memo->kl[memo->next].second;
+        rc = choose_plan(s0->n, k, L, ops.data(), (int)ops.size(), params, P, false);
+    } else {
+        rc = tuned_plan(ctx, s0->n, s0->prec, ops.data(), (int)ops.size(), params, P, false, &k, &L);
+        if (memo && rc == QSE_OK) memo->kl.push_back({k, L});
+    }
+    if (memo) ++memo->next;
     if (rc) return rc;
     for (qse_state* st : states) {
         rc = run_plan(ctx, st, P, k, JIT_BY_STRUCTURE);
@@ -351,18 +368,19 @@ template <class R>
 int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double* params,
                 const std::vector<AdjSegment>& segs, const std::vector<qse_op>& prefix, const uint64_t* hm,
                 const uint64_t* dm, const double* dw, int nterms, int ckpt_every, double* d_res, int* next,
-                SetSlots& slots) {
+                SetSlots& slots, ShapeMemo& memo) {
+    memo.next = 0;
     int rc = qse_state_set_basis(ctx, ws.psi, 0);
     if (rc) return rc;
     const int S = (int)segs.size();
     // forward: whole list fused, or segment by segment storing checkpoints
     std::map<int, int> ckpt_at;  // segment -> checkpoint slot holding psi at its evaluation point
     if (ckpt_every > 0 && S > 1) {
-        rc = run_on(ctx, {ws.psi}, prefix, params);
+        rc = run_on(ctx, {ws.psi}, prefix, params, &memo);
         if (rc) return rc;
         int slot = 0;
         for (int g = S - 2; g >= 0; --g) {  // forward order; the last segment's gates are in the prefix
-            rc = run_on(ctx, {ws.psi}, segs[g].forward, params);
+            rc = run_on(ctx, {ws.psi}, segs[g].forward, params, &memo);
             if (rc) return rc;
             if (g > 0 && ((S - 1 - g) % ckpt_every) == 0 && slot < (int)ws.ckpt.size()) {
                 rc = qse_state_copy(ctx, ws.ckpt[slot], ws.psi);
@@ -371,7 +389,7 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
             }
         }
     } else {
-        rc = run_on(ctx, {ws.psi}, std::vector<qse_op>(ops, ops + nops), params);
+        rc = run_on(ctx, {ws.psi}, std::vector<qse_op>(ops, ops + nops), params, &memo);
         if (rc) return rc;
     }
     slots.energy = (*next)++;
@@ -384,7 +402,7 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
     if (rc) return rc;
     // rotate into the first evaluation point's frame (X/Y generators become Z-strings)
     if (S > 0) {
-        rc = run_on(ctx, {ws.psi, ws.lam}, segs[0].pre, nullptr);
+        rc = run_on(ctx, {ws.psi, ws.lam}, segs[0].pre, nullptr, &memo);
         if (rc) return rc;
     }
     for (int g = 0; g < S; ++g) {
@@ -393,7 +411,7 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
         if (ck != ckpt_at.end()) {
             rc = qse_state_copy(ctx, ws.psi, ws.ckpt[ck->second]);
             if (rc) return rc;
-            rc = run_on(ctx, {ws.psi}, seg.pre, nullptr);
+            rc = run_on(ctx, {ws.psi}, seg.pre, nullptr, &memo);
             if (rc) return rc;
         }
         // generator terms at this point: one pass for up to 16 Z-strings
@@ -413,8 +431,8 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
             *next += (int)masks.size();
         }
         if (g + 1 < S && !seg.unapply.empty()) {
-            if (ckpt_at.find(g + 1) == ckpt_at.end()) rc = run_on(ctx, {ws.psi, ws.lam}, seg.unapply, nullptr);
-            else rc = run_on(ctx, {ws.lam}, seg.unapply, nullptr);
+            if (ckpt_at.find(g + 1) == ckpt_at.end()) rc = run_on(ctx, {ws.psi, ws.lam}, seg.unapply, nullptr, &memo);
+            else rc = run_on(ctx, {ws.lam}, seg.unapply, nullptr, &memo);
             if (rc) return rc;
         }
     }
@@ -495,6 +513,7 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
     double* d_res = (double*)rbuf;
     std::vector<SetSlots> slots(batch);
     int next = 0;
+    ShapeMemo memo;
     for (int b = 0; b < batch; ++b) {
         const double* pb = params ? params + (size_t)b * nparams : nullptr;
         std::vector<AdjSegment> segs;
@@ -504,9 +523,9 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
             return set_error(QSE_ERR_UNSUPPORTED, "adjoint: schedule changed between parameter sets");
         rc = (precision == QSE_C128)
                  ? adjoint_one<double>(ctx, ws, ops, nops, pb, segs, prefix, obs_masks, dm, dw, nterms, ckpt_every, d_res, &next,
-                                       slots[b])
+                                       slots[b], memo)
                  : adjoint_one<float>(ctx, ws, ops, nops, pb, segs, prefix, obs_masks, dm, dw, nterms, ckpt_every, d_res, &next,
-                                      slots[b]);
+                                      slots[b], memo);
         if (rc) return rc;
     }
     void* pin;
@@ -548,11 +567,13 @@ int qse_batch_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops
         QSE_HIP(hipMemcpyAsync(dw, obs_weights, 8 * nterms, hipMemcpyHostToDevice, ctx->stream));
     }
     const std::vector<qse_op> list(ops, ops + nops);
+    ShapeMemo memo;
     for (int b = 0; b < batch; ++b) {
         const double* pb = params ? params + (size_t)b * nparams : nullptr;
+        memo.next = 0;
         rc = qse_state_set_basis(ctx, ws.psi, 0);
         if (rc) return rc;
-        rc = run_on(ctx, {ws.psi}, list, pb);
+        rc = run_on(ctx, {ws.psi}, list, pb, &memo);
         if (rc) return rc;
         rc = (precision == QSE_C128) ? energy_async<double>(ctx, ws.psi, obs_masks, dm, dw, nterms, d_res + b)
                                      : energy_async<float>(ctx, ws.psi, obs_masks, dm, dw, nterms, d_res + b);

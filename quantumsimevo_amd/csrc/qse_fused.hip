@@ -459,16 +459,6 @@ int run_plan(qse_ctx* ctx, qse_state* st, const HostPlan& plan, int k, int jit_k
     return QSE_OK;
 }
 
-// In-place plan of ops for an n-qubit state with the context's default shape
-// (the adjoint's internal programs: planned once, run on psi and lambda).
-int plan_in_place(qse_ctx* ctx, int n, int prec, const qse_op* ops, int nops, const double* params, HostPlan& P,
-                  int* k_out) {
-    int k, L;
-    fusion_shape(ctx, n, prec, &k, &L);
-    *k_out = k;
-    return choose_plan(n, k, L, ops, nops, params, P, false);
-}
-
 int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int* passes_out,
               bool dry_run, bool allow_jit) {
     if (st->n < FUSE_MIN_QUBITS) {
