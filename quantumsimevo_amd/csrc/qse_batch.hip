@@ -148,12 +148,25 @@ __device__ void batch_pass(cx<R>* s, int n, const FPass& P, const BArgs& A, cx<R
                             case 2: rxlike_p<R, 2>(a, op); break;
                             default: rxlike_p<R, 3>(a, op); break;
                         }
-                    } else if (op.type == FOP_XSWAP && op.rctl == 0 && op.tctl == 0 && op.ectl == 0) {
-                        switch (op.p) {
-                            case 0: xswap_p<R, 0>(a); break;
-                            case 1: xswap_p<R, 1>(a); break;
-                            case 2: xswap_p<R, 2>(a); break;
-                            default: xswap_p<R, 3>(a); break;
+                    } else if (op.type == FOP_XSWAP) {
+                        // X / CNOT target: an exchange of registers (controls: register bits per
+                        // pair, thread bits and external qubits for the whole thread)
+                        if (((toff & op.tctl) == op.tctl) && ((base & op.ectl) == op.ectl)) {
+                            if (op.rctl == 0) {
+                                switch (op.p) {
+                                    case 0: xswap_p<R, 0>(a); break;
+                                    case 1: xswap_p<R, 1>(a); break;
+                                    case 2: xswap_p<R, 2>(a); break;
+                                    default: xswap_p<R, 3>(a); break;
+                                }
+                            } else {
+                                switch (op.p) {
+                                    case 0: xswap_ctl_p<R, 0>(a, op.rctl); break;
+                                    case 1: xswap_ctl_p<R, 1>(a, op.rctl); break;
+                                    case 2: xswap_ctl_p<R, 2>(a, op.rctl); break;
+                                    default: xswap_ctl_p<R, 3>(a, op.rctl); break;
+                                }
+                            }
                         }
                     } else if (((toff & op.tctl) == op.tctl) && ((base & op.ectl) == op.ectl)) {
                         switch (op.p) {

@@ -54,6 +54,23 @@ __device__ __forceinline__ void xswap_p(cx<R>* a) {
     }
 }
 
+// X on register position P where the register controls hold: a wave-uniform
+// branch per register pair (the pair's index is known at compile time, the
+// control mask at run time), then a register exchange; no arithmetic.
+template <class R, int P>
+__device__ __forceinline__ void xswap_ctl_p(cx<R>* a, uint32_t rctl) {
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+        const int j0 = ((jj >> P) << (P + 1)) | (jj & ((1 << P) - 1));
+        const int j1 = j0 | (1 << P);
+        if (((uint32_t)j0 & rctl) == rctl) {
+            const cx<R> v0 = a[j0];
+            a[j0] = a[j1];
+            a[j1] = v0;
+        }
+    }
+}
+
 // One 2x2 op on register position P over the 8 register pairs (j0, j0|2^P).
 // Pairs whose register controls fail get the identity (a wave-uniform select
 // of coefficients, not a branch). H and X use the same formula: X = [0,1;1,0]
