@@ -313,18 +313,50 @@ int grad_diag_async(qse_ctx* ctx, qse_state* psi, qse_state* lam, const std::vec
     return QSE_OK;
 }
 
-// Tile shapes the shape tuner chose for the programs of the first parameter
-// set of a call, reused in the same order for the other sets (their programs
-// differ only in angles), so only the first set pays for tuning.
-struct ShapeMemo {
-    std::vector<std::pair<int, int>> kl;
-    size_t next = 0;
-};
+// Plan ops in place with the tile shape tuned for their structure (tuned on
+// first sight, then cached in the context).
+int plan_tuned_shape(qse_ctx* ctx, int n, int prec, const std::vector<qse_op>& ops, const double* params,
+                     HostPlan& P, int* k_out) {
+    const int head[2] = {n, prec};
+    std::vector<char> key(sizeof(head) + sizeof(qse_op) * ops.size());
+    std::memcpy(key.data(), head, sizeof(head));
+    qse_op* kops = (qse_op*)(key.data() + sizeof(head));
+    for (size_t i = 0; i < ops.size(); ++i) {
+        qse_op o = ops[i];
+        o.theta = 0.0;
+        o.scale = 0.0;
+        std::fill(o.m, o.m + 8, 0.0);
+        std::memcpy(&kops[i], &o, sizeof(qse_op));
+    }
+    for (ShapeEntry& e : ctx->shape_cache)
+        if (e.key == key) {
+            e.stamp = ++ctx->prog_clock;
+            *k_out = e.k;
+            return choose_plan(n, e.k, e.L, ops.data(), (int)ops.size(), params, P, false);
+        }
+    int k = 0, L = 0;
+    const int rc = tuned_plan(ctx, n, prec, ops.data(), (int)ops.size(), params, P, false, &k, &L);
+    if (rc) return rc;
+    if (ctx->shape_cache.size() >= 256) {
+        size_t oldest = 0;
+        for (size_t i = 1; i < ctx->shape_cache.size(); ++i)
+            if (ctx->shape_cache[i].stamp < ctx->shape_cache[oldest].stamp) oldest = i;
+        ctx->shape_cache.erase(ctx->shape_cache.begin() + oldest);
+    }
+    ShapeEntry e;
+    e.key = std::move(key);
+    e.k = k;
+    e.L = L;
+    e.stamp = ++ctx->prog_clock;
+    ctx->shape_cache.push_back(std::move(e));
+    *k_out = k;
+    return QSE_OK;
+}
 
 // Run ops on each of the given states (same qubit count and precision) with
 // one in-place plan of the tuned shape.
 int run_on(qse_ctx* ctx, std::initializer_list<qse_state*> states, const std::vector<qse_op>& ops,
-           const double* params, ShapeMemo* memo) {
+           const double* params) {
     if (ops.empty()) return QSE_OK;
     qse_state* s0 = *states.begin();
     int rc;
@@ -336,16 +368,8 @@ int run_on(qse_ctx* ctx, std::initializer_list<qse_state*> states, const std::ve
         return QSE_OK;
     }
     HostPlan P;
-    int k = 0, L = 0;
-    if (memo && memo->next < memo->kl.size()) {
-        k = memo->kl[memo->next].first;
-        L = memo->kl[memo->next].second;
-        rc = choose_plan(s0->n, k, L, ops.data(), (int)ops.size(), params, P, false);
-    } else {
-        rc = tuned_plan(ctx, s0->n, s0->prec, ops.data(), (int)ops.size(), params, P, false, &k, &L);
-        if (memo && rc == QSE_OK) memo->kl.push_back({k, L});
-    }
-    if (memo) ++memo->next;
+    int k = 0;
+    rc = plan_tuned_shape(ctx, s0->n, s0->prec, ops, params, P, &k);
     if (rc) return rc;
     for (qse_state* st : states) {
         rc = run_plan(ctx, st, P, k, JIT_BY_STRUCTURE);
@@ -368,19 +392,18 @@ template <class R>
 int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double* params,
                 const std::vector<AdjSegment>& segs, const std::vector<qse_op>& prefix, const uint64_t* hm,
                 const uint64_t* dm, const double* dw, int nterms, int ckpt_every, double* d_res, int* next,
-                SetSlots& slots, ShapeMemo& memo) {
-    memo.next = 0;
+                SetSlots& slots) {
     int rc = qse_state_set_basis(ctx, ws.psi, 0);
     if (rc) return rc;
     const int S = (int)segs.size();
     // forward: whole list fused, or segment by segment storing checkpoints
     std::map<int, int> ckpt_at;  // segment -> checkpoint slot holding psi at its evaluation point
     if (ckpt_every > 0 && S > 1) {
-        rc = run_on(ctx, {ws.psi}, prefix, params, &memo);
+        rc = run_on(ctx, {ws.psi}, prefix, params);
         if (rc) return rc;
         int slot = 0;
         for (int g = S - 2; g >= 0; --g) {  // forward order; the last segment's gates are in the prefix
-            rc = run_on(ctx, {ws.psi}, segs[g].forward, params, &memo);
+            rc = run_on(ctx, {ws.psi}, segs[g].forward, params);
             if (rc) return rc;
             if (g > 0 && ((S - 1 - g) % ckpt_every) == 0 && slot < (int)ws.ckpt.size()) {
                 rc = qse_state_copy(ctx, ws.ckpt[slot], ws.psi);
@@ -389,7 +412,7 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
             }
         }
     } else {
-        rc = run_on(ctx, {ws.psi}, std::vector<qse_op>(ops, ops + nops), params, &memo);
+        rc = run_on(ctx, {ws.psi}, std::vector<qse_op>(ops, ops + nops), params);
         if (rc) return rc;
     }
     slots.energy = (*next)++;
@@ -402,7 +425,7 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
     if (rc) return rc;
     // rotate into the first evaluation point's frame (X/Y generators become Z-strings)
     if (S > 0) {
-        rc = run_on(ctx, {ws.psi, ws.lam}, segs[0].pre, nullptr, &memo);
+        rc = run_on(ctx, {ws.psi, ws.lam}, segs[0].pre, nullptr);
         if (rc) return rc;
     }
     for (int g = 0; g < S; ++g) {
@@ -411,7 +434,7 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
         if (ck != ckpt_at.end()) {
             rc = qse_state_copy(ctx, ws.psi, ws.ckpt[ck->second]);
             if (rc) return rc;
-            rc = run_on(ctx, {ws.psi}, seg.pre, nullptr, &memo);
+            rc = run_on(ctx, {ws.psi}, seg.pre, nullptr);
             if (rc) return rc;
         }
         // generator terms at this point: one pass for up to 16 Z-strings
@@ -431,8 +454,8 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
             *next += (int)masks.size();
         }
         if (g + 1 < S && !seg.unapply.empty()) {
-            if (ckpt_at.find(g + 1) == ckpt_at.end()) rc = run_on(ctx, {ws.psi, ws.lam}, seg.unapply, nullptr, &memo);
-            else rc = run_on(ctx, {ws.lam}, seg.unapply, nullptr, &memo);
+            if (ckpt_at.find(g + 1) == ckpt_at.end()) rc = run_on(ctx, {ws.psi, ws.lam}, seg.unapply, nullptr);
+            else rc = run_on(ctx, {ws.lam}, seg.unapply, nullptr);
             if (rc) return rc;
         }
     }
@@ -513,7 +536,6 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
     double* d_res = (double*)rbuf;
     std::vector<SetSlots> slots(batch);
     int next = 0;
-    ShapeMemo memo;
     for (int b = 0; b < batch; ++b) {
         const double* pb = params ? params + (size_t)b * nparams : nullptr;
         std::vector<AdjSegment> segs;
@@ -523,9 +545,9 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
             return set_error(QSE_ERR_UNSUPPORTED, "adjoint: schedule changed between parameter sets");
         rc = (precision == QSE_C128)
                  ? adjoint_one<double>(ctx, ws, ops, nops, pb, segs, prefix, obs_masks, dm, dw, nterms, ckpt_every, d_res, &next,
-                                       slots[b], memo)
+                                       slots[b])
                  : adjoint_one<float>(ctx, ws, ops, nops, pb, segs, prefix, obs_masks, dm, dw, nterms, ckpt_every, d_res, &next,
-                                      slots[b], memo);
+                                      slots[b]);
         if (rc) return rc;
     }
     void* pin;
@@ -567,13 +589,11 @@ int qse_batch_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops
         QSE_HIP(hipMemcpyAsync(dw, obs_weights, 8 * nterms, hipMemcpyHostToDevice, ctx->stream));
     }
     const std::vector<qse_op> list(ops, ops + nops);
-    ShapeMemo memo;
     for (int b = 0; b < batch; ++b) {
         const double* pb = params ? params + (size_t)b * nparams : nullptr;
-        memo.next = 0;
         rc = qse_state_set_basis(ctx, ws.psi, 0);
         if (rc) return rc;
-        rc = run_on(ctx, {ws.psi}, list, pb, &memo);
+        rc = run_on(ctx, {ws.psi}, list, pb);
         if (rc) return rc;
         rc = (precision == QSE_C128) ? energy_async<double>(ctx, ws.psi, obs_masks, dm, dw, nterms, d_res + b)
                                      : energy_async<float>(ctx, ws.psi, obs_masks, dm, dw, nterms, d_res + b);

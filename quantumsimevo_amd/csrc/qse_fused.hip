@@ -423,7 +423,14 @@ int run_plan(qse_ctx* ctx, qse_state* st, const HostPlan& plan, int k, int jit_k
     if (jit_kind == JIT_BY_PROGRAM) {
         bool jit = false;
         rc = jit_for(ctx, *entry, plan, st->prec, &jit);
-        if (jit) fns = &entry->fns;
+        if (jit) {
+            fns = &entry->fns;
+        } else if (rc == QSE_OK && ctx->jit_mode == QSE_JIT_AUTO && entry->runs == 1) {
+            // a program seen for the first time: when its structure keeps
+            // coming back with other angles (a parameter loop), the
+            // structure-keyed kernels serve it
+            rc = jit_for_structure(ctx, plan, S, st->prec, &fns);
+        }
     } else if (jit_kind == JIT_BY_STRUCTURE) {
         rc = jit_for_structure(ctx, plan, S, st->prec, &fns);
     }

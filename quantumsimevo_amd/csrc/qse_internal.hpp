@@ -82,6 +82,16 @@ struct StructEntry {
     std::shared_ptr<JitJob> job;
 };
 
+// Tile shape the shape tuner chose for a gate list, keyed by its structure
+// (qubit count, precision, gate kinds, qubits and parameter bindings, with
+// angles and matrices zeroed): runs of the same circuit at other angles reuse
+// it instead of tuning again.
+struct ShapeEntry {
+    std::vector<char> key;
+    int k = 0, L = 0;
+    uint64_t stamp = 0;
+};
+
 // A tuned plan for one exact input (qubit count, precision, gate list,
 // parameter values, shape settings), so repeated runs skip planning.
 struct PlanEntry {
@@ -106,6 +116,9 @@ struct qse_ctx {
     uint64_t prog_clock = 0;
     // Kernels by plan structure (the adjoint's and batched expectations' runs).
     std::vector<qse::StructEntry> struct_cache;
+    // Tuned tile shapes by gate-list structure (internal runs of the adjoint
+    // and batched expectations).
+    std::vector<qse::ShapeEntry> shape_cache;
     // Tuned plans of recent qse_run inputs.
     std::vector<qse::PlanEntry> plan_cache;
     // Pinned host staging for small readbacks.

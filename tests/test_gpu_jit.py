@@ -117,3 +117,29 @@ def test_same_program_both_precisions(ctx):
             st.run(ops)
             assert np.max(np.abs(st.download() - want)) < tol, prec
             st.close()
+
+
+def test_parameter_loop_uses_structure_keyed_kernels():
+    """qse_run with other angles on every call (an optimiser loop): no program
+    repeats, so the program-keyed kernels never compile; the structure seen
+    twice is compiled on a worker thread and serves the later calls."""
+    n = 13
+    edges = G.random_regular_graph(n - 1, 3, seed=6)
+    ops = G.qaoa_maxcut(n, edges, 2).ops()
+    rng = np.random.default_rng(9)
+    c = capi.Context(0)
+    try:
+        st = c.state(n, capi.C128)
+        for it in range(8):
+            x = rng.random(4) * math.pi
+            st.set_basis(0)
+            st.run(ops, x)
+            want = O.run_ops(O.basis(n), n, ops, x)
+            assert np.max(np.abs(st.download() - want)) < 1e-12, it
+            if it == 1:
+                c.jit_wait()
+                before = c.jit_stats()["launches"]
+        assert c.jit_stats()["launches"] > before
+        st.close()
+    finally:
+        c.close()
