@@ -195,6 +195,8 @@ def main() -> int:
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--precision", choices=["c128", "c64"], default="c128")
     ap.add_argument("--unfused", action="store_true", help="one sweep per gate instead of fused passes")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the tuner's best 3 tile shapes on the state first and keep the fastest")
     ap.add_argument("--jit", choices=["auto", "off", "always"], default="auto",
                     help="run-time specialised passes: compiled on the program's second run (auto), never, or at once")
     ap.add_argument("--no-cpu", action="store_true")
@@ -241,6 +243,10 @@ def main() -> int:
     copy_runs = ctx.copy_bandwidth(log2_el, run_el) if run_el < 12 else copy_contig
     log(f"copy ceiling: contiguous {copy_contig:.0f} GB/s, runs of 2^{run_el} x 16 B {copy_runs:.0f} GB/s")
     st = ctx.state(n, prec)
+    tune = None
+    if args.autotune and not args.unfused:
+        tune = ctx.autotune(st, ops, candidates=3)
+        log(f"autotune: {tune}")
     st.set_random(12345)
     plan = ctx.plan_info(n, prec, ops) if not args.unfused else {"passes": len(ops), "tile_bits": 0, "low_bits": 0,
                                                                   "stages": 0}
@@ -309,7 +315,8 @@ def main() -> int:
         "config": {"workload": f"QFT-{n} {args.precision} (QFT.cpp:9-22), {len(ops)} gates, state resident in HBM",
                    "qubits": n, "precision": args.precision, "fused": fused, "passes_per_step": passes,
                    "tile_qubits": plan["tile_bits"], "contiguous_low_qubits": plan["low_bits"],
-                   "jit": args.jit, "jit_timed": jit_timed, "jit_compile_ms": jit1["compile_ms"]},
+                   "jit": args.jit, "jit_timed": jit_timed, "jit_compile_ms": jit1["compile_ms"],
+                   "autotune": tune},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": bytes_per_launch, "mean_launch_ms": mean_ms,

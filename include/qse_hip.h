@@ -124,6 +124,16 @@ int qse_ctx_sync(qse_ctx* ctx);
  * counterpart in the reference (one kernel launch per gate). qse_ctx_get_fusion
  * returns the settings (0 = tuned). */
 int qse_ctx_set_fusion(qse_ctx* ctx, int precision, int tile_bits, int low_bits);
+/* Measured tile shape (no counterpart in the reference): the shape tuner's
+ * best `max_candidates` shapes for this gate list are each compiled and timed
+ * on st (two runs, then three timed; st's amplitudes are overwritten); the
+ * fastest is kept for the gate list's structure (kinds, qubits, parameter
+ * bindings) and used by later qse_run calls of it while the context's tile
+ * size is unset. Returns the shape, its time per run in ms, and per candidate
+ * tried_k/L/ms (arrays of max_candidates entries, may be NULL) with the count
+ * in *ntried. Below 10 qubits nothing is tuned (k = L = 0). */
+int qse_autotune(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int max_candidates,
+                 int* k, int* L, double* ms, int* tried_k, int* tried_L, double* tried_ms, int* ntried);
 int qse_ctx_get_fusion(qse_ctx* ctx, int precision, int* tile_bits, int* low_bits);
 /* Run-time specialisation of fused passes (no counterpart in the reference,
  * whose GPU backend launches one fixed kernel per gate,

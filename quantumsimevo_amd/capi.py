@@ -34,7 +34,7 @@ JIT_OFF, JIT_AUTO, JIT_ALWAYS = range(3)
 EXPORTS = [
     "qse_abi_version", "qse_status_string", "qse_last_error", "qse_device_count", "qse_device_mem_info",
     "qse_device_info", "qse_ctx_create", "qse_ctx_destroy", "qse_ctx_sync", "qse_ctx_set_fusion",
-    "qse_ctx_get_fusion", "qse_ctx_set_jit", "qse_ctx_jit_stats", "qse_ctx_jit_wait", "qse_ctx_enable_timing", "qse_ctx_timing", "qse_ctx_reset_timing",
+    "qse_ctx_get_fusion", "qse_autotune", "qse_ctx_set_jit", "qse_ctx_jit_stats", "qse_ctx_jit_wait", "qse_ctx_enable_timing", "qse_ctx_timing", "qse_ctx_reset_timing",
     "qse_state_create", "qse_state_destroy", "qse_state_info", "qse_state_upload", "qse_state_download",
     "qse_state_download_range", "qse_state_set_basis", "qse_state_copy", "qse_state_set_random", "qse_apply_matrix",
     "qse_apply_hadamard", "qse_apply_x", "qse_apply_y", "qse_apply_z", "qse_apply_phase", "qse_apply_rx",
@@ -80,6 +80,8 @@ def _sigs(lib):
         "qse_ctx_set_jit": ([_vp, _i], _i),
         "qse_ctx_jit_stats": ([_vp, C.POINTER(_i64), C.POINTER(_i64), _pd], _i),
         "qse_ctx_jit_wait": ([_vp], _i),
+        "qse_autotune": ([_vp, _vp, _vp, _i, _vp, _i, C.POINTER(_i), C.POINTER(_i), _pd, _vp, _vp, _vp,
+                          C.POINTER(_i)], _i),
         "qse_ctx_enable_timing": ([_vp, _i], _i),
         "qse_ctx_timing": ([_vp, _i, C.POINTER(_i64), _pd, _pd], _i),
         "qse_ctx_reset_timing": ([_vp], _i),
@@ -215,6 +217,21 @@ class Context:
         c, l, ms = C.c_int64(0), C.c_int64(0), C.c_double(0)
         check(self.lib.qse_ctx_jit_stats(self.h, C.byref(c), C.byref(l), C.byref(ms)), "qse_ctx_jit_stats")
         return {"compiled": c.value, "launches": l.value, "compile_ms": ms.value}
+
+    def autotune(self, state: "State", ops, params=None, candidates: int = 3) -> dict:
+        """Time the tuner's best `candidates` tile shapes on `state` (overwritten) and keep the fastest
+        for this gate list's structure; returns {"k", "L", "ms", "tried": [(k, L, ms), ...]}."""
+        a = as_ops(ops)
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        k, L, ms, nt = C.c_int(0), C.c_int(0), C.c_double(0), C.c_int(0)
+        tk = np.zeros(candidates, np.int32)
+        tl = np.zeros(candidates, np.int32)
+        tm = np.zeros(candidates, np.float64)
+        check(self.lib.qse_autotune(self.h, state.h, _ptr(a), len(a), None if p is None else _ptr(p), candidates,
+                                    C.byref(k), C.byref(L), C.byref(ms), _ptr(tk), _ptr(tl), _ptr(tm), C.byref(nt)),
+              "qse_autotune")
+        return {"k": k.value, "L": L.value, "ms": ms.value,
+                "tried": [(int(tk[i]), int(tl[i]), float(tm[i])) for i in range(nt.value)]}
 
     def jit_wait(self):
         """Wait for the structure compilations running on worker threads."""

@@ -317,17 +317,7 @@ int grad_diag_async(qse_ctx* ctx, qse_state* psi, qse_state* lam, const std::vec
 // first sight, then cached in the context).
 int plan_tuned_shape(qse_ctx* ctx, int n, int prec, const std::vector<qse_op>& ops, const double* params,
                      HostPlan& P, int* k_out) {
-    const int head[2] = {n, prec};
-    std::vector<char> key(sizeof(head) + sizeof(qse_op) * ops.size());
-    std::memcpy(key.data(), head, sizeof(head));
-    qse_op* kops = (qse_op*)(key.data() + sizeof(head));
-    for (size_t i = 0; i < ops.size(); ++i) {
-        qse_op o = ops[i];
-        o.theta = 0.0;
-        o.scale = 0.0;
-        std::fill(o.m, o.m + 8, 0.0);
-        std::memcpy(&kops[i], &o, sizeof(qse_op));
-    }
+    std::vector<char> key = gate_structure_key(n, prec, ops.data(), (int)ops.size());
     for (ShapeEntry& e : ctx->shape_cache)
         if (e.key == key) {
             e.stamp = ++ctx->prog_clock;
@@ -337,18 +327,7 @@ int plan_tuned_shape(qse_ctx* ctx, int n, int prec, const std::vector<qse_op>& o
     int k = 0, L = 0;
     const int rc = tuned_plan(ctx, n, prec, ops.data(), (int)ops.size(), params, P, false, &k, &L);
     if (rc) return rc;
-    if (ctx->shape_cache.size() >= 256) {
-        size_t oldest = 0;
-        for (size_t i = 1; i < ctx->shape_cache.size(); ++i)
-            if (ctx->shape_cache[i].stamp < ctx->shape_cache[oldest].stamp) oldest = i;
-        ctx->shape_cache.erase(ctx->shape_cache.begin() + oldest);
-    }
-    ShapeEntry e;
-    e.key = std::move(key);
-    e.k = k;
-    e.L = L;
-    e.stamp = ++ctx->prog_clock;
-    ctx->shape_cache.push_back(std::move(e));
+    remember_shape(ctx, std::move(key), k, L);
     *k_out = k;
     return QSE_OK;
 }

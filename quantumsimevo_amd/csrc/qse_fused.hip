@@ -36,6 +36,7 @@ int choose_plan(int n, int k, int L, const qse_op* ops, int nops, const double* 
 int tuned_plan(const qse_ctx* ctx, int n, int prec, const qse_op* ops, int nops, const double* params, HostPlan& P,
                bool allow_oop, int* k_out, int* L_out);
 int run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params);
+int validate_ops(int n, const qse_op* ops, int nops, const double* params);
 std::string jit_source(const HostPlan& P, int prec, const size_t* table_off);
 int jit_compile(const std::string& src, int npasses, bool table, hipModule_t* mod, std::vector<hipFunction_t>& fns);
 int jit_launch(qse_ctx* ctx, hipFunction_t fn, const void* src, void* dst, const void* table, uint64_t ntiles, int K,
@@ -539,6 +540,82 @@ int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const do
 }  // namespace qse
 
 using namespace qse;
+
+// Measured choice of the tile shape: the cost model's best candidates are each
+// compiled and timed on st (three runs after two), the fastest is kept for the
+// gate list's structure and used by later runs of it.
+extern "C" int qse_autotune(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params,
+                            int max_candidates, int* k_out, int* L_out, double* ms_out, int* tried_k, int* tried_L,
+                            double* tried_ms, int* ntried) {
+    if (!ctx || !st || !st->d) return set_error(QSE_ERR_INVALID_ARG, "null context or state");
+    if (nops < 0 || (nops > 0 && !ops) || max_candidates < 1)
+        return set_error(QSE_ERR_INVALID_ARG, "bad op list or candidate count");
+    int rc = validate_ops(st->n, ops, nops, params);
+    if (rc) return rc;
+    if (ntried) *ntried = 0;
+    if (st->n < FUSE_MIN_QUBITS || nops == 0) {
+        if (k_out) *k_out = 0;
+        if (L_out) *L_out = 0;
+        if (ms_out) *ms_out = 0.0;
+        return QSE_OK;
+    }
+    const int prec = st->prec;
+    std::vector<ShapeCandidate> cand;
+    rc = rank_shapes(st->n, prec, ops, nops, params, true, cand);
+    if (rc) return rc;
+    if (cand.empty()) return set_error(QSE_ERR_UNSUPPORTED, "autotune: no candidate tile shape");
+    if ((int)cand.size() > max_candidates) cand.resize(max_candidates);
+    const int saved_mode = ctx->jit_mode, saved_k = ctx->tile_bits[prec], saved_L = ctx->low_bits[prec];
+    const bool saved_timing = ctx->timing;
+    ctx->jit_mode = QSE_JIT_ALWAYS;
+    ctx->timing = false;
+    hipEvent_t a = nullptr, b = nullptr;
+    int best = -1;
+    double best_ms = 0.0;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e != hipSuccess) rc = hip_error(e, "hipEventCreate");
+    for (int c = 0; c < (int)cand.size() && rc == QSE_OK; ++c) {
+        ctx->tile_bits[prec] = cand[c].k;
+        ctx->low_bits[prec] = cand[c].L;
+        for (int r = 0; r < 2 && rc == QSE_OK; ++r) rc = run_fused(ctx, st, ops, nops, params, nullptr, false, true);
+        if (rc) break;
+        e = hipEventRecord(a, ctx->stream);
+        for (int r = 0; r < 3 && rc == QSE_OK && e == hipSuccess; ++r)
+            rc = run_fused(ctx, st, ops, nops, params, nullptr, false, true);
+        if (rc) break;
+        if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        if (e != hipSuccess) {
+            rc = hip_error(e, "autotune timing");
+            break;
+        }
+        const double per = ms / 3.0;
+        if (tried_k) tried_k[c] = cand[c].k;
+        if (tried_L) tried_L[c] = cand[c].L;
+        if (tried_ms) tried_ms[c] = per;
+        if (ntried) *ntried = c + 1;
+        if (best < 0 || per < best_ms) {
+            best = c;
+            best_ms = per;
+        }
+    }
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    ctx->jit_mode = saved_mode;
+    ctx->tile_bits[prec] = saved_k;
+    ctx->low_bits[prec] = saved_L;
+    ctx->timing = saved_timing;
+    if (rc) return rc;
+    remember_shape(ctx, gate_structure_key(st->n, prec, ops, nops), cand[best].k, cand[best].L);
+    ctx->plan_cache.clear();  // plans cached before the measurement may have another shape
+    if (k_out) *k_out = cand[best].k;
+    if (L_out) *L_out = cand[best].L;
+    if (ms_out) *ms_out = best_ms;
+    return QSE_OK;
+}
 
 extern "C" int qse_plan_info(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops, int nops, int* passes,
                              int* tile_bits, int* low_bits, int* stages) {

@@ -92,6 +92,21 @@ struct ShapeEntry {
     uint64_t stamp = 0;
 };
 
+// Key of a gate list's structure for shape_cache (ops with angles and
+// matrices zeroed), and the cached entry for it (null if none).
+std::vector<char> gate_structure_key(int n, int prec, const qse_op* ops, int nops);
+const ShapeEntry* find_shape(const qse_ctx* ctx, const std::vector<char>& key);
+// Candidate tile shapes of a gate list ranked by the static cost model
+// (passes x measured relative pass cost of the shape).
+struct ShapeCandidate {
+    int k, L, passes;
+    double est;
+};
+int rank_shapes(int n, int prec, const qse_op* ops, int nops, const double* params, bool allow_oop,
+                std::vector<ShapeCandidate>& out);
+// Insert or replace the shape of a structure (least recently used out at 256).
+void remember_shape(qse_ctx* ctx, std::vector<char> key, int k, int L);
+
 // A tuned plan for one exact input (qubit count, precision, gate list,
 // parameter values, shape settings), so repeated runs skip planning.
 struct PlanEntry {

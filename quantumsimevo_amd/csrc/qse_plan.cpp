@@ -725,11 +725,84 @@ const ShapeCost kShapesC64[] = {{13, 4, 1.24}, {13, 5, 1.22}, {13, 6, 1.10},
 // The plan run_fused executes: with a tile size set on the context, that
 // shape; otherwise the candidate shape of least estimated time (passes x
 // relative pass cost, stages as a tie-break).
+std::vector<char> gate_structure_key(int n, int prec, const qse_op* ops, int nops) {
+    const int head[2] = {n, prec};
+    std::vector<char> key(sizeof(head) + sizeof(qse_op) * (size_t)nops);
+    std::memcpy(key.data(), head, sizeof(head));
+    char* w = key.data() + sizeof(head);
+    for (int i = 0; i < nops; ++i) {
+        qse_op o = ops[i];
+        o.theta = 0.0;
+        o.scale = 0.0;
+        std::fill(o.m, o.m + 8, 0.0);
+        std::memcpy(w + sizeof(qse_op) * (size_t)i, &o, sizeof(qse_op));
+    }
+    return key;
+}
+
+const ShapeEntry* find_shape(const qse_ctx* ctx, const std::vector<char>& key) {
+    if (!ctx) return nullptr;
+    for (const ShapeEntry& e : ctx->shape_cache)
+        if (e.key == key) return &e;
+    return nullptr;
+}
+
+int rank_shapes(int n, int prec, const qse_op* ops, int nops, const double* params, bool allow_oop,
+                std::vector<ShapeCandidate>& out) {
+    out.clear();
+    const ShapeCost* shapes = (prec == QSE_C128) ? kShapesC128 : kShapesC64;
+    const int ns = (prec == QSE_C128) ? (int)(sizeof(kShapesC128) / sizeof(ShapeCost))
+                                      : (int)(sizeof(kShapesC64) / sizeof(ShapeCost));
+    for (int i = 0; i < ns; ++i) {
+        const int k = shapes[i].k, L = shapes[i].L;
+        if (k > n - 2 || L > k - FUSE_REG) continue;
+        HostPlan C;
+        const int rc = choose_plan(n, k, L, ops, nops, params, C, allow_oop);
+        if (rc) return rc;
+        out.push_back({k, L, (int)C.passes.size(),
+                       (double)C.passes.size() * shapes[i].cost + 0.001 * (double)C.stages.size()});
+    }
+    std::stable_sort(out.begin(), out.end(),
+                     [](const ShapeCandidate& a, const ShapeCandidate& b) { return a.est < b.est; });
+    return QSE_OK;
+}
+
+void remember_shape(qse_ctx* ctx, std::vector<char> key, int k, int L) {
+    for (ShapeEntry& e : ctx->shape_cache)
+        if (e.key == key) {
+            e.k = k;
+            e.L = L;
+            e.stamp = ++ctx->prog_clock;
+            return;
+        }
+    if (ctx->shape_cache.size() >= 256) {
+        size_t oldest = 0;
+        for (size_t i = 1; i < ctx->shape_cache.size(); ++i)
+            if (ctx->shape_cache[i].stamp < ctx->shape_cache[oldest].stamp) oldest = i;
+        ctx->shape_cache.erase(ctx->shape_cache.begin() + oldest);
+    }
+    ShapeEntry e;
+    e.key = std::move(key);
+    e.k = k;
+    e.L = L;
+    e.stamp = ++ctx->prog_clock;
+    ctx->shape_cache.push_back(std::move(e));
+}
+
 int tuned_plan(const qse_ctx* ctx, int n, int prec, const qse_op* ops, int nops, const double* params, HostPlan& P,
                bool allow_oop, int* k_out, int* L_out) {
     int k0, L0;
     fusion_shape(ctx, n, prec, &k0, &L0);
     const bool fixed = ctx && ctx->tile_bits[prec] > 0;
+    if (!fixed && ctx && !ctx->shape_cache.empty()) {
+        // a shape measured by qse_autotune (or tuned before) for this gate-list structure
+        const ShapeEntry* e = find_shape(ctx, gate_structure_key(n, prec, ops, nops));
+        if (e && e->k <= n && e->L <= e->k - FUSE_REG) {
+            *k_out = e->k;
+            *L_out = e->L;
+            return choose_plan(n, e->k, e->L, ops, nops, params, P, allow_oop);
+        }
+    }
     if (!fixed) {
         const ShapeCost* shapes = (prec == QSE_C128) ? kShapesC128 : kShapesC64;
         const int ns = (prec == QSE_C128) ? (int)(sizeof(kShapesC128) / sizeof(ShapeCost))

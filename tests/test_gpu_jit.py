@@ -143,3 +143,35 @@ def test_parameter_loop_uses_structure_keyed_kernels():
         st.close()
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("n, make", [
+    (18, lambda n: G.qft(n).ops()),
+    (18, lambda n: G.random_layered(n, 8, seed=2).ops()),
+])
+def test_autotune_measures_and_keeps_the_fastest_shape(n, make):
+    """qse_autotune times the cost model's best shapes on the device and the
+    fastest is used by later runs of the same gate list (also at other
+    angles); results still match the oracle."""
+    ops = make(n)
+    c = capi.Context(0)
+    try:
+        st = c.state(n, capi.C128)
+        st.set_random(3)
+        r = c.autotune(st, ops, candidates=3)
+        assert 1 <= len(r["tried"]) <= 3
+        times = [t[2] for t in r["tried"]]
+        assert r["ms"] == min(times) and (r["k"], r["L"]) == r["tried"][times.index(min(times))][:2]
+        info = c.plan_info(n, capi.C128, ops)
+        assert (info["tile_bits"], info["low_bits"]) == (r["k"], r["L"])
+        # same structure, other angles: the measured shape is used too
+        other = ops.copy()
+        other["theta"] = other["theta"] * 0.5 + 0.1
+        assert c.plan_info(n, capi.C128, other)["tile_bits"] == r["k"]
+        st.set_random(3)
+        psi0 = st.download()
+        st.run(ops)
+        assert np.max(np.abs(st.download() - O.run_ops(psi0, n, ops))) < 1e-12
+        st.close()
+    finally:
+        c.close()
