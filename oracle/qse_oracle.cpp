@@ -1,7 +1,7 @@
 /*
  * qse_oracle.cpp — CPU oracle for the state-vector gate-sweep path.
  *
- * TEST INFRASTRUCTURE ONLY. This file is the checker, not the product: only
+ * TEST INFRASTRUCTURE ONLY. This file is the oracle, not the product: only
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it
  * (liboracle.so via ctypes). The product path (libqse_hip.so and the C++/Python
  * layers above it) never links, loads or calls it.

@@ -1,7 +1,7 @@
 /*
  * ref_driver.cpp — ctypes driver for the reference simulator's own CPU code.
  *
- * TEST INFRASTRUCTURE ONLY (the checker and the CPU timing baseline). Built by
+ * TEST INFRASTRUCTURE ONLY (the reference oracle and the CPU timing baseline). Built by
  * oracle/Makefile into oracle/_ref/libqse_ref.so, together with the reference's
  * CPU sources compiled where they lie under the reference tree:
  *   QuantumSimEvo/CircuitUnitaryOperation.cpp     (scalar OpenMP backend)

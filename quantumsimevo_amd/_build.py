@@ -4,7 +4,7 @@ Products (all written next to the sources, never installed):
   quantumsimevo_amd/libqse_hip.so        HIP kernels for gfx950 + the C ABI (include/qse_hip.h)
   quantumsimevo_amd/_quantumsimevo*.so   pybind11 module: the C++ Circuit / StateVector /
                                          Noise / CircuitUnitaryOperationHIP host classes
-  oracle/liboracle.so (+ oracle/_ref/)   the CPU checker (test infrastructure, via oracle/Makefile)
+  oracle/liboracle.so (+ oracle/_ref/)   the CPU oracle (test infrastructure, via oracle/Makefile)
 
 Every HIP source is compiled with hipcc --offload-arch=gfx950 and with packed
 FP32 instructions disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).

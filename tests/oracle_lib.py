@@ -1,4 +1,4 @@
-"""ctypes access to the CPU checker libraries (test infrastructure).
+"""ctypes access to the CPU oracle libraries (test infrastructure).
 
   oracle/liboracle.so       the restated scalar FP64 oracle (oracle/qse_oracle.cpp)
   oracle/_ref/libqse_ref.so the reference simulator's own CPU backends (built only

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/qse_oracle.cpp) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/qse_oracle.cpp) before the GPU tests compare against it.
 
 Each oracle function is checked against something independent of its own formula:
 dense Kronecker-product matrices (numpy), the reference's known-answer states
