@@ -6,7 +6,7 @@ import math
 
 import numpy as np
 import pytest
-import torch  # imported at collection time (first import can be slow on a fresh box)
+import torch  # imported at collection time (the first import in a fresh environment can take minutes)
 
 import oracle_lib as O
 from quantumsimevo_amd import autograd as AG
