@@ -237,7 +237,8 @@ def main() -> int:
     plan0 = ctx.plan_info(n, prec, ops) if not args.unfused else {"low_bits": 0}
     # practical copy ceiling of this device for the pass's access pattern (tiled out-of-place copy of the
     # state's size; complex64 runs of 2^L amplitudes are 2^(L-1) 16-byte elements)
-    log2_el = n if prec == capi.C128 else n - 1
+    # (buffers of the state's size, at most 2^31 16-byte elements = 32 GiB each)
+    log2_el = min(n if prec == capi.C128 else n - 1, 31)
     run_el = max(plan0["low_bits"] - (0 if prec == capi.C128 else 1), 0)
     copy_contig = ctx.copy_bandwidth(log2_el, 12)
     copy_runs = ctx.copy_bandwidth(log2_el, run_el) if run_el < 12 else copy_contig
