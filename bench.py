@@ -187,6 +187,54 @@ def uneven_batch_leg(ctx) -> dict:
             "max_abs_grad_err_vs_fp64": float(max(np.max(np.abs(a - b)) for a, b in zip(G64, Gr)))}
 
 
+def dense_block_leg(ctx, restore_jit: int, n: int = 30) -> dict:
+    """Matrix-core dense blocks against the fused gate path: a random layered
+    circuit of `layers` layers on qubits 0..m-1 (one of H/RX/RY/RZ/phase per
+    qubit per layer, CNOT brickwork) applied to an n-qubit complex128 state,
+    either gate by gate in one fused pass (run-time specialised kernel) or as
+    its composed 2^m x 2^m unitary with FP64 MFMA (qse_apply_dense). Kernel
+    times are HIP events of the library (mean of 3 launches after 2 warm-up
+    runs); both paths move 2 x 16 x 2^n bytes. The results of the two paths
+    are compared on a 20-qubit state."""
+    rows = []
+    st = ctx.state(n, capi.C128)
+    st.set_random(7)
+    a, b = ctx.state(20, capi.C128), ctx.state(20, capi.C128)
+    ctx.set_jit(capi.JIT_ALWAYS)
+    for m in (4, 5, 6):
+        for layers in (1, 16, 64):
+            ops = C.random_layered(m, layers, seed=layers).ops()
+            u = capi.compose_dense(m, ops)
+            for _ in range(2):
+                st.run(ops)
+                st.apply_dense(m, u)
+            ctx.sync()
+            ctx.reset_timing()
+            ctx.enable_timing(True)
+            for _ in range(3):
+                st.run(ops)
+            for _ in range(3):
+                st.apply_dense(m, u)
+            ctx.sync()
+            ctx.enable_timing(False)
+            f, d = ctx.timing(capi.FAM_FUSED), ctx.timing(capi.FAM_DENSE)
+            a.set_random(3)
+            b.set_random(3)
+            a.run(ops)
+            b.apply_dense(m, u)
+            diff = float(np.max(np.abs(a.download() - b.download())))
+            fms, dms = f["ms"] / max(f["launches"], 1), d["ms"] / max(d["launches"], 1)
+            rows.append({"m": m, "layers": layers, "gates": int(len(ops)), "fused_ms": fms, "dense_mfma_ms": dms,
+                         "dense_gbs": 2.0 * 16 * 2.0 ** n / (dms * 1e-3) / 1e9, "faster": "mfma" if dms < fms else "fused",
+                         "max_abs_diff_20q": diff})
+            log(f"dense block m={m} layers={layers}: fused {fms:.3f} ms, MFMA {dms:.3f} ms")
+    ctx.set_jit(restore_jit)
+    for s_ in (st, a, b):
+        s_.close()
+    return {"workload": f"random layered block on qubits 0..m-1 of a {n}-qubit complex128 state: fused gate pass "
+                        "vs composed 2^m x 2^m unitary on the FP64 matrix cores", "rows": rows}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +253,7 @@ def main() -> int:
     ap.add_argument("--qaoa-batch", type=int, default=64)
     ap.add_argument("--qaoa-shift-sets", type=int, default=1)
     ap.add_argument("--no-extra", action="store_true", help="skip the noise-trajectory and uneven-batch legs")
+    ap.add_argument("--no-dense", action="store_true", help="skip the matrix-core dense-block leg")
     ap.add_argument("--verify", type=int, default=16, metavar="K",
                     help="after timing, rerun one QFT from the seed and compare K output amplitudes with the FP64 "
                          "DFT closed form evaluated on the GPU (qse_qft_probe)")
@@ -343,6 +392,9 @@ def main() -> int:
                                          "seeded random state"}
     st.close()
 
+    if not args.no_dense:
+        result["dense_block"] = dense_block_leg(
+            ctx, {"auto": capi.JIT_AUTO, "off": capi.JIT_OFF, "always": capi.JIT_ALWAYS}[args.jit])
     if not args.no_qaoa:
         result["qaoa"] = qaoa_leg(ctx, args.qaoa_batch, args.qaoa_shift_sets)
     if not args.no_extra:

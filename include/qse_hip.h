@@ -158,7 +158,8 @@ int qse_ctx_jit_stats(qse_ctx* ctx, int64_t* compiled, int64_t* jit_launches, do
 int qse_ctx_jit_wait(qse_ctx* ctx);
 /* Kernel timing: when enabled, every kernel launched by the context records HIP
  * events; qse_ctx_timing() returns the count and summed milliseconds per kernel
- * family. Families: 0 pair sweep, 1 fused tile pass, 2 readout, 3 adjoint, 4 batched circuits. */
+ * family. Families: 0 pair sweep, 1 fused tile pass, 2 readout, 3 adjoint, 4 batched circuits,
+ * 5 dense matrix-core block. */
 int qse_ctx_enable_timing(qse_ctx* ctx, int enable);
 int qse_ctx_timing(qse_ctx* ctx, int family, int64_t* launches, double* total_ms, double* bytes);
 int qse_ctx_reset_timing(qse_ctx* ctx);
@@ -242,6 +243,17 @@ int qse_qft_probe(qse_ctx* ctx, int nqubits, uint64_t seed, const uint64_t* indi
  * pattern of a fused pass with run_log2 contiguous low qubits (complex128).
  * Allocates two buffers of 16 * 2^log2_elems bytes for the call; synchronises. */
 int qse_copy_bandwidth(qse_ctx* ctx, int log2_elems, int run_log2, double* gbs);
+
+/* ---- dense blocks on the matrix cores (no counterpart in the reference) ----
+ * qse_compose_dense: the 2^m x 2^m unitary of a gate list on qubits 0..m-1
+ * (gate semantics of CircuitUnitaryOperation.cpp:10-338), computed on the host;
+ * u is row-major, (re, im) interleaved, 2 * 4^m doubles; row = output index.
+ * qse_apply_dense: psi <- (U (x) on qubits 0..m-1) psi for m = 4, 5, 6 on a
+ * complex128 state of at least m + 4 qubits, one pass over the state with FP64
+ * MFMA (v_mfma_f64_16x16x4f64); QSE_ERR_UNSUPPORTED for complex64 states or
+ * other m. Stream-ordered. */
+int qse_compose_dense(int m, const qse_op* ops, int nops, const double* params, double* u);
+int qse_apply_dense(qse_ctx* ctx, qse_state* st, int m, const double* u);
 
 /* ---- readout (StateVector.cpp:77-156), driven by host-supplied uniforms ----
  * u are canonical uniforms in [0,1) (std::generate_canonical<double,53>); the

@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 LIB_SOURCES = ["qse_capi.cpp", "qse_run.cpp", "qse_gates.hip", "qse_readout.hip", "qse_fused.hip",
                "qse_plan.cpp", "qse_adjoint.hip", "qse_jit.cpp", "qse_jit_gen.cpp", "qse_check.hip",
-               "qse_adjoint_plan.cpp", "qse_batch_plan.cpp", "qse_batch.hip"]
+               "qse_adjoint_plan.cpp", "qse_batch_plan.cpp", "qse_batch.hip", "qse_dense.hip"]
 HOST_SOURCES = ["StateVector.cpp", "Noise.cpp", "CircuitUnitaryOperationHIP.cpp",
                 "CircuitUnitaryOperationFactory.cpp", "Circuit.cpp", "HardwareCheck.cpp", "GateListFile.cpp"]
 

@@ -27,7 +27,7 @@ OP_DTYPE = np.dtype([("kind", "<i4"), ("target", "<i4"), ("control", "<i4"), ("p
                      ("controls", "<u8"), ("theta", "<f8"), ("scale", "<f8"), ("m", "<f8", (8,))])
 assert OP_DTYPE.itemsize == 104
 
-FAM_SWEEP, FAM_FUSED, FAM_READOUT, FAM_ADJOINT, FAM_BATCH = range(5)
+FAM_SWEEP, FAM_FUSED, FAM_READOUT, FAM_ADJOINT, FAM_BATCH, FAM_DENSE = range(6)
 JIT_OFF, JIT_AUTO, JIT_ALWAYS = range(3)
 
 # Every symbol include/qse_hip.h declares (checked by the CPU tests).
@@ -42,6 +42,7 @@ EXPORTS = [
     "qse_apply_swap", "qse_pauli_frame", "qse_run", "qse_run_repeat", "qse_run_sweeps", "qse_plan_passes", "qse_plan_info", "qse_qft_probe", "qse_copy_bandwidth", "qse_sample",
     "qse_measure", "qse_norm2", "qse_prob_one", "qse_collapse_qubit", "qse_expect_zstrings",
     "qse_adjoint_expect", "qse_batch_expect", "qse_batch_run_small", "qse_sample_frames",
+    "qse_compose_dense", "qse_apply_dense",
 ]
 
 
@@ -115,6 +116,8 @@ def _sigs(lib):
         "qse_plan_info": ([_vp, _i, _i, _vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)], _i),
         "qse_qft_probe": ([_vp, _i, C.c_uint64, _vp, _i, _vp], _i),
         "qse_copy_bandwidth": ([_vp, _i, _i, _pd], _i),
+        "qse_compose_dense": ([_i, _vp, _i, _vp, _vp], _i),
+        "qse_apply_dense": ([_vp, _vp, _i, _vp], _i),
         "qse_sample": ([_vp, _vp, _vp, _i64, _vp], _i),
         "qse_measure": ([_vp, _vp, _d, _pu64], _i),
         "qse_norm2": ([_vp, _vp, _pd], _i),
@@ -170,6 +173,16 @@ def device_info(device: int = 0) -> dict:
     check(load().qse_device_mem_info(device, C.byref(free), C.byref(tot2)), "qse_device_mem_info")
     return {"name": name.value.decode(), "compute_units": cus.value, "total_bytes": tot.value,
             "free_bytes": free.value}
+
+
+def compose_dense(m: int, ops, params=None) -> np.ndarray:
+    """The (2^m, 2^m) unitary of a gate list on qubits 0..m-1 (host computation)."""
+    a = as_ops(ops)
+    p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+    u = np.empty((1 << m, 1 << m), dtype=np.complex128)
+    check(load().qse_compose_dense(int(m), _ptr(a), len(a), None if p is None else _ptr(p), _ptr(u)),
+          "qse_compose_dense")
+    return u
 
 
 def as_ops(ops) -> np.ndarray:
@@ -422,6 +435,14 @@ class State:
         p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
         check(self.lib.qse_run_repeat(self.ctx.h, self.h, _ptr(a), len(a), None if p is None else _ptr(p), int(reps)),
               "qse_run_repeat")
+
+    def apply_dense(self, m: int, u):
+        """psi <- (U on qubits 0..m-1) psi with the FP64 matrix cores (m = 4, 5, 6;
+        complex128 states); u is a complex (2^m, 2^m) matrix."""
+        uu = np.ascontiguousarray(u, dtype=np.complex128)
+        if uu.shape != (1 << m, 1 << m):
+            raise ValueError("u must be a (2^m, 2^m) matrix")
+        check(self.lib.qse_apply_dense(self.ctx.h, self.h, int(m), _ptr(uu)), "qse_apply_dense")
 
     # readout
     def sample(self, u) -> np.ndarray:

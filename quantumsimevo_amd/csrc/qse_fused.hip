@@ -241,8 +241,8 @@ __global__ __launch_bounds__(1 << (K - 4)) __attribute__((amdgpu_waves_per_eu(2,
 
 // Device copy of a serialized program, reused when the same program runs again.
 namespace {
-
 size_t ring_len(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+}  // namespace
 
 // Copy `bytes` into a fresh place of the program ring (growing the ring when
 // the program does not fit) and return its offset. Cache entries whose device
@@ -295,8 +295,6 @@ int ring_upload(qse_ctx* ctx, const std::vector<char>& bytes, size_t* off) {
     *off = b;
     return QSE_OK;
 }
-
-}  // namespace
 
 int program_on_device(qse_ctx* ctx, const Serialized& S, int prec, ProgEntry** entry) {
     ProgEntry* hit = nullptr;

@@ -16,7 +16,15 @@
 namespace qse {
 
 // Kernel families for the timing report (qse_ctx_timing).
-enum Family { FAM_SWEEP = 0, FAM_FUSED = 1, FAM_READOUT = 2, FAM_ADJOINT = 3, FAM_BATCH = 4, FAM_COUNT = 5 };
+enum Family {
+    FAM_SWEEP = 0,
+    FAM_FUSED = 1,
+    FAM_READOUT = 2,
+    FAM_ADJOINT = 3,
+    FAM_BATCH = 4,
+    FAM_DENSE = 5,
+    FAM_COUNT = 6
+};
 
 struct TimedLaunch {
     int family;
@@ -187,6 +195,12 @@ int pinned(qse_ctx* ctx, size_t bytes, void** out);
 void launch_begin(qse_ctx* ctx, int family, TimedLaunch* t);
 void launch_end(qse_ctx* ctx, TimedLaunch* t, double bytes);
 int fold_timing(qse_ctx* ctx);
+
+// Stream-ordered upload of a small host block (programs, matrices) into the
+// device program ring; the device copy stays valid until later uploads wrap
+// the ring around to its place, which happens only after the kernels launched
+// before them. Returns the byte offset in ctx->ring.dev.
+int ring_upload(qse_ctx* ctx, const std::vector<char>& bytes, size_t* off);
 
 // Grid for a grid-stride sweep over `work` items with `block` threads.
 unsigned grid_for(const qse_ctx* ctx, uint64_t work, int block, int waves_per_cu = 32);
