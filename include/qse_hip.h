@@ -268,6 +268,21 @@ int qse_sample(qse_ctx* ctx, qse_state* st, const double* u, int64_t nshots, uin
 int qse_measure(qse_ctx* ctx, qse_state* st, double u, uint64_t* outcome);
 /* sum_i |a_i|^2 in FP64. */
 int qse_norm2(qse_ctx* ctx, qse_state* st, double* out);
+/* FP64 renormalisation on the device (no counterpart in the reference):
+ * psi <- psi / sqrt(sum_i |a_i|^2), the norm reduced and the scale factor
+ * computed in FP64 whatever the state precision, without a host round trip;
+ * stream-ordered unless norm2_before is non-null, which then receives the
+ * norm^2 before scaling (synchronises). A state of zero or non-finite norm is
+ * left unchanged. */
+int qse_state_renormalize(qse_ctx* ctx, qse_state* st, double* norm2_before);
+/* Periodic renormalisation (no counterpart in the reference): with
+ * every_passes > 0, qse_run, qse_run_repeat and qse_run_sweeps count the passes
+ * they make over a state (fused passes, or one per gate on the sweep path) and
+ * renormalise it at the end of the call that reaches every_passes passes since
+ * the last renormalisation. 0 (the default) turns it off. Bounds the norm drift
+ * of long complex64 runs; internal runs (adjoint, batched expectations) are not
+ * renormalised. */
+int qse_ctx_set_renormalize(qse_ctx* ctx, int every_passes);
 /* Probability of qubit q being 1 (StateVector::measureQubit, StateVector.cpp:158-185). */
 int qse_prob_one(qse_ctx* ctx, qse_state* st, int q, double* out);
 /* Mid-circuit single-qubit collapse with a host uniform u: outcome = (u < P(1));

@@ -58,6 +58,11 @@ public:
     int collapseQubit(StateVector& sv, size_t q, double u, bool reset);
     double expectZStrings(StateVector& sv, const std::vector<uint64_t>& masks, const std::vector<double>& weights);
     double norm2(StateVector& sv);
+    // psi / ||psi|| in FP64 on the device; returns the norm^2 before.
+    double renormalize(StateVector& sv);
+    // Renormalise once gate execution has made `passes` passes over the state
+    // since the last time (0: never; qse_ctx_set_renormalize).
+    void setRenormalizePeriod(int passes);
     // Replace the state's content (host layout: complex<double>).
     void setState(StateVector& sv, const std::vector<Complex>& amps);
     // Copy of the device state without touching the StateVector's host buffer.

@@ -42,7 +42,7 @@ EXPORTS = [
     "qse_apply_swap", "qse_pauli_frame", "qse_run", "qse_run_repeat", "qse_run_sweeps", "qse_plan_passes", "qse_plan_info", "qse_qft_probe", "qse_copy_bandwidth", "qse_sample",
     "qse_measure", "qse_norm2", "qse_prob_one", "qse_collapse_qubit", "qse_expect_zstrings",
     "qse_adjoint_expect", "qse_batch_expect", "qse_batch_run_small", "qse_sample_frames",
-    "qse_compose_dense", "qse_apply_dense",
+    "qse_compose_dense", "qse_apply_dense", "qse_state_renormalize", "qse_ctx_set_renormalize",
 ]
 
 
@@ -117,6 +117,8 @@ def _sigs(lib):
         "qse_qft_probe": ([_vp, _i, C.c_uint64, _vp, _i, _vp], _i),
         "qse_copy_bandwidth": ([_vp, _i, _i, _pd], _i),
         "qse_compose_dense": ([_i, _vp, _i, _vp, _vp], _i),
+        "qse_state_renormalize": ([_vp, _vp, _vp], _i),
+        "qse_ctx_set_renormalize": ([_vp, _i], _i),
         "qse_apply_dense": ([_vp, _vp, _i, _vp], _i),
         "qse_sample": ([_vp, _vp, _vp, _i64, _vp], _i),
         "qse_measure": ([_vp, _vp, _d, _pu64], _i),
@@ -221,6 +223,11 @@ class Context:
         k, l = C.c_int(0), C.c_int(0)
         check(self.lib.qse_ctx_get_fusion(self.h, precision, C.byref(k), C.byref(l)), "qse_ctx_get_fusion")
         return k.value, l.value
+
+    def set_renormalize(self, every_passes: int):
+        """Renormalise states (FP64, on the device) once qse_run calls have made
+        every_passes passes over them since the last time; 0 turns it off."""
+        check(self.lib.qse_ctx_set_renormalize(self.h, int(every_passes)), "qse_ctx_set_renormalize")
 
     def set_jit(self, mode: int):
         """JIT_OFF, JIT_AUTO (default: compile a program on its second run) or JIT_ALWAYS."""
@@ -435,6 +442,12 @@ class State:
         p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
         check(self.lib.qse_run_repeat(self.ctx.h, self.h, _ptr(a), len(a), None if p is None else _ptr(p), int(reps)),
               "qse_run_repeat")
+
+    def renormalize(self) -> float:
+        """psi / ||psi|| in FP64 on the device; returns the norm^2 before."""
+        v = C.c_double(0)
+        check(self.lib.qse_state_renormalize(self.ctx.h, self.h, C.byref(v)), "qse_state_renormalize")
+        return v.value
 
     def apply_dense(self, m: int, u):
         """psi <- (U on qubits 0..m-1) psi with the FP64 matrix cores (m = 4, 5, 6;

@@ -218,6 +218,18 @@ double CircuitUnitaryOperationHIP::norm2(StateVector& sv) {
     return v;
 }
 
+double CircuitUnitaryOperationHIP::renormalize(StateVector& sv) {
+    ensureOnDevice(sv);
+    execute();
+    double v = 0.0;
+    check(qse_state_renormalize(ctx, d_state, &v), "qse_state_renormalize");
+    return v;
+}
+
+void CircuitUnitaryOperationHIP::setRenormalizePeriod(int passes) {
+    check(qse_ctx_set_renormalize(ctx, passes), "qse_ctx_set_renormalize");
+}
+
 void CircuitUnitaryOperationHIP::setState(StateVector& sv, const std::vector<Complex>& amps) {
     if (amps.size() != sv.size()) throw std::invalid_argument("setState: size mismatch");
     ensureOnDevice(sv);

@@ -584,6 +584,27 @@ int qse_norm2(qse_ctx* ctx, qse_state* st, double* out) {
     return reduce(ctx, st, 0, 0, nullptr, nullptr, 0, out);
 }
 
+int qse_state_renormalize(qse_ctx* ctx, qse_state* st, double* norm2_before) {
+    int rc = check_state(ctx, st);
+    if (rc) return rc;
+    double* d = nullptr;
+    rc = renormalize_async(ctx, st, &d);
+    if (rc || !norm2_before) return rc;
+    void* pin;
+    rc = pinned(ctx, sizeof(double), &pin);
+    if (rc) return rc;
+    QSE_HIP(hipMemcpyAsync(pin, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    QSE_HIP(hipStreamSynchronize(ctx->stream));
+    *norm2_before = *(double*)pin;
+    return QSE_OK;
+}
+
+int qse_ctx_set_renormalize(qse_ctx* ctx, int every_passes) {
+    if (!ctx || every_passes < 0) return set_error(QSE_ERR_INVALID_ARG, "bad renormalisation period");
+    ctx->renorm_every = every_passes;
+    return QSE_OK;
+}
+
 int qse_prob_one(qse_ctx* ctx, qse_state* st, int q, double* out) {
     int rc = check_state(ctx, st);
     if (rc) return rc;

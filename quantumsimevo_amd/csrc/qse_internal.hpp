@@ -132,7 +132,7 @@ struct qse_ctx {
     size_t lds_per_block = 65536;
     hipStream_t stream = nullptr;
     // Device scratch, grown on demand (partials, flags, cumulative arrays).
-    qse::Scratch scratch[10];
+    qse::Scratch scratch[11];
     // Recently run fused programs, their device copies in the program ring.
     std::vector<qse::ProgEntry> prog_cache;
     qse::ProgRing ring;
@@ -147,6 +147,8 @@ struct qse_ctx {
     // Pinned host staging for small readbacks.
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
+    // Renormalise states after this many passes of qse_run calls (0: never).
+    int64_t renorm_every = 0;
     // Fusion shape per precision (tile qubits k, contiguous low qubits L).
     int tile_bits[2] = {0, 0};
     int low_bits[2] = {0, 0};
@@ -173,6 +175,8 @@ struct qse_state {
     void* shadow = nullptr;
     size_t bytes = 0;
     bool owned = true;
+    // Passes of qse_run calls since the last periodic renormalisation.
+    int64_t passes_since_renorm = 0;
 };
 
 namespace qse {
@@ -232,5 +236,12 @@ int launch_convert_upload(qse_ctx* ctx, qse_state* st, const double* dev_c128, u
 int launch_convert_download(qse_ctx* ctx, const qse_state* st, double* dev_c128, uint64_t count, uint64_t first);
 int launch_scale(qse_ctx* ctx, qse_state* st, double s);
 int launch_random(qse_ctx* ctx, qse_state* st, uint64_t seed);
+
+// ---- readout helpers (qse_readout.hip) ----
+// Stream-ordered FP64 renormalisation; *d_nrm2 (if given) receives the device
+// address of the norm^2 measured before scaling.
+int renormalize_async(qse_ctx* ctx, qse_state* st, double** d_nrm2);
+// Count `passes` against the context's renormalisation period.
+int renorm_tick(qse_ctx* ctx, qse_state* st, int64_t passes);
 
 }  // namespace qse

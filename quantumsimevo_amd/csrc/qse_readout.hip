@@ -424,6 +424,63 @@ int reduce_zstr_async(qse_ctx* ctx, const qse_state* st, const uint64_t* d_m, co
     return QSE_OK;
 }
 
+// FP64 renormalisation on the device: norm^2 = sum |a_i|^2 with the partial
+// sums and summation order of reduce(RED_NORM), then every amplitude times
+// 1/sqrt(norm^2), computed in FP64 from the device copy of norm^2 (no host
+// round trip). A state whose norm^2 is zero or not finite is left unchanged.
+template <class R>
+__global__ __launch_bounds__(RB_THREADS) void k_scale_norm(cx<R>* __restrict__ s, uint64_t n,
+                                                           const double* __restrict__ nrm2) {
+    const double v = *nrm2;
+    const double f = (v > 0.0 && v <= 1.7976931348623157e308) ? 1.0 / sqrt(v) : 1.0;
+    for (uint64_t k = gtid(); k < n; k += gstride()) {
+        const cx<R> a = s[k];
+        s[k] = {(R)((double)a.re * f), (R)((double)a.im * f)};
+    }
+}
+
+int renormalize_async(qse_ctx* ctx, qse_state* st, double** d_nrm2) {
+    const uint64_t N = 1ULL << st->n;
+    const unsigned g = std::min<unsigned>(grid_for(ctx, N, RB_THREADS, 8), 2048u);
+    void* base;
+    int rc = scratch(ctx, 10, sizeof(double) * (g + 1), &base);
+    if (rc) return rc;
+    double* d_part = (double*)base;
+    double* d_out = d_part + g;
+    TimedLaunch t;
+    launch_begin(ctx, FAM_READOUT, &t);
+    if (st->prec == QSE_C128) {
+        k_reduce<double, RED_NORM><<<g, RB_THREADS, 0, ctx->stream>>>((const cx<double>*)st->d, N, 0, nullptr,
+                                                                       nullptr, 0, d_part);
+    } else {
+        k_reduce<float, RED_NORM><<<g, RB_THREADS, 0, ctx->stream>>>((const cx<float>*)st->d, N, 0, nullptr,
+                                                                      nullptr, 0, d_part);
+    }
+    QSE_HIP(hipGetLastError());
+    k_sum_partials<<<1, 64, 0, ctx->stream>>>(d_part, (int)g, d_out);
+    QSE_HIP(hipGetLastError());
+    const unsigned gs = grid_for(ctx, N, RB_THREADS);
+    if (st->prec == QSE_C128)
+        k_scale_norm<double><<<gs, RB_THREADS, 0, ctx->stream>>>((cx<double>*)st->d, N, d_out);
+    else
+        k_scale_norm<float><<<gs, RB_THREADS, 0, ctx->stream>>>((cx<float>*)st->d, N, d_out);
+    QSE_HIP(hipGetLastError());
+    launch_end(ctx, &t, 3.0 * (double)amp_bytes(st->prec) * N);
+    if (d_nrm2) *d_nrm2 = d_out;
+    return QSE_OK;
+}
+
+// Periodic renormalisation of qse_run / qse_run_repeat results: count the
+// passes a call made over the state and renormalise once the count reaches
+// the context's period.
+int renorm_tick(qse_ctx* ctx, qse_state* st, int64_t passes) {
+    if (ctx->renorm_every <= 0) return QSE_OK;
+    st->passes_since_renorm += passes;
+    if (st->passes_since_renorm < ctx->renorm_every) return QSE_OK;
+    st->passes_since_renorm = 0;
+    return renormalize_async(ctx, st, nullptr);
+}
+
 // Single-qubit collapse (StateVector.cpp:158-221): keep the `outcome` half
 // scaled by 1/nf; with reset, move it to the |0> half and zero the |1> half.
 template <class R>

@@ -62,7 +62,10 @@ int qse_run(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const doub
     int rc = validate_ops(st->n, ops, nops, params);
     if (rc) return rc;
     if (nops == 0) return QSE_OK;
-    return run_fused(ctx, st, ops, nops, params, nullptr, false, true);
+    int passes = 0;
+    rc = run_fused(ctx, st, ops, nops, params, &passes, false, true);
+    if (rc) return rc;
+    return renorm_tick(ctx, st, passes);
 }
 
 int qse_run_repeat(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int reps) {
@@ -74,12 +77,14 @@ int qse_run_repeat(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, con
     // Eager runs first: they cache the plan, place the program on the device,
     // size the scratch buffers and (under QSE_JIT_AUTO) compile the passes, so
     // the captured run makes no allocation, upload, compile or host wait.
-    int done = 0;
+    int done = 0, passes = 0;
     for (; done < std::min(reps, 2); ++done) {
-        rc = run_fused(ctx, st, ops, nops, params, nullptr, false, true);
+        rc = run_fused(ctx, st, ops, nops, params, &passes, false, true);
         if (rc) return rc;
     }
-    if (done == reps) return QSE_OK;
+    // periodic renormalisation is counted over the whole call and applied at
+    // its end, so the captured runs hold no renormalisation
+    if (done == reps) return renorm_tick(ctx, st, (int64_t)passes * reps);
     // Capture one run, or two when a run exchanges the state's buffers (an
     // out-of-place last pass), so that the replay leaves the pointers as the
     // capture found them.
@@ -116,7 +121,7 @@ int qse_run_repeat(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, con
         rc = run_fused(ctx, st, ops, nops, params, nullptr, false, true);
         if (rc) return rc;
     }
-    return QSE_OK;
+    return renorm_tick(ctx, st, (int64_t)passes * reps);
 }
 
 int qse_run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params) {
@@ -124,7 +129,9 @@ int qse_run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, con
     if (nops < 0 || (nops > 0 && !ops)) return set_error(QSE_ERR_INVALID_ARG, "bad op list");
     int rc = validate_ops(st->n, ops, nops, params);
     if (rc) return rc;
-    return run_sweeps(ctx, st, ops, nops, params);
+    rc = run_sweeps(ctx, st, ops, nops, params);
+    if (rc) return rc;
+    return renorm_tick(ctx, st, nops);
 }
 
 }  // extern "C"
