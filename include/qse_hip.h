@@ -309,6 +309,17 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision,
                        const uint64_t* obs_masks, const double* obs_weights, int nterms,
                        int ckpt_every,
                        double* energies, double* grads);
+/* How qse_adjoint_expect evaluates the terms of X and Y generators (RX, RY)
+ * on states of 13 qubits or more (no counterpart in the reference):
+ * QSE_ADJ_TERMS_ROTATE turns them into Z-strings by rotating psi and lambda
+ * (H, or H S^dag, on their qubits, folded into the un-applications), which
+ * costs passes where no un-application absorbs the rotation;
+ * QSE_ADJ_TERMS_PAIRS sums amplitude pairs, Im<lambda|X_q|psi>, in read-only
+ * passes over both states (one per 12 tile qubits); QSE_ADJ_TERMS_AUTO (the
+ * default) takes the schedule with fewer passes by the planner's count. The
+ * results agree to rounding. */
+enum { QSE_ADJ_TERMS_AUTO = 0, QSE_ADJ_TERMS_ROTATE = 1, QSE_ADJ_TERMS_PAIRS = 2 };
+int qse_ctx_set_adjoint_terms(qse_ctx* ctx, int mode);
 /* Plain forward expectation for a batch of parameter sets (used by parameter
  * shift and finite differences). */
 int qse_batch_expect(qse_ctx* ctx, int nqubits, int precision,

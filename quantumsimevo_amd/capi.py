@@ -29,6 +29,7 @@ assert OP_DTYPE.itemsize == 104
 
 FAM_SWEEP, FAM_FUSED, FAM_READOUT, FAM_ADJOINT, FAM_BATCH, FAM_DENSE = range(6)
 JIT_OFF, JIT_AUTO, JIT_ALWAYS = range(3)
+ADJ_TERMS_AUTO, ADJ_TERMS_ROTATE, ADJ_TERMS_PAIRS = range(3)
 
 # Every symbol include/qse_hip.h declares (checked by the CPU tests).
 EXPORTS = [
@@ -43,6 +44,7 @@ EXPORTS = [
     "qse_measure", "qse_norm2", "qse_prob_one", "qse_collapse_qubit", "qse_expect_zstrings",
     "qse_adjoint_expect", "qse_batch_expect", "qse_batch_run_small", "qse_sample_frames",
     "qse_compose_dense", "qse_apply_dense", "qse_state_renormalize", "qse_ctx_set_renormalize",
+    "qse_ctx_set_adjoint_terms",
 ]
 
 
@@ -119,6 +121,7 @@ def _sigs(lib):
         "qse_compose_dense": ([_i, _vp, _i, _vp, _vp], _i),
         "qse_state_renormalize": ([_vp, _vp, _vp], _i),
         "qse_ctx_set_renormalize": ([_vp, _i], _i),
+        "qse_ctx_set_adjoint_terms": ([_vp, _i], _i),
         "qse_apply_dense": ([_vp, _vp, _i, _vp], _i),
         "qse_sample": ([_vp, _vp, _vp, _i64, _vp], _i),
         "qse_measure": ([_vp, _vp, _d, _pu64], _i),
@@ -223,6 +226,11 @@ class Context:
         k, l = C.c_int(0), C.c_int(0)
         check(self.lib.qse_ctx_get_fusion(self.h, precision, C.byref(k), C.byref(l)), "qse_ctx_get_fusion")
         return k.value, l.value
+
+    def set_adjoint_terms(self, mode: int):
+        """ADJ_TERMS_AUTO (default), ADJ_TERMS_ROTATE or ADJ_TERMS_PAIRS: how the
+        adjoint evaluates RX/RY generator terms on 13+ qubit states."""
+        check(self.lib.qse_ctx_set_adjoint_terms(self.h, int(mode)), "qse_ctx_set_adjoint_terms")
 
     def set_renormalize(self, every_passes: int):
         """Renormalise states (FP64, on the device) once qse_run calls have made

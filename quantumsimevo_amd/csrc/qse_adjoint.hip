@@ -44,6 +44,7 @@ int tuned_plan(const qse_ctx* ctx, int n, int prec, const qse_op* ops, int nops,
 int validate_ops(int n, const qse_op* ops, int nops, const double* params);
 int reduce_zstr_async(qse_ctx* ctx, const qse_state* st, const uint64_t* d_m, const double* d_w, int nterms,
                       double* d_out);
+bool is_zz_triple(const qse_op* ops, int nops, int k);
 
 static constexpr int AD_THREADS = 256;
 static constexpr int DIAG_CHUNK = 16;
@@ -192,6 +193,144 @@ __global__ __launch_bounds__(AD_THREADS) void k_sum_columns(const double* __rest
     if (threadIdx.x == 0) out[d] = sh[0];
 }
 
+// X/Y generator terms without rotating psi and lambda: for a tile qubit q with
+// local bit b, Im<lambda|X_q|psi> = sum over pairs (a, a|b) of
+// Im(conj(l_a) p_{a|b}) + Im(conj(l_{a|b}) p_a), and Im<lambda|Y_q|psi> =
+// sum of -Re(conj(l_a) p_{a|b}) + Re(conj(l_{a|b}) p_a). One read-only pass
+// over both vectors covers the terms of up to 12 tile qubits: the tile's 2^12
+// amplitudes of each vector sit 16 per thread in registers, 4 local bits at a
+// time (groups {3,4,5,6}, {7,8,9,10}, {0,1,2,11}), and move between the
+// groups through LDS, one vector at a time. Local bits 0..2 are global qubits
+// 0..2 (runs of 8 contiguous amplitudes); the other 9 local bits and the
+// tile index bits map to global qubits through the pass description.
+constexpr int PAIR_K = 12;
+constexpr int PAIR_THREADS = 256;
+
+struct PairPass {
+    uint32_t tq[PAIR_K];     // local bit -> global qubit (tq[0..2] = 0, 1, 2)
+    uint32_t epos[64];       // tile index bit -> global qubit (ascending)
+    int32_t next;            // tile index bits (n - 12)
+    int32_t nstages;         // register groups to visit (terms beyond are absent)
+    int32_t col[PAIR_K];     // output column of the term on each local bit, -1: none
+    int32_t type[PAIR_K];    // 0: X, 1: Y
+};
+
+__device__ __forceinline__ constexpr int pair_group(int s, int j) {
+    return s == 0 ? 3 + j : s == 1 ? 7 + j : (j < 3 ? j : 11);
+}
+// thread-id bit i of stage s -> local bit (the 8 local bits outside group s, ascending)
+__device__ __forceinline__ constexpr int pair_tbit(int s, int i) {
+    return s == 0 ? (i < 3 ? i : i + 4) : s == 1 ? (i < 7 ? i : 11) : 3 + i;
+}
+__device__ __forceinline__ uint32_t pair_local(int s, uint32_t t, int r) {
+    uint32_t li = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) li |= (uint32_t)((r >> j) & 1) << pair_group(s, j);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) li |= ((t >> i) & 1u) << pair_tbit(s, i);
+    return li;
+}
+// LDS place of local index li: low bits XORed with bits 4..7, so that the 16
+// lanes of each quarter-wave hit 16 different 16-byte bank groups in all three
+// layouts
+__device__ __forceinline__ uint32_t pair_swz(uint32_t li) {
+    return li ^ ((li >> 4) & 7u) ^ (((li >> 7) & 1u) << 3);
+}
+
+template <class R, int S>
+__device__ __forceinline__ void pair_terms(const cx<R> (&p)[16], const cx<R> (&l)[16], const PairPass& P,
+                                           double (&acc)[PAIR_K]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int lb = pair_group(S, j);
+        if (P.col[lb] < 0) continue;
+        const bool y = P.type[lb] != 0;
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < 16; ++a) {
+            if ((a >> j) & 1) continue;
+            const int b = a | (1 << j);
+            const double lar = l[a].re, lai = l[a].im, lbr = l[b].re, lbi = l[b].im;
+            const double par = p[a].re, pai = p[a].im, pbr = p[b].re, pbi = p[b].im;
+            if (!y) v += (lar * pbi - lai * pbr) + (lbr * pai - lbi * par);
+            else v += -(lar * pbr + lai * pbi) + (lbr * par + lbi * pai);
+        }
+        acc[lb] += v;
+    }
+}
+
+template <class R>
+__device__ __forceinline__ void pair_exchange(cx<R> (&v)[16], cx<R>* lds, uint32_t t, int from) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const uint32_t li = from == 0 ? pair_local(0, t, r) : pair_local(1, t, r);
+        lds[pair_swz(li)] = v[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const uint32_t li = from == 0 ? pair_local(1, t, r) : pair_local(2, t, r);
+        v[r] = lds[pair_swz(li)];
+    }
+}
+
+template <class R>
+__global__ __launch_bounds__(PAIR_THREADS) void k_grad_pair(const cx<R>* __restrict__ psi,
+                                                            const cx<R>* __restrict__ lam, uint64_t ntiles,
+                                                            PairPass P, double* __restrict__ part) {
+    __shared__ cx<R> lds[1 << PAIR_K];
+    __shared__ double red[PAIR_THREADS / 64][PAIR_K];
+    const uint32_t t = threadIdx.x;
+    double acc[PAIR_K];
+#pragma unroll
+    for (int b = 0; b < PAIR_K; ++b) acc[b] = 0.0;
+    // stage-0 offsets: thread part and the 16 register combinations
+    uint64_t toff = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) toff |= (uint64_t)((t >> i) & 1u) << P.tq[pair_tbit(0, i)];
+    uint64_t roff[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) roff[j] = 1ULL << P.tq[pair_group(0, j)];
+    for (uint64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        uint64_t base = toff;
+        for (int e = 0; e < P.next; ++e) base |= ((ti >> e) & 1ULL) << P.epos[e];
+        cx<R> p[16], l[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            uint64_t g = base;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if ((r >> j) & 1) g |= roff[j];
+            p[r] = psi[g];
+            l[r] = lam[g];
+        }
+        pair_terms<R, 0>(p, l, P, acc);
+        if (P.nstages > 1) {
+            pair_exchange<R>(p, lds, t, 0);
+            pair_exchange<R>(l, lds, t, 0);
+            pair_terms<R, 1>(p, l, P, acc);
+        }
+        if (P.nstages > 2) {
+            pair_exchange<R>(p, lds, t, 1);
+            pair_exchange<R>(l, lds, t, 1);
+            pair_terms<R, 2>(p, l, P, acc);
+        }
+        __syncthreads();  // LDS reads of this tile before the next tile's writes
+    }
+#pragma unroll
+    for (int b = 0; b < PAIR_K; ++b) {
+        const double v = wave_sum(acc[b]);
+        if ((t & 63) == 0) red[t >> 6][b] = v;
+    }
+    __syncthreads();
+    if (t < PAIR_K && P.col[t] >= 0) {
+        double v = 0.0;
+        for (int w = 0; w < PAIR_THREADS / 64; ++w) v += red[w][t];
+        part[(uint64_t)blockIdx.x * PAIR_K + P.col[t]] = v;
+    }
+}
+
 namespace {
 
 struct Ws {
@@ -313,6 +452,113 @@ int grad_diag_async(qse_ctx* ctx, qse_state* psi, qse_state* lam, const std::vec
     return QSE_OK;
 }
 
+// Pair terms (qubit, type) of one evaluation point: one read-only pass over psi
+// and lambda per tile of 12 qubits (terms on qubits < 12 in the tile of qubits
+// 0..11; the others 9 at a time beside qubits 0..2). Values land on the device
+// at d_out[0 .. terms.size()) in the order returned in `order` (order[k] = the
+// term whose value is d_out[k]). Needs n >= 13.
+template <class R>
+int pair_terms_async(qse_ctx* ctx, const qse_state* psi, const qse_state* lam,
+                     const std::vector<std::pair<int, int>>& terms, double* d_out, std::vector<int>& order) {
+    const int n = psi->n;
+    std::vector<int> lo, hi;
+    for (size_t i = 0; i < terms.size(); ++i) (terms[i].first < PAIR_K ? lo : hi).push_back((int)i);
+    std::sort(hi.begin(), hi.end(), [&](int a, int b) { return terms[a].first < terms[b].first; });
+    std::vector<std::vector<int>> passes;
+    if (!lo.empty()) passes.push_back(lo);
+    for (size_t c = 0; c < hi.size(); c += PAIR_K - 3)
+        passes.emplace_back(hi.begin() + c, hi.begin() + std::min(hi.size(), c + (PAIR_K - 3)));
+    const uint64_t ntiles = 1ULL << (n - PAIR_K);
+    const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)ctx->num_cus * 2);
+    void* pbuf;
+    int rc = scratch(ctx, 6, sizeof(double) * PAIR_K * grid, &pbuf);
+    if (rc) return rc;
+    order.clear();
+    for (const std::vector<int>& pass : passes) {
+        PairPass P{};
+        for (int b = 0; b < PAIR_K; ++b) P.col[b] = -1;
+        uint64_t in_tile = 0;
+        const bool low = terms[pass[0]].first < PAIR_K;
+        if (low) {
+            for (int b = 0; b < PAIR_K; ++b) P.tq[b] = (uint32_t)b;
+        } else {
+            for (int b = 0; b < 3; ++b) P.tq[b] = (uint32_t)b;
+            for (size_t c = 0; c < pass.size(); ++c) P.tq[3 + c] = (uint32_t)terms[pass[c]].first;
+            for (int b = 0; b < 3 + (int)pass.size(); ++b) in_tile |= 1ULL << P.tq[b];
+            int fill = 3;  // pad with the lowest qubits outside the tile
+            for (int b = 3 + (int)pass.size(); b < PAIR_K; ++b) {
+                while ((in_tile >> fill) & 1) ++fill;
+                P.tq[b] = (uint32_t)fill;
+                in_tile |= 1ULL << fill;
+            }
+        }
+        in_tile = 0;
+        for (int b = 0; b < PAIR_K; ++b) in_tile |= 1ULL << P.tq[b];
+        P.next = 0;
+        for (int q = 0; q < n; ++q)
+            if (!((in_tile >> q) & 1)) P.epos[P.next++] = (uint32_t)q;
+        int last_stage = 0;
+        for (size_t c = 0; c < pass.size(); ++c) {
+            const int q = terms[pass[c]].first;
+            int lb = -1;
+            for (int b = 0; b < PAIR_K; ++b)
+                if ((int)P.tq[b] == q) lb = b;
+            P.col[lb] = (int)c;
+            P.type[lb] = terms[pass[c]].second;
+            last_stage = std::max(last_stage, (lb >= 3 && lb <= 6) ? 0 : (lb >= 7 && lb <= 10) ? 1 : 2);
+            order.push_back(pass[c]);
+        }
+        P.nstages = last_stage + 1;
+        TimedLaunch t;
+        launch_begin(ctx, FAM_ADJOINT, &t);
+        k_grad_pair<R><<<grid, PAIR_THREADS, 0, ctx->stream>>>((const cx<R>*)psi->d, (const cx<R>*)lam->d, ntiles, P,
+                                                               (double*)pbuf);
+        QSE_HIP(hipGetLastError());
+        launch_end(ctx, &t, 2.0 * amp_bytes(psi->prec) * (double)(1ULL << n));
+        k_sum_columns<<<(unsigned)pass.size(), AD_THREADS, 0, ctx->stream>>>((const double*)pbuf, (int)grid, PAIR_K,
+                                                                             d_out + (order.size() - pass.size()));
+        QSE_HIP(hipGetLastError());
+    }
+    return QSE_OK;
+}
+
+bool all_diagonal(const std::vector<qse_op>& ops) {
+    for (int k = 0; k < (int)ops.size(); ++k) {
+        if (is_zz_triple(ops.data(), (int)ops.size(), k)) {
+            k += 2;
+            continue;
+        }
+        switch (ops[k].kind) {
+            case QSE_GATE_IDENTITY: case QSE_GATE_Z: case QSE_GATE_PHASE: case QSE_GATE_RZ: case QSE_GATE_CPHASE:
+            case QSE_GATE_MCPHASE:
+                break;
+            default: return false;
+        }
+    }
+    return true;
+}
+
+// A segment with Z-string terms only whose un-application is diagonal can
+// un-apply before its evaluation (the same diagonal unitary on psi and lambda
+// leaves conj(lambda_i) psi_i unchanged): its gates join the previous
+// segment's un-application, which saves that program's own passes.
+void merge_diagonal_unapply(std::vector<AdjSegment>& segs) {
+    for (size_t g = 1; g < segs.size(); ++g) {
+        AdjSegment& s = segs[g];
+        if (!s.pair.empty() || s.unapply.empty() || !all_diagonal(s.unapply)) continue;
+        segs[g - 1].unapply.insert(segs[g - 1].unapply.end(), s.unapply.begin(), s.unapply.end());
+        s.unapply.clear();
+    }
+}
+
+// Distinct (qubit, type) pair terms of a segment, in first-seen order.
+std::vector<std::pair<int, int>> distinct_pairs(const AdjSegment& seg) {
+    std::vector<std::pair<int, int>> v;
+    for (const PairTerm& p : seg.pair)
+        if (std::find(v.begin(), v.end(), std::make_pair(p.q, p.type)) == v.end()) v.push_back({p.q, p.type});
+    return v;
+}
+
 // Plan ops in place with the tile shape tuned for their structure (tuned on
 // first sight, then cached in the context).
 int plan_tuned_shape(qse_ctx* ctx, int n, int prec, const std::vector<qse_op>& ops, const double* params,
@@ -432,6 +678,19 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
             if (rc) return rc;
             *next += (int)masks.size();
         }
+        if (!seg.pair.empty()) {
+            const std::vector<std::pair<int, int>> pt = distinct_pairs(seg);
+            std::vector<int> order;
+            rc = pair_terms_async<R>(ctx, ws.psi, ws.lam, pt, d_res + *next, order);
+            if (rc) return rc;
+            std::vector<int> slot_of(pt.size());
+            for (size_t k = 0; k < order.size(); ++k) slot_of[order[k]] = *next + (int)k;
+            for (const PairTerm& p : seg.pair) {
+                const size_t i = std::find(pt.begin(), pt.end(), std::make_pair(p.q, p.type)) - pt.begin();
+                slots.terms.push_back({slot_of[i], p.t});
+            }
+            *next += (int)pt.size();
+        }
         if (g + 1 < S && !seg.unapply.empty()) {
             if (ckpt_at.find(g + 1) == ckpt_at.end()) rc = run_on(ctx, {ws.psi, ws.lam}, seg.unapply, nullptr);
             else rc = run_on(ctx, {ws.lam}, seg.unapply, nullptr);
@@ -448,9 +707,36 @@ int values_per_set(const std::vector<AdjSegment>& segs) {
         std::vector<uint64_t> masks;
         for (const auto& dt : seg.diag)
             if (std::find(masks.begin(), masks.end(), dt.first) == masks.end()) masks.push_back(dt.first);
-        v += (int)masks.size();
+        v += (int)masks.size() + (int)distinct_pairs(seg).size();
     }
     return v;
+}
+
+// Passes over psi and lambda of a schedule's reverse sweep, in units of one
+// read and one write of a state: un-applications (twice, psi and lambda) and
+// the rotation into the first frame, plus the read-only reductions (Z-string
+// passes of up to 64 masks, pair passes of up to 9 or 12 qubits, each reading
+// both states).
+double sweep_cost(int n, const std::vector<AdjSegment>& segs) {
+    double c = 0.0;
+    auto passes = [&](const std::vector<qse_op>& ops) -> int {
+        if (ops.empty()) return 0;
+        HostPlan P;
+        if (choose_plan(n, PAIR_K, 3, ops.data(), (int)ops.size(), nullptr, P, false)) return 1 << 20;
+        return (int)P.passes.size();
+    };
+    if (!segs.empty()) c += 2.0 * passes(segs[0].pre);
+    for (const AdjSegment& seg : segs) {
+        c += 2.0 * passes(seg.unapply);
+        std::vector<uint64_t> masks;
+        for (const auto& dt : seg.diag)
+            if (std::find(masks.begin(), masks.end(), dt.first) == masks.end()) masks.push_back(dt.first);
+        c += (double)((masks.size() + 63) / 64);
+        int lo = 0, hi = 0;
+        for (const auto& p : distinct_pairs(seg)) (p.first < PAIR_K ? lo : hi) += 1;
+        c += (lo > 0 ? 1 : 0) + (hi + PAIR_K - 4) / (PAIR_K - 3);
+    }
+    return c;
 }
 
 int check_common(qse_ctx* ctx, int nqubits, int precision, const qse_op* ops, int nops, const double* params,
@@ -479,10 +765,29 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
     if (rc) return rc;
     if (!energies || (nparams > 0 && !grads)) return set_error(QSE_ERR_INVALID_ARG, "null output");
     // the schedule's shape does not depend on the parameter values
+    // X/Y generators: rotated into Z-strings, or reduced as amplitude pairs
+    // when that makes fewer passes (rotations that no un-application absorbs)
+    auto schedule = [&](const double* pb, bool pm, std::vector<AdjSegment>& sg, std::vector<qse_op>& pf) {
+        const int r = adjoint_segments(ops, nops, pb, sg, pf, pm);
+        if (r == QSE_OK && pm && ckpt_every <= 0) merge_diagonal_unapply(sg);
+        return r;
+    };
     std::vector<AdjSegment> segs0;
     std::vector<qse_op> prefix;
-    rc = adjoint_segments(ops, nops, params, segs0, prefix);
+    rc = schedule(params, false, segs0, prefix);
     if (rc) return rc;
+    bool pair_mode = false;
+    if (nqubits >= WHT_MIN_QUBITS && nqubits > PAIR_K && ctx->adjoint_terms != QSE_ADJ_TERMS_ROTATE) {
+        std::vector<AdjSegment> segsP;
+        std::vector<qse_op> prefixP;
+        rc = schedule(params, true, segsP, prefixP);
+        if (rc) return rc;
+        if (ctx->adjoint_terms == QSE_ADJ_TERMS_PAIRS || sweep_cost(nqubits, segsP) < sweep_cost(nqubits, segs0)) {
+            pair_mode = true;
+            segs0.swap(segsP);
+            prefix.swap(prefixP);
+        }
+    }
     Ws ws;
     ws.ctx = ctx;
     rc = qse_state_create(ctx, nqubits, precision, &ws.psi);
@@ -518,7 +823,7 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
     for (int b = 0; b < batch; ++b) {
         const double* pb = params ? params + (size_t)b * nparams : nullptr;
         std::vector<AdjSegment> segs;
-        rc = adjoint_segments(ops, nops, pb, segs, prefix);
+        rc = schedule(pb, pair_mode, segs, prefix);
         if (rc) return rc;
         if (values_per_set(segs) != vps)
             return set_error(QSE_ERR_UNSUPPORTED, "adjoint: schedule changed between parameter sets");
@@ -542,6 +847,13 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
         std::fill(gb, gb + nparams, 0.0);
         for (const auto& st : slots[b].terms) gb[st.second.param] += st.second.coeff * vals[st.first];
     }
+    return QSE_OK;
+}
+
+int qse_ctx_set_adjoint_terms(qse_ctx* ctx, int mode) {
+    if (!ctx || mode < QSE_ADJ_TERMS_AUTO || mode > QSE_ADJ_TERMS_PAIRS)
+        return set_error(QSE_ERR_INVALID_ARG, "bad adjoint term mode");
+    ctx->adjoint_terms = mode;
     return QSE_OK;
 }
 

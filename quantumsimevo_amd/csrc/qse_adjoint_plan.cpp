@@ -167,7 +167,7 @@ qse_op gate1(int kind, int q, double theta) {
 }  // namespace
 
 int adjoint_segments(const qse_op* ops, int nops, const double* params, std::vector<AdjSegment>& segs,
-                     std::vector<qse_op>& prefix) {
+                     std::vector<qse_op>& prefix, bool pair_mode) {
     std::vector<AdjItem> items;
     const int rc = build_items(ops, nops, items);
     if (rc) return rc;
@@ -223,12 +223,17 @@ int adjoint_segments(const qse_op* ops, int nops, const double* params, std::vec
         segs.back().forward.clear();
         segs.back().unapply.clear();
     }
-    // X/Y generators -> Z-strings in the frame rotated by V (H for X, H S^dag for Y)
+    // X/Y generators -> Z-strings in the frame rotated by V (H for X, H S^dag for Y),
+    // or, in pair mode, kept as pair terms of the unrotated state
     const double half_pi = 1.5707963267948966192;
     std::vector<std::vector<qse_op>> vdag(segs.size());
     for (size_t g = 0; g < segs.size(); ++g) {
         uint64_t done = 0;
         for (const Pair& p : pairs[g]) {
+            if (pair_mode) {
+                segs[g].pair.push_back({p.q, p.type, p.t});
+                continue;
+            }
             segs[g].diag.push_back({1ULL << p.q, p.t});
             if ((done >> p.q) & 1) continue;  // one rotation per qubit (same-qubit terms share a type)
             done |= 1ULL << p.q;

@@ -34,8 +34,16 @@ struct AdjItem {
     GradTerm pair{-1, 0.0};
 };
 
+// An X (type 0) or Y (type 1) generator term on qubit q, evaluated directly
+// as Im<lambda|X_q|psi> or Im<lambda|Y_q|psi> (sums over amplitude pairs).
+struct PairTerm {
+    int q, type;
+    GradTerm t;
+};
+
 struct AdjSegment {
     std::vector<std::pair<uint64_t, GradTerm>> diag;  // Z-string terms at this point (rotated frame)
+    std::vector<PairTerm> pair;   // X/Y terms left unrotated (pair mode)
     std::vector<qse_op> pre;      // V_s: rotation into this point's frame (from the unrotated state)
     std::vector<qse_op> unapply;  // V_s^dag, the segment's daggers, V_{s+1}, in application order
     std::vector<qse_op> forward;  // the segment's gates, forward order
@@ -51,7 +59,9 @@ qse_op dagger(const qse_op& o, const double* params);
 // Segment 0's rotation (segs[0].pre) must be applied to psi and lambda once
 // lambda = O psi is formed; a psi restored from a checkpoint at segment s needs
 // segs[s].pre.
+// With pair_mode, X/Y generators stay in seg.pair and no rotation is made
+// (pre and the rotations in unapply are empty of them).
 int adjoint_segments(const qse_op* ops, int nops, const double* params, std::vector<AdjSegment>& segs,
-                     std::vector<qse_op>& prefix);
+                     std::vector<qse_op>& prefix, bool pair_mode = false);
 
 }  // namespace qse

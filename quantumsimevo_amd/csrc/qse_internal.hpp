@@ -147,6 +147,8 @@ struct qse_ctx {
     // Pinned host staging for small readbacks.
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
+    // How the adjoint evaluates X/Y generator terms (QSE_ADJ_TERMS_*).
+    int adjoint_terms = 0;
     // Renormalise states after this many passes of qse_run calls (0: never).
     int64_t renorm_every = 0;
     // Fusion shape per precision (tile qubits k, contiguous low qubits L).

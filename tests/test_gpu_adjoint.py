@@ -193,3 +193,60 @@ def test_structure_keyed_kernels():
             assert np.max(np.abs(E - E0)) < 1e-11 and np.max(np.abs(Gr - G0)) < tol
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("n", [14, 16])
+def test_pair_terms_vs_rotated_frames_and_oracle(ctx, n):
+    """RX/RY generator terms summed over amplitude pairs (read-only passes over
+    psi and lambda, terms on qubits below and above the 12-qubit tile) against
+    the rotated-frame Z-strings and the oracle's parameter shift; complex64 and
+    a checkpointed sweep in pair mode."""
+    ops, npar = mixed_circuit(n, 7)
+    zm, zw = G.zsum_observable(n)
+    params = np.random.default_rng(n).random(npar) * 2 * math.pi
+    res = {}
+    try:
+        for mode in (capi.ADJ_TERMS_ROTATE, capi.ADJ_TERMS_PAIRS):
+            ctx.set_adjoint_terms(mode)
+            res[mode] = ctx.adjoint_expect(n, capi.C128, ops, params, zm, zw)
+        ctx.set_adjoint_terms(capi.ADJ_TERMS_PAIRS)
+        Ec, Gc = ctx.adjoint_expect(n, capi.C128, ops, params, zm, zw, ckpt_every=2)
+        E32, G32 = ctx.adjoint_expect(n, capi.C64, ops, params, zm, zw)
+    finally:
+        ctx.set_adjoint_terms(capi.ADJ_TERMS_AUTO)
+    (Er, Gr), (Ep, Gp) = res[capi.ADJ_TERMS_ROTATE], res[capi.ADJ_TERMS_PAIRS]
+    gs = oracle_shift(n, ops, params, zm, zw)
+    assert abs(Ep[0] - oracle_energy(n, ops, params, zm, zw)) < 1e-11
+    assert np.max(np.abs(Gp[0] - gs)) < 1e-10
+    assert np.max(np.abs(Gp - Gr)) < 1e-11
+    assert abs(Ec[0] - Ep[0]) < 1e-12 and np.max(np.abs(Gc - Gp)) < 1e-11
+    assert np.max(np.abs(G32 - Gp)) < 1e-3
+
+
+def test_pair_terms_two_high_tiles_qaoa22(ctx):
+    """QAOA on 22 qubits: the mixer's X terms need the low tile and two tiles of
+    high qubits (9 + 1). Pair mode (also the automatic choice here) against the
+    rotated-frame schedule and central finite differences on the GPU."""
+    n, p = 22, 1
+    edges = G.random_regular_graph(n, 3, seed=3)
+    ops = G.qaoa_maxcut(n, edges, p).ops()
+    masks, w = G.maxcut_observable(edges)
+    params = np.array([[0.41, 1.07]])
+    try:
+        ctx.set_adjoint_terms(capi.ADJ_TERMS_ROTATE)
+        Er, Gr = ctx.adjoint_expect(n, capi.C128, ops, params, masks, w)
+        ctx.set_adjoint_terms(capi.ADJ_TERMS_PAIRS)
+        Ep, Gp = ctx.adjoint_expect(n, capi.C128, ops, params, masks, w)
+        ctx.set_adjoint_terms(capi.ADJ_TERMS_AUTO)
+        Ea, Ga = ctx.adjoint_expect(n, capi.C128, ops, params, masks, w)
+    finally:
+        ctx.set_adjoint_terms(capi.ADJ_TERMS_AUTO)
+    assert np.max(np.abs(Gp - Gr)) < 1e-10 and abs(Ep[0] - Er[0]) < 1e-11
+    assert np.array_equal(Ga, Gp)
+    h = 1e-5
+    for j in range(2):
+        Pp, Pm = params.copy(), params.copy()
+        Pp[0, j] += h
+        Pm[0, j] -= h
+        fd = (ctx.batch_expect(n, capi.C128, ops, Pp, masks, w)[0] - ctx.batch_expect(n, capi.C128, ops, Pm, masks, w)[0]) / (2 * h)
+        assert abs(Gp[0, j] - fd) < 1e-6
