@@ -205,6 +205,14 @@ __global__ __launch_bounds__(AD_THREADS) void k_sum_columns(const double* __rest
 // tile index bits map to global qubits through the pass description.
 constexpr int PAIR_K = 12;
 constexpr int PAIR_THREADS = 256;
+// Z-string terms Im<lambda|Z_m|psi> = sum_x (-1)^popcount(x & m) Im(conj(l_x) p_x)
+// evaluated in the same pass, up to PAIR_Z per pass: in the last stage each
+// thread takes the 16-point Walsh-Hadamard transform W of its 16 values w_r
+// over the register qubits, and the term gets s_tile(m) s_thread(m) W[u_m]
+// (parities of m on the tile-index and thread qubits, u_m = m on the register
+// qubits).
+constexpr int PAIR_Z = 12;
+constexpr int PAIR_COLS = PAIR_K + PAIR_Z;
 
 struct PairPass {
     uint32_t tq[PAIR_K];     // local bit -> global qubit (tq[0..2] = 0, 1, 2)
@@ -213,6 +221,10 @@ struct PairPass {
     int32_t nstages;         // register groups to visit (terms beyond are absent)
     int32_t col[PAIR_K];     // output column of the term on each local bit, -1: none
     int32_t type[PAIR_K];    // 0: X, 1: Y
+    int32_t nz;              // Z-string terms of this pass (<= PAIR_Z), columns zcol0 + z
+    int32_t zcol0;
+    uint32_t zu[PAIR_Z];     // mask bits on the last stage's register qubits (4-bit pattern)
+    uint64_t zm[PAIR_Z];     // the masks (global qubits)
 };
 
 __device__ __forceinline__ constexpr int pair_group(int s, int j) {
@@ -276,15 +288,19 @@ __device__ __forceinline__ void pair_exchange(cx<R> (&v)[16], cx<R>* lds, uint32
 }
 
 template <class R>
-__global__ __launch_bounds__(PAIR_THREADS) void k_grad_pair(const cx<R>* __restrict__ psi,
+__global__ __launch_bounds__(PAIR_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_grad_pair(const cx<R>* __restrict__ psi,
                                                             const cx<R>* __restrict__ lam, uint64_t ntiles,
                                                             PairPass P, double* __restrict__ part) {
     __shared__ cx<R> lds[1 << PAIR_K];
-    __shared__ double red[PAIR_THREADS / 64][PAIR_K];
+    __shared__ double red[PAIR_THREADS / 64][PAIR_COLS];
+    double* hw = reinterpret_cast<double*>(lds);  // W[u] of thread t at hw[u * 256 + t] (stage 0)
+    static_assert(sizeof(cx<R>) * (1 << PAIR_K) >= sizeof(double) * 16 * PAIR_THREADS, "LDS too small for W");
     const uint32_t t = threadIdx.x;
-    double acc[PAIR_K];
+    double acc[PAIR_K], zacc[PAIR_Z];
 #pragma unroll
     for (int b = 0; b < PAIR_K; ++b) acc[b] = 0.0;
+#pragma unroll
+    for (int z = 0; z < PAIR_Z; ++z) zacc[z] = 0.0;
     // stage-0 offsets: thread part and the 16 register combinations
     uint64_t toff = 0;
 #pragma unroll
@@ -292,9 +308,18 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_grad_pair(const cx<R>* __restr
     uint64_t roff[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) roff[j] = 1ULL << P.tq[pair_group(0, j)];
+    // bit z: parity of mask z on the thread's qubits in the last stage's layout
+    const int zs = P.nstages - 1;
+    uint64_t toffz = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        toffz |= (uint64_t)((t >> i) & 1u) << P.tq[zs == 0 ? pair_tbit(0, i) : zs == 1 ? pair_tbit(1, i) : pair_tbit(2, i)];
+    uint32_t sthr = 0;
+    for (int z = 0; z < P.nz; ++z) sthr |= (uint32_t)(__popcll(P.zm[z] & toffz) & 1) << z;
     for (uint64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
-        uint64_t base = toff;
-        for (int e = 0; e < P.next; ++e) base |= ((ti >> e) & 1ULL) << P.epos[e];
+        uint64_t ebase = 0;
+        for (int e = 0; e < P.next; ++e) ebase |= ((ti >> e) & 1ULL) << P.epos[e];
+        const uint64_t base = ebase | toff;
         cx<R> p[16], l[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
@@ -316,6 +341,32 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_grad_pair(const cx<R>* __restr
             pair_exchange<R>(l, lds, t, 1);
             pair_terms<R, 2>(p, l, P, acc);
         }
+        if (P.nz > 0) {
+            // in the last stage's layout, after which p and l are no longer needed
+            double w[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) w[r] = (double)l[r].re * (double)p[r].im - (double)l[r].im * (double)p[r].re;
+#pragma unroll
+            for (int h = 1; h < 16; h <<= 1)
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    if (!(r & h)) {
+                        const double a = w[r], b = w[r | h];
+                        w[r] = a + b;
+                        w[r | h] = a - b;
+                    }
+            __syncthreads();  // every thread's exchange reads before W overwrites the LDS
+#pragma unroll
+            for (int u = 0; u < 16; ++u) hw[u * PAIR_THREADS + t] = w[u];
+            // each thread reads back only its own column: no barrier
+#pragma unroll
+            for (int z = 0; z < PAIR_Z; ++z) {
+                if (z >= P.nz) break;
+                const double v = hw[P.zu[z] * PAIR_THREADS + t];
+                const bool neg = (((sthr >> z) ^ (uint32_t)__popcll(P.zm[z] & ebase)) & 1) != 0;
+                zacc[z] += neg ? -v : v;
+            }
+        }
         __syncthreads();  // LDS reads of this tile before the next tile's writes
     }
 #pragma unroll
@@ -323,11 +374,19 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_grad_pair(const cx<R>* __restr
         const double v = wave_sum(acc[b]);
         if ((t & 63) == 0) red[t >> 6][b] = v;
     }
+#pragma unroll
+    for (int z = 0; z < PAIR_Z; ++z) {
+        const double v = wave_sum(zacc[z]);
+        if ((t & 63) == 0) red[t >> 6][PAIR_K + z] = v;
+    }
     __syncthreads();
-    if (t < PAIR_K && P.col[t] >= 0) {
+    int c = -1;
+    if (t < PAIR_K) c = P.col[t];
+    else if (t < PAIR_K + (uint32_t)P.nz) c = P.zcol0 + (int)(t - PAIR_K);
+    if (t < PAIR_COLS && c >= 0) {
         double v = 0.0;
         for (int w = 0; w < PAIR_THREADS / 64; ++w) v += red[w][t];
-        part[(uint64_t)blockIdx.x * PAIR_K + P.col[t]] = v;
+        part[(uint64_t)blockIdx.x * PAIR_COLS + c] = v;
     }
 }
 
@@ -452,14 +511,23 @@ int grad_diag_async(qse_ctx* ctx, qse_state* psi, qse_state* lam, const std::vec
     return QSE_OK;
 }
 
-// Pair terms (qubit, type) of one evaluation point: one read-only pass over psi
-// and lambda per tile of 12 qubits (terms on qubits < 12 in the tile of qubits
-// 0..11; the others 9 at a time beside qubits 0..2). Values land on the device
-// at d_out[0 .. terms.size()) in the order returned in `order` (order[k] = the
-// term whose value is d_out[k]). Needs n >= 13.
+// Pair terms (qubit, type) of one evaluation point, and up to PAIR_Z Z-string
+// masks per pass alongside: one read-only pass over psi and lambda per tile
+// of 12 qubits (terms on qubits < 12 in the tile of qubits 0..11; the others
+// 9 at a time beside qubits 0..2). Values land on the device at
+// d_out[0 .. terms.size() + zmasks.size()) in the order returned in `order`
+// (order[k] = i for pair term i, -1 - m for mask m). Needs n >= 13 and
+// zmasks.size() <= PAIR_Z x the number of passes (pair_passes()).
+int pair_passes(const std::vector<std::pair<int, int>>& terms) {
+    int lo = 0, hi = 0;
+    for (const auto& t : terms) (t.first < PAIR_K ? lo : hi) += 1;
+    return (lo > 0 ? 1 : 0) + (hi + PAIR_K - 4) / (PAIR_K - 3);
+}
+
 template <class R>
 int pair_terms_async(qse_ctx* ctx, const qse_state* psi, const qse_state* lam,
-                     const std::vector<std::pair<int, int>>& terms, double* d_out, std::vector<int>& order) {
+                     const std::vector<std::pair<int, int>>& terms, const std::vector<uint64_t>& zmasks,
+                     double* d_out, std::vector<int>& order) {
     const int n = psi->n;
     std::vector<int> lo, hi;
     for (size_t i = 0; i < terms.size(); ++i) (terms[i].first < PAIR_K ? lo : hi).push_back((int)i);
@@ -468,13 +536,16 @@ int pair_terms_async(qse_ctx* ctx, const qse_state* psi, const qse_state* lam,
     if (!lo.empty()) passes.push_back(lo);
     for (size_t c = 0; c < hi.size(); c += PAIR_K - 3)
         passes.emplace_back(hi.begin() + c, hi.begin() + std::min(hi.size(), c + (PAIR_K - 3)));
+    if (zmasks.size() > PAIR_Z * passes.size()) return set_error(QSE_ERR_LIMIT, "adjoint: too many Z-strings for the pair passes");
     const uint64_t ntiles = 1ULL << (n - PAIR_K);
     const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)ctx->num_cus * 2);
     void* pbuf;
-    int rc = scratch(ctx, 6, sizeof(double) * PAIR_K * grid, &pbuf);
+    int rc = scratch(ctx, 6, sizeof(double) * PAIR_COLS * grid, &pbuf);
     if (rc) return rc;
     order.clear();
-    for (const std::vector<int>& pass : passes) {
+    size_t zdone = 0;
+    for (size_t pi = 0; pi < passes.size(); ++pi) {
+        const std::vector<int>& pass = passes[pi];
         PairPass P{};
         for (int b = 0; b < PAIR_K; ++b) P.col[b] = -1;
         uint64_t in_tile = 0;
@@ -497,6 +568,7 @@ int pair_terms_async(qse_ctx* ctx, const qse_state* psi, const qse_state* lam,
         P.next = 0;
         for (int q = 0; q < n; ++q)
             if (!((in_tile >> q) & 1)) P.epos[P.next++] = (uint32_t)q;
+        const size_t first = order.size();
         int last_stage = 0;
         for (size_t c = 0; c < pass.size(); ++c) {
             const int q = terms[pass[c]].first;
@@ -508,6 +580,22 @@ int pair_terms_async(qse_ctx* ctx, const qse_state* psi, const qse_state* lam,
             last_stage = std::max(last_stage, (lb >= 3 && lb <= 6) ? 0 : (lb >= 7 && lb <= 10) ? 1 : 2);
             order.push_back(pass[c]);
         }
+        // Z-strings: the first passes take PAIR_Z each, in their last stage
+        if (zdone < zmasks.size()) last_stage = std::max(last_stage, 0);
+        P.zcol0 = (int)pass.size();
+        P.nz = (int)std::min<size_t>(PAIR_Z, zmasks.size() - zdone);
+        for (int z = 0; z < P.nz; ++z) {
+            const uint64_t m = zmasks[zdone + z];
+            P.zm[z] = m;
+            uint32_t u = 0;
+            for (int j = 0; j < 4; ++j) {
+                const int lb = last_stage == 0 ? 3 + j : last_stage == 1 ? 7 + j : (j < 3 ? j : 11);
+                u |= (uint32_t)((m >> P.tq[lb]) & 1) << j;
+            }
+            P.zu[z] = u;
+            order.push_back(-1 - (int)(zdone + z));
+        }
+        zdone += (size_t)P.nz;
         P.nstages = last_stage + 1;
         TimedLaunch t;
         launch_begin(ctx, FAM_ADJOINT, &t);
@@ -515,8 +603,8 @@ int pair_terms_async(qse_ctx* ctx, const qse_state* psi, const qse_state* lam,
                                                                (double*)pbuf);
         QSE_HIP(hipGetLastError());
         launch_end(ctx, &t, 2.0 * amp_bytes(psi->prec) * (double)(1ULL << n));
-        k_sum_columns<<<(unsigned)pass.size(), AD_THREADS, 0, ctx->stream>>>((const double*)pbuf, (int)grid, PAIR_K,
-                                                                             d_out + (order.size() - pass.size()));
+        k_sum_columns<<<(unsigned)(order.size() - first), AD_THREADS, 0, ctx->stream>>>(
+            (const double*)pbuf, (int)grid, PAIR_COLS, d_out + first);
         QSE_HIP(hipGetLastError());
     }
     return QSE_OK;
@@ -653,6 +741,15 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
         rc = run_on(ctx, {ws.psi, ws.lam}, segs[0].pre, nullptr);
         if (rc) return rc;
     }
+    // Z-string masks of the current evaluation point and the terms each feeds;
+    // a segment that un-applies nothing shares its point with the next one,
+    // whose pair passes then take the masks along
+    std::vector<uint64_t> masks;
+    std::vector<std::vector<GradTerm>> mterms;
+    auto land = [&](const std::vector<int>& which_masks, int first_slot) {
+        for (size_t k = 0; k < which_masks.size(); ++k)
+            for (const GradTerm& gt : mterms[which_masks[k]]) slots.terms.push_back({first_slot + (int)k, gt});
+    };
     for (int g = 0; g < S; ++g) {
         const AdjSegment& seg = segs[g];
         auto ck = ckpt_at.find(g);
@@ -662,35 +759,47 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
             rc = run_on(ctx, {ws.psi}, seg.pre, nullptr);
             if (rc) return rc;
         }
-        // generator terms at this point: one pass for up to 16 Z-strings
-        std::vector<uint64_t> masks;
-        std::vector<int> mslot;
         for (const auto& dt : seg.diag) {
             const size_t pos = std::find(masks.begin(), masks.end(), dt.first) - masks.begin();
             if (pos == masks.size()) {
                 masks.push_back(dt.first);
-                mslot.push_back(*next + (int)pos);
+                mterms.emplace_back();
             }
-            slots.terms.push_back({mslot[pos], dt.second});
+            mterms[pos].push_back(dt.second);
         }
-        if (!masks.empty()) {
-            rc = grad_diag_async<R>(ctx, ws.psi, ws.lam, masks, d_res + *next);
+        if (g + 1 < S && seg.unapply.empty() && seg.pair.empty() && !segs[g + 1].pair.empty() &&
+            ckpt_at.find(g + 1) == ckpt_at.end())
+            continue;
+        const std::vector<std::pair<int, int>> pt = distinct_pairs(seg);
+        const size_t nfused = pt.empty() ? 0 : std::min(masks.size(), (size_t)PAIR_Z * pair_passes(pt));
+        if (masks.size() > nfused) {
+            // Z-strings beyond what the pair passes take: their own reduction passes
+            const std::vector<uint64_t> rest(masks.begin() + nfused, masks.end());
+            rc = grad_diag_async<R>(ctx, ws.psi, ws.lam, rest, d_res + *next);
             if (rc) return rc;
-            *next += (int)masks.size();
+            std::vector<int> idx;
+            for (size_t k = nfused; k < masks.size(); ++k) idx.push_back((int)k);
+            land(idx, *next);
+            *next += (int)rest.size();
         }
-        if (!seg.pair.empty()) {
-            const std::vector<std::pair<int, int>> pt = distinct_pairs(seg);
+        if (!pt.empty()) {
+            const std::vector<uint64_t> zf(masks.begin(), masks.begin() + nfused);
             std::vector<int> order;
-            rc = pair_terms_async<R>(ctx, ws.psi, ws.lam, pt, d_res + *next, order);
+            rc = pair_terms_async<R>(ctx, ws.psi, ws.lam, pt, zf, d_res + *next, order);
             if (rc) return rc;
             std::vector<int> slot_of(pt.size());
-            for (size_t k = 0; k < order.size(); ++k) slot_of[order[k]] = *next + (int)k;
+            for (size_t k = 0; k < order.size(); ++k) {
+                if (order[k] >= 0) slot_of[order[k]] = *next + (int)k;
+                else land({-1 - order[k]}, *next + (int)k);
+            }
             for (const PairTerm& p : seg.pair) {
                 const size_t i = std::find(pt.begin(), pt.end(), std::make_pair(p.q, p.type)) - pt.begin();
                 slots.terms.push_back({slot_of[i], p.t});
             }
-            *next += (int)pt.size();
+            *next += (int)order.size();
         }
+        masks.clear();
+        mterms.clear();
         if (g + 1 < S && !seg.unapply.empty()) {
             if (ckpt_at.find(g + 1) == ckpt_at.end()) rc = run_on(ctx, {ws.psi, ws.lam}, seg.unapply, nullptr);
             else rc = run_on(ctx, {ws.lam}, seg.unapply, nullptr);
@@ -732,9 +841,7 @@ double sweep_cost(int n, const std::vector<AdjSegment>& segs) {
         for (const auto& dt : seg.diag)
             if (std::find(masks.begin(), masks.end(), dt.first) == masks.end()) masks.push_back(dt.first);
         c += (double)((masks.size() + 63) / 64);
-        int lo = 0, hi = 0;
-        for (const auto& p : distinct_pairs(seg)) (p.first < PAIR_K ? lo : hi) += 1;
-        c += (lo > 0 ? 1 : 0) + (hi + PAIR_K - 4) / (PAIR_K - 3);
+        c += pair_passes(distinct_pairs(seg));
     }
     return c;
 }

@@ -250,3 +250,30 @@ def test_pair_terms_two_high_tiles_qaoa22(ctx):
         Pm[0, j] -= h
         fd = (ctx.batch_expect(n, capi.C128, ops, Pp, masks, w)[0] - ctx.batch_expect(n, capi.C128, ops, Pm, masks, w)[0]) / (2 * h)
         assert abs(Gp[0, j] - fd) < 1e-6
+
+
+@pytest.mark.parametrize("n, complete", [(14, True), (22, False)])
+def test_zstrings_carried_by_pair_passes(ctx, n, complete):
+    """QAOA p = 2: the cost layer's Z_a Z_b terms share their evaluation point
+    with the next mixer's X terms and ride along in its pair passes (12 per
+    pass); on K_14 the 91 masks exceed what two pair passes take and the rest
+    go through the Walsh-Hadamard reduction. Pair mode against the rotated
+    frames, and K_14 against the oracle's parameter shift."""
+    edges = ([(a, b) for a in range(n) for b in range(a + 1, n)] if complete
+             else G.random_regular_graph(n, 3, seed=5))
+    ops = G.qaoa_maxcut(n, edges, 2).ops()
+    masks, w = G.maxcut_observable(edges)
+    params = np.array([[0.21, 0.93, 0.47, 1.31]])
+    try:
+        ctx.set_adjoint_terms(capi.ADJ_TERMS_ROTATE)
+        Er, Gr = ctx.adjoint_expect(n, capi.C128, ops, params, masks, w)
+        ctx.set_adjoint_terms(capi.ADJ_TERMS_PAIRS)
+        Ep, Gp = ctx.adjoint_expect(n, capi.C128, ops, params, masks, w)
+    finally:
+        ctx.set_adjoint_terms(capi.ADJ_TERMS_AUTO)
+    scale = max(1.0, float(np.max(np.abs(Gr))))
+    assert abs(Ep[0] - Er[0]) < 1e-11 * max(1.0, abs(Er[0]))
+    assert np.max(np.abs(Gp - Gr)) < 1e-11 * scale
+    if complete:
+        gs = oracle_shift(n, ops, params[0], masks, w)
+        assert np.max(np.abs(Gp[0] - gs)) < 1e-10 * scale
