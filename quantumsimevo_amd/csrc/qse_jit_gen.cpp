@@ -127,6 +127,10 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
         o << "  }\n";
     }
     o << "  for (unsigned long long ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {\n";
+    // the thread id as a value the compiler cannot carry across iterations:
+    // thread-dependent phase factors are recomputed per tile instead of being
+    // hoisted out of the loop, where they would hold registers for its length
+    o << "    unsigned tidv = tid;\n    QSE_OPAQUE(tidv);\n";
     o << "    unsigned long long base = 0, obase = 0;\n";
     for (int e = 0; e < fp.next; ++e)
         o << "    base |= ((ti >> " << e << ") & 1ull) << " << fp.epos[e] << "; obase |= ((ti >> " << e
@@ -154,7 +158,7 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
     for (int sg = 0; sg < fp.nstages; ++sg) {
         const FStage& S = P.stages[fp.stage_begin + sg];
         o << "    {\n      unsigned toff = 0;\n";
-        for (int t = 0; t < TB; ++t) o << "      if (tid & " << (1u << t) << "u) toff |= " << (1u << S.tmap[t]) << "u;\n";
+        for (int t = 0; t < TB; ++t) o << "      if (tidv & " << (1u << t) << "u) toff |= " << (1u << S.tmap[t]) << "u;\n";
         o << "      const unsigned so = swz(toff);\n      R";
         for (int v = 0; v < 16; ++v) o << (v ? ", " : " ") << "ar" << v << ", ai" << v;
         o << ";\n";
@@ -175,7 +179,7 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
                 o << "      {\n      double gr = " << opv(oi, 0) << ", gi = " << opv(oi, 1) << ";\n";
                 for (int t = 0; t < TB; ++t)
                     if ((op.tctl >> t) & 1) {
-                        o << "      if (tid & " << (1u << t) << "u)\n";
+                        o << "      if (tidv & " << (1u << t) << "u)\n";
                         emit_mul_g(o, alphav(op.alpha_begin + 2 * t), alphav(op.alpha_begin + 2 * t + 1));
                     }
                 for (uint32_t ti = op.term_begin; ti < op.term_end; ++ti) {
@@ -319,7 +323,8 @@ std::string jit_source(const HostPlan& P, int prec, const size_t* table_off) {
          "r.im = a.re * b.im + a.im * b.re; return r; }\n"
          "__device__ __forceinline__ cx cadd(cx a, cx b) { cx r; r.re = a.re + b.re; r.im = a.im + b.im; return r; }\n"
          "__device__ __forceinline__ unsigned swz(unsigned li) { return li ^ (((li >> 4) ^ (li >> 8) ^ (li >> 12)) & "
-         "15u); }\n";
+         "15u); }\n"
+         "#ifndef QSE_OPAQUE\n#define QSE_OPAQUE(v) __asm__ volatile(\"\" : \"+v\"(v))\n#endif\n";
     // complex128 state loads and stores are non-temporal (streamed past the caches): every amplitude is
     // read and written once per pass, and the hint made 30-qubit QFT passes 2-5% faster; for complex64
     // (whose passes prefetch the next tile) it made them 4% slower, so they stay plain. One vector access

@@ -23,5 +23,8 @@ void shim_syncthreads();
 // Non-temporal state accesses become plain ones; the two-component vector type
 // of the generated code becomes a GCC vector (same [i] subscripts).
 #define ext_vector_type(n) vector_size((n) * sizeof(R))
+// The generated code's register barrier (keeps the thread id from being
+// treated as loop-invariant) in its host form.
+#define QSE_OPAQUE(v) __asm__ volatile("" : "+r"(v))
 #define __builtin_nontemporal_load(p) (*(p))
 #define __builtin_nontemporal_store(v, p) (*(p) = (v))
