@@ -187,6 +187,40 @@ def uneven_batch_leg(ctx) -> dict:
             "max_abs_grad_err_vs_fp64": float(max(np.max(np.abs(a - b)) for a, b in zip(G64, Gr)))}
 
 
+def layered_leg(ctx, n: int = 30, layers: int = 20) -> dict:
+    """The survey's plain-method circuit at 30 qubits complex128: per layer one
+    of H/RX/RY/RZ/phase per qubit (angles uniform in [0, 2pi)), then a CNOT
+    brickwork; executed as fused passes on a resident seeded random state.
+    Per-pass HBM rate from the library's HIP events (2 x 16 x 2^n bytes per
+    pass), mean of 3 runs after 2 warm-up runs (the second compiles)."""
+    ops = C.random_layered(n, layers, seed=7).ops()
+    st = ctx.state(n, capi.C128)
+    st.set_random(11)
+    info = ctx.plan_info(n, capi.C128, ops)
+    for _ in range(2):
+        st.run(ops)
+    ctx.sync()
+    ctx.reset_timing()
+    ctx.enable_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        st.run(ops)
+    ctx.sync()
+    wall = (time.perf_counter() - t0) / 3
+    ctx.enable_timing(False)
+    f = ctx.timing(capi.FAM_FUSED)
+    norm_err = abs(st.norm2() - 1.0)
+    st.close()
+    pass_ms = f["ms"] / max(f["launches"], 1)
+    gbs = 2.0 * 16 * 2.0 ** n / (pass_ms * 1e-3) / 1e9
+    log(f"layered {n}q x {layers}: {wall * 1e3:.1f} ms per run, {f['launches'] // 3} passes, {pass_ms:.3f} ms each "
+        f"= {gbs:.0f} GB/s")
+    return {"workload": f"random layered circuit, {n} qubits complex128, {layers} layers ({len(ops)} gates)",
+            "ms_per_run": wall * 1e3, "passes_per_run": f["launches"] // 3, "tile_qubits": info["tile_bits"],
+            "contiguous_low_qubits": info["low_bits"], "pass_mean_ms": pass_ms, "pass_gbs": gbs,
+            "pass_frac_of_peak": gbs / 8000.0, "norm_err_after_3_runs": norm_err}
+
+
 def dense_block_leg(ctx, restore_jit: int, n: int = 30) -> dict:
     """Matrix-core dense blocks against the fused gate path: a random layered
     circuit of `layers` layers on qubits 0..m-1 (one of H/RX/RY/RZ/phase per
@@ -392,6 +426,10 @@ def main() -> int:
                                          "seeded random state"}
     st.close()
 
+    if not args.no_extra and n == 30 and prec == capi.C128:
+        result["layered30"] = layered_leg(ctx)
+        ceiling = result["roofline"]["copy_ceiling_gbs"]["runs_of_the_plan"]
+        result["layered30"]["pass_frac_of_copy_ceiling"] = result["layered30"]["pass_gbs"] / ceiling
     if not args.no_dense:
         result["dense_block"] = dense_block_leg(
             ctx, {"auto": capi.JIT_AUTO, "off": capi.JIT_OFF, "always": capi.JIT_ALWAYS}[args.jit])

@@ -78,3 +78,44 @@ def test_qft33_complex128_full_size():
 
 def test_qft34_complex64_full_size():
     full_size(34, capi.C64, 1e-4)
+
+
+def inverse_ops(ops):
+    """The gate list of U^dagger: reversed order, angles negated, 2x2 matrices
+    conjugate-transposed (H, X, Y, Z, CNOT and SWAP are their own inverses)."""
+    inv = ops[::-1].copy()
+    angle = np.isin(inv["kind"], [G.PHASE, G.RX, G.RY, G.RZ, G.CPHASE, G.MCPHASE])
+    inv["theta"][angle] = -inv["theta"][angle]
+    mat = inv["kind"] == G.MATRIX
+    m = inv["m"][mat]
+    inv["m"][mat] = np.stack([m[:, 0], -m[:, 1], m[:, 4], -m[:, 5], m[:, 2], -m[:, 3], m[:, 6], -m[:, 7]], axis=1)
+    return inv
+
+
+def test_random_layered_30q_round_trip():
+    """The survey's plain-method circuit (one of H/RX/RY/RZ/phase per qubit
+    per layer, CNOT brickwork) at 30 qubits complex128, 20 layers (29 fused
+    passes): the FP64 norm stays 1 and U then U^dagger returns the seeded
+    state at 4096 sampled amplitudes (size-independent properties; the
+    oracle comparison of this circuit runs at 20 qubits)."""
+    n = 30
+    if capi.device_info(0)["total_bytes"] < 2 * 16 * (1 << n) + (4 << 30):
+        pytest.skip("device memory below two 30-qubit state buffers")
+    ctx = capi.Context(0)
+    try:
+        ops = G.random_layered(n, 20, seed=7).ops()
+        st = ctx.state(n, capi.C128)
+        st.set_random(31)
+        rng = np.random.default_rng(5)
+        ks = np.unique(rng.integers(0, 1 << n, size=4096))
+        before = np.concatenate([st.download_range(int(k), 1) for k in ks[:64]] +
+                                [st.download_range(0, 4032)])
+        st.run(ops)
+        assert abs(st.norm2() - 1.0) < 1e-12
+        st.run(inverse_ops(ops))
+        after = np.concatenate([st.download_range(int(k), 1) for k in ks[:64]] +
+                               [st.download_range(0, 4032)])
+        st.close()
+        assert np.max(np.abs(after - before)) < 1e-12
+    finally:
+        ctx.close()
