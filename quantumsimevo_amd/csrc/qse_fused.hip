@@ -37,8 +37,7 @@ int tuned_plan(const qse_ctx* ctx, int n, int prec, const qse_op* ops, int nops,
                bool allow_oop, int* k_out, int* L_out);
 int run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params);
 int validate_ops(int n, const qse_op* ops, int nops, const double* params);
-std::string jit_source(const HostPlan& P, int prec, const size_t* table_off);
-int jit_compile(const std::string& src, int npasses, bool table, hipModule_t* mod, std::vector<hipFunction_t>& fns);
+int jit_build(const HostPlan& P, int prec, const size_t* toff, hipModule_t* mod, std::vector<hipFunction_t>& fns);
 int jit_launch(qse_ctx* ctx, hipFunction_t fn, const void* src, void* dst, const void* table, uint64_t ntiles, int K,
                const void* prog);
 int jit_for_structure(qse_ctx* ctx, const HostPlan& P, const Serialized& S, int prec,
@@ -346,7 +345,7 @@ int jit_for(qse_ctx* ctx, ProgEntry& e, const HostPlan& P, int prec, bool* use) 
     if (ctx->jit_mode == QSE_JIT_AUTO && e.runs < 2) return QSE_OK;
     if (!e.mod) {
         const auto t0 = std::chrono::steady_clock::now();
-        const int rc = jit_compile(jit_source(P, prec, nullptr), (int)P.passes.size(), false, &e.mod, e.fns);
+        const int rc = jit_build(P, prec, nullptr, &e.mod, e.fns);
         ctx->jit_compile_ms +=
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         if (rc) {

@@ -16,6 +16,7 @@
 #pragma once
 #include <stdint.h>
 
+#include <string>
 #include <vector>
 
 namespace qse {
@@ -129,6 +130,12 @@ struct Serialized {
     size_t off[7];
 };
 void serialize(const HostPlan& P, Serialized& S);
+
+// HIP source of every pass of plan P as run-time specialised kernels
+// (qse_jit_gen.cpp); complex128 passes p with (*paired)[p] process tiles in
+// adjacent pairs.
+std::string jit_source(const HostPlan& P, int prec, const size_t* table_off,
+                       const std::vector<bool>* paired = nullptr);
 
 // The serialized plan with every value field zeroed (gate coefficients, phase
 // factors and angles, thread-bit factors): plans that differ only in their

@@ -23,7 +23,6 @@
 namespace qse {
 
 std::string jit_kernel_name(int pass, bool table);
-std::string jit_source(const HostPlan& P, int prec, const size_t* table_off);
 
 // Compile `src` for gfx950 and load it; one function per pass (names per jit_kernel_name).
 int jit_compile(const std::string& src, int npasses, bool table, hipModule_t* mod, std::vector<hipFunction_t>& fns) {
@@ -60,6 +59,41 @@ int jit_compile(const std::string& src, int npasses, bool table, hipModule_t* mo
     return QSE_OK;
 }
 
+// Tile-index bits from which complex128 passes are first generated with
+// paired tiles (2048 pairs or more keep every CU busy).
+constexpr int JIT_PAIR_MIN_NEXT = 12;
+
+// Compile plan P's passes (coefficients from the program at toff when given).
+// Complex128 passes with enough tiles are generated with paired tiles; a pass
+// whose paired kernel spills more than a few registers to scratch memory (its
+// two tiles' amplitudes do not fit the registers of two waves per SIMD) is
+// generated again unpaired.
+constexpr int JIT_PAIR_MAX_SCRATCH = 64;  // bytes per lane
+int jit_build(const HostPlan& P, int prec, const size_t* toff, hipModule_t* mod, std::vector<hipFunction_t>& fns) {
+    const int np = (int)P.passes.size();
+    std::vector<bool> paired(np, false);
+    bool any = false;
+    for (int p = 0; p < np && prec == QSE_C128; ++p)
+        if (P.passes[p].next >= JIT_PAIR_MIN_NEXT) paired[p] = any = true;
+    int rc = jit_compile(jit_source(P, prec, toff, &paired), np, toff != nullptr, mod, fns);
+    if (rc || !any) return rc;
+    bool redo = false;
+    for (int p = 0; p < np; ++p) {
+        if (!paired[p]) continue;
+        int local = 0;
+        QSE_HIP(hipFuncGetAttribute(&local, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fns[p]));
+        if (local > JIT_PAIR_MAX_SCRATCH) {
+            paired[p] = false;
+            redo = true;
+        }
+    }
+    if (!redo) return QSE_OK;
+    QSE_HIP(hipModuleUnload(*mod));
+    *mod = nullptr;
+    fns.clear();
+    return jit_compile(jit_source(P, prec, toff, &paired), np, toff != nullptr, mod, fns);
+}
+
 int jit_launch(qse_ctx* ctx, hipFunction_t fn, const void* src, void* dst, const void* table, uint64_t ntiles, int K,
                const void* prog) {
     const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)ctx->num_cus * 16);
@@ -76,13 +110,13 @@ constexpr int MAX_COMPILE_WORKERS = 4;
 
 void start_compile(qse_ctx* ctx, StructEntry& e, const HostPlan& P, const Serialized& S, int prec) {
     auto job = std::make_shared<JitJob>();
-    std::string src = jit_source(P, prec, S.off);
-    const int np = (int)P.passes.size(), dev = ctx->device;
+    const int dev = ctx->device;
     JitJob* j = job.get();
-    j->worker = std::thread([j, src = std::move(src), np, dev] {
+    std::vector<size_t> off(S.off, S.off + 7);
+    j->worker = std::thread([j, P, off = std::move(off), prec, dev] {
         hipSetDevice(dev);
         const auto t0 = std::chrono::steady_clock::now();
-        const int rc = jit_compile(src, np, true, &j->mod, j->fns);
+        const int rc = jit_build(P, prec, off.data(), &j->mod, j->fns);
         j->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         j->state.store(rc ? 2 : 1, std::memory_order_release);
     });

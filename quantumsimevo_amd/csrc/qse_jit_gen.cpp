@@ -25,7 +25,7 @@ namespace qse {
 
 namespace {
 
-constexpr uint32_t swz_h(uint32_t li) { return li ^ (((li >> 4) ^ (li >> 8) ^ (li >> 12)) & 15u); }
+constexpr uint32_t swz_h(uint32_t li) { return li + (li >> 4); }
 
 // Exact decimal form of a double (17 significant digits round-trip).
 std::string lit(double v) {
@@ -74,7 +74,7 @@ void emit_mul_g(std::ostringstream& o, const std::string& dr, const std::string&
 // read from the program on the device (pv, its base) instead of being a
 // literal, so one compiled module serves all plans of the same structure.
 void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::string& name, bool prefetch,
-               const size_t* toff) {
+               const size_t* toff, bool paired) {
     const FPass& fp = P.passes[pi];
     // coefficient expressions (double): literal, or a load from the device program
     auto opv = [&](uint32_t oi, int j) -> std::string {
@@ -94,7 +94,7 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
     o << "extern \"C\" __global__ __launch_bounds__(" << NT << ") __attribute__((amdgpu_waves_per_eu(2, 8)))\n"
       << "void " << name << "(const cx* src, cx* dst, const SlotVal* __restrict__ slot_table, unsigned long long ntiles, "
       << "const double* __restrict__ pv) {\n"
-      << "  __shared__ cx tile[" << (1 << K) << "];\n"
+      << "  __shared__ cx tile[" << (1 << K) + (1 << (K - 4)) << "];\n"
       << "  __shared__ SlotVal slotv[" << (nslots > 0 ? nslots : 1) << "];\n"
       << "  const unsigned tid = threadIdx.x;\n"
       << "  unsigned long long gt = 0, ot = 0; unsigned st_t = 0;\n";
@@ -121,183 +121,233 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
         }
     };
     o << "  cx ld0, ld1, ld2, ld3, ld4, ld5, ld6, ld7, ld8, ld9, ld10, ld11, ld12, ld13, ld14, ld15;\n";
+    // The tile's stages: amplitudes in `tile`; `base` (external qubits of the
+    // tile), `tidv` and `slotv` in scope.
+    auto emit_stages = [&]() {
+        // Stage code works on scalar components (ar<v>, ai<v>) of 16 amplitude
+        // variables. nm[j] names the variable holding register j: an X/CNOT whose
+        // condition is known at generation time is a renaming, not code. (Whole
+        // struct copies for register exchanges made the compiler keep the
+        // amplitudes in scratch memory.)
+        for (int sg = 0; sg < fp.nstages; ++sg) {
+            const FStage& S = P.stages[fp.stage_begin + sg];
+            o << "    {\n      unsigned toff = 0;\n";
+            for (int t = 0; t < TB; ++t) o << "      if (tidv & " << (1u << t) << "u) toff |= " << (1u << S.tmap[t]) << "u;\n";
+            o << "      const unsigned so = swz(toff);\n      R";
+            for (int v = 0; v < 16; ++v) o << (v ? ", " : " ") << "ar" << v << ", ai" << v;
+            o << ";\n";
+            uint32_t rj[16];
+            int nm[16];
+            for (int j = 0; j < 16; ++j) {
+                uint32_t r = 0;
+                for (int q = 0; q < FUSE_REG; ++q)
+                    if ((j >> q) & 1) r |= 1u << S.rmap[q];
+                rj[j] = swz_h(r);
+                nm[j] = j;
+                o << "      { const cx t = tile[so + " << rj[j] << "u]; ar" << j << " = t.re; ai" << j << " = t.im; }\n";
+            }
+            for (uint32_t oi = S.op_begin; oi < S.op_end; ++oi) {
+                const FOp& op = P.ops[oi];
+                if (op.type == FOP_MONO) {
+                    // constant x thread-bit factors (ascending bit) x terms, as in the generic kernel
+                    o << "      {\n      double gr = " << opv(oi, 0) << ", gi = " << opv(oi, 1) << ";\n";
+                    for (int t = 0; t < TB; ++t)
+                        if ((op.tctl >> t) & 1) {
+                            o << "      if (tidv & " << (1u << t) << "u)\n";
+                            emit_mul_g(o, alphav(op.alpha_begin + 2 * t), alphav(op.alpha_begin + 2 * t + 1));
+                        }
+                    for (uint32_t ti = op.term_begin; ti < op.term_end; ++ti) {
+                        const FTerm& T = P.terms[ti];
+                        std::string cond = "((toff & " + std::to_string(T.tmask) + "u) == " + std::to_string(T.tmask) + "u)";
+                        if (T.slot < 0 && T.emask) cond += " && ((base & " + u64(T.emask) + ") == " + u64(T.emask) + ")";
+                        o << "      if (" << cond << ")\n";
+                        if (T.slot >= 0) {
+                            const std::string sv = "slotv[" + std::to_string(T.slot) + "]";
+                            emit_mul_g(o, sv + ".c", sv + ".s");
+                        } else {
+                            emit_mul_g(o, termv(ti, 0), termv(ti, 1));
+                        }
+                    }
+                    o << "      const R g_r = (R)gr, g_i = (R)gi;\n";
+                    for (int j = 0; j < 16; ++j)
+                        if (((uint32_t)j & op.rctl) == op.rctl) {
+                            const std::string v = std::to_string(nm[j]);
+                            o << "      { const R x = ar" << v << ", y = ai" << v << "; ar" << v << " = g_r * x - g_i * y; ai"
+                              << v << " = g_r * y + g_i * x; }\n";
+                        }
+                    o << "      }\n";
+                    continue;
+                }
+                const bool guard = op.tctl != 0 || op.ectl != 0;
+                const std::string cond = "((toff & " + std::to_string(op.tctl) + "u) == " + std::to_string(op.tctl) +
+                                         "u) && ((base & " + u64(op.ectl) + ") == " + u64(op.ectl) + ")";
+                const int pp = (int)op.p;
+                std::vector<std::pair<int, int>> prs;
+                for (int jj = 0; jj < 8; ++jj) {
+                    const int j0 = ((jj >> pp) << (pp + 1)) | (jj & ((1 << pp) - 1));
+                    // REAL, RXLIKE and H are uncontrolled (rctl = 0); a failed register
+                    // control of DENSE/XSWAP is the identity, i.e. nothing to emit.
+                    if (((uint32_t)j0 & op.rctl) == op.rctl) prs.push_back({j0, j0 | (1 << pp)});
+                }
+                if (op.type == FOP_XSWAP) {
+                    if (!guard) {
+                        for (auto& pr : prs) std::swap(nm[pr.first], nm[pr.second]);
+                        continue;
+                    }
+                    o << "      { const bool c = " << cond << ";\n";
+                    for (auto& pr : prs) {
+                        const std::string a = std::to_string(nm[pr.first]), b = std::to_string(nm[pr.second]);
+                        o << "      { const R x0 = ar" << a << ", y0 = ai" << a << ", x1 = ar" << b << ", y1 = ai" << b << ";\n"
+                          << "        ar" << a << " = c ? x1 : x0; ai" << a << " = c ? y1 : y0; ar" << b << " = c ? x0 : x1; ai"
+                          << b << " = c ? y0 : y1; }\n";
+                    }
+                    o << "      }\n";
+                    continue;
+                }
+                o << (guard ? "      if (" + cond + ") {\n" : std::string("      {\n"));
+                if (toff && op.type != FOP_H) {
+                    o << "      const R";
+                    for (int j = 0; j < 8; ++j) o << (j ? ", " : " ") << "c" << j << " = (R)" << opv(oi, j);
+                    o << ";\n";
+                }
+                for (auto& pr : prs) {
+                    if (toff && op.type != FOP_H) {
+                        // the same products and sums as below, with every coefficient from the table
+                        const std::string a = std::to_string(nm[pr.first]), b = std::to_string(nm[pr.second]);
+                        o << "      { const R x0 = ar" << a << ", y0 = ai" << a << ", x1 = ar" << b << ", y1 = ai" << b << ";\n";
+                        if (op.type == FOP_REAL)
+                            o << "        ar" << a << " = c0 * x0 + c2 * x1; ai" << a << " = c0 * y0 + c2 * y1;\n        ar"
+                              << b << " = c4 * x0 + c6 * x1; ai" << b << " = c4 * y0 + c6 * y1; }\n";
+                        else if (op.type == FOP_RXLIKE)
+                            o << "        ar" << a << " = c0 * x0 - c3 * y1; ai" << a << " = c0 * y0 + c3 * x1;\n        ar"
+                              << b << " = c6 * x1 - c5 * y0; ai" << b << " = c6 * y1 + c5 * x0; }\n";
+                        else
+                            o << "        ar" << a << " = (c0 * x0 - c1 * y0) + (c2 * x1 - c3 * y1); ai" << a
+                              << " = (c0 * y0 + c1 * x0) + (c2 * y1 + c3 * x1);\n        ar" << b
+                              << " = (c4 * x0 - c5 * y0) + (c6 * x1 - c7 * y1); ai" << b
+                              << " = (c4 * y0 + c5 * x0) + (c6 * y1 + c7 * x1); }\n";
+                        continue;
+                    }
+                    const std::string a = std::to_string(nm[pr.first]), b = std::to_string(nm[pr.second]);
+                    o << "      { const R x0 = ar" << a << ", y0 = ai" << a << ", x1 = ar" << b << ", y1 = ai" << b << ";\n";
+                    switch (op.type) {
+                        case FOP_H:
+                            o << "        ar" << a << " = H * (x0 + x1); ai" << a << " = H * (y0 + y1); ar" << b
+                              << " = H * (x0 - x1); ai" << b << " = H * (y0 - y1); }\n";
+                            break;
+                        case FOP_REAL:
+                            o << "        ar" << a << " = " << lin({{op.m[0], "x0"}, {op.m[2], "x1"}}) << "; ai" << a << " = "
+                              << lin({{op.m[0], "y0"}, {op.m[2], "y1"}}) << ";\n        ar" << b << " = "
+                              << lin({{op.m[4], "x0"}, {op.m[6], "x1"}}) << "; ai" << b << " = "
+                              << lin({{op.m[4], "y0"}, {op.m[6], "y1"}}) << "; }\n";
+                            break;
+                        case FOP_RXLIKE:
+                            o << "        ar" << a << " = " << lin({{op.m[0], "x0"}, {-op.m[3], "y1"}}) << "; ai" << a << " = "
+                              << lin({{op.m[0], "y0"}, {op.m[3], "x1"}}) << ";\n        ar" << b << " = "
+                              << lin({{op.m[6], "x1"}, {-op.m[5], "y0"}}) << "; ai" << b << " = "
+                              << lin({{op.m[6], "y1"}, {op.m[5], "x0"}}) << "; }\n";
+                            break;
+                        default:  // FOP_DENSE: (m00 v0) + (m01 v1), (m10 v0) + (m11 v1)
+                            o << "        ar" << a << " = " << sum2(lin({{op.m[0], "x0"}, {-op.m[1], "y0"}}), lin({{op.m[2], "x1"}, {-op.m[3], "y1"}}))
+                              << "; ai" << a << " = " << sum2(lin({{op.m[0], "y0"}, {op.m[1], "x0"}}), lin({{op.m[2], "y1"}, {op.m[3], "x1"}}))
+                              << ";\n        ar" << b << " = " << sum2(lin({{op.m[4], "x0"}, {-op.m[5], "y0"}}), lin({{op.m[6], "x1"}, {-op.m[7], "y1"}}))
+                              << "; ai" << b << " = " << sum2(lin({{op.m[4], "y0"}, {op.m[5], "x0"}}), lin({{op.m[6], "y1"}, {op.m[7], "x1"}}))
+                              << "; }\n";
+                            break;
+                    }
+                }
+                o << "      }\n";
+            }
+            for (int j = 0; j < 16; ++j)
+                o << "      { cx t; t.re = ar" << nm[j] << "; t.im = ai" << nm[j] << "; tile[so + " << rj[j] << "u] = t; }\n";
+            o << "      __syncthreads();\n    }\n";
+        }
+    };
+    auto emit_tile_in = [&](const char* ld) {
+        if (nslots > 0)
+            o << "    for (int sl = tid; sl < " << nslots << "; sl += " << NT << ") slotv[sl] = slot_table[ti * "
+              << nslots << "ull + sl];\n";
+        for (int i = 0; i < 16; ++i)
+            o << "    tile[swz_tid + " << swz_h((uint32_t)(i * NT)) << "u] = " << ld << i << ";\n";
+        o << "    __syncthreads();\n";
+    };
+    auto emit_store = [&](const char* obase) {
+        for (int i = 0; i < 16; ++i) {
+            uint64_t g = 0;
+            uint32_t sl = 0;
+            for (int c = 0; c < 4; ++c)
+                if ((i >> c) & 1) {
+                    g |= 1ULL << fp.so_q[TB + c];
+                    sl |= 1u << fp.so_l[TB + c];
+                }
+            o << "    stnt(dst + (" << obase << " | ot | " << u64(g) << "), tile[swz_st + " << swz_h(sl) << "u]);\n";
+        }
+    };
+    auto emit_bases = [&](const char* ti, const char* base, const char* obase) {
+        o << "    unsigned long long " << base << " = 0, " << obase << " = 0;\n";
+        for (int e = 0; e < fp.next; ++e)
+            o << "    " << base << " |= ((" << ti << " >> " << e << ") & 1ull) << " << fp.epos[e] << "; " << obase
+              << " |= ((" << ti << " >> " << e << ") & 1ull) << " << fp.eout[e] << ";\n";
+    };
+    // the thread id as a value the compiler cannot carry across iterations:
+    // thread-dependent phase factors are recomputed per tile instead of being
+    // hoisted out of the loop, where they would hold registers for its length
+    const char* opaque_tid = "    unsigned tidv = tid;\n    QSE_OPAQUE(tidv);\n";
+    // Paired tiles: a workgroup takes tiles 2p and 2p + 1, which differ in the
+    // lowest external qubit, and issues the loads of both at once. When that
+    // qubit sits right above the tile's contiguous low qubits, the two tiles'
+    // runs are adjacent, and HBM serves the pair like runs of twice the length
+    // (128-byte runs: 4.75 against 4.08 TB/s for a copy of that pattern). The
+    // second tile waits in registers while the first one's stages run; each
+    // tile is stored when its stages are done.
+    if (paired && fp.next >= 1 && !prefetch) {
+        o << "  cx lb0, lb1, lb2, lb3, lb4, lb5, lb6, lb7, lb8, lb9, lb10, lb11, lb12, lb13, lb14, lb15;\n";
+        o << "  for (unsigned long long tp = blockIdx.x; tp < ntiles / 2; tp += gridDim.x) {\n";
+        o << opaque_tid;
+        o << "    const unsigned long long ta = 2 * tp;\n";
+        emit_bases("ta", "base_a", "obase_a");
+        o << "    const unsigned long long base_b = base_a | " << u64(1ULL << fp.epos[0]) << ", obase_b = obase_a | "
+          << u64(1ULL << fp.eout[0]) << ";\n";
+        for (int i = 0; i < 16; ++i) {
+            uint64_t g = 0;
+            for (int c = 0; c < 4; ++c)
+                if ((i >> c) & 1) g |= 1ULL << fp.tq[TB + c];
+            o << "    ld" << i << " = ldnt(src + (base_a | gt | " << u64(g) << ")); lb" << i
+              << " = ldnt(src + (base_b | gt | " << u64(g) << "));\n";
+        }
+        o << "    {\n    const unsigned long long ti = ta, base = base_a;\n";
+        emit_tile_in("ld");
+        emit_stages();
+        emit_store("obase_a");
+        o << "    __syncthreads();\n    }\n";
+        o << "    {\n    const unsigned long long ti = ta + 1, base = base_b;\n";
+        emit_tile_in("lb");
+        emit_stages();
+        emit_store("obase_b");
+        o << "    __syncthreads();\n    }\n  }\n}\n\n";
+        return;
+    }
     if (prefetch) {
         o << "  if ((unsigned long long)blockIdx.x < ntiles) {\n    const unsigned long long t0 = blockIdx.x;\n";
         emit_loads("t0", "b0");
         o << "  }\n";
     }
     o << "  for (unsigned long long ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {\n";
-    // the thread id as a value the compiler cannot carry across iterations:
-    // thread-dependent phase factors are recomputed per tile instead of being
-    // hoisted out of the loop, where they would hold registers for its length
-    o << "    unsigned tidv = tid;\n    QSE_OPAQUE(tidv);\n";
-    o << "    unsigned long long base = 0, obase = 0;\n";
-    for (int e = 0; e < fp.next; ++e)
-        o << "    base |= ((ti >> " << e << ") & 1ull) << " << fp.epos[e] << "; obase |= ((ti >> " << e
-          << ") & 1ull) << " << fp.eout[e] << ";\n";
+    o << opaque_tid;
+    emit_bases("ti", "base", "obase");
     if (!prefetch) {
         o << "    {\n";
         emit_loads("ti", "bt");
         o << "    }\n";
     }
-    if (nslots > 0)
-        o << "    for (int sl = tid; sl < " << nslots << "; sl += " << NT << ") slotv[sl] = slot_table[ti * "
-          << nslots << "ull + sl];\n";
-    for (int i = 0; i < 16; ++i) o << "    tile[swz_tid ^ " << swz_h((uint32_t)(i * NT)) << "u] = ld" << i << ";\n";
-    o << "    __syncthreads();\n";
+    emit_tile_in("ld");
     if (prefetch) {
         o << "    if (ti + gridDim.x < ntiles) {\n      const unsigned long long tn = ti + gridDim.x;\n";
         emit_loads("tn", "bn");
         o << "    }\n";
     }
-    // Stage code works on scalar components (ar<v>, ai<v>) of 16 amplitude
-    // variables. nm[j] names the variable holding register j: an X/CNOT whose
-    // condition is known at generation time is a renaming, not code. (Whole
-    // struct copies for register exchanges made the compiler keep the
-    // amplitudes in scratch memory.)
-    for (int sg = 0; sg < fp.nstages; ++sg) {
-        const FStage& S = P.stages[fp.stage_begin + sg];
-        o << "    {\n      unsigned toff = 0;\n";
-        for (int t = 0; t < TB; ++t) o << "      if (tidv & " << (1u << t) << "u) toff |= " << (1u << S.tmap[t]) << "u;\n";
-        o << "      const unsigned so = swz(toff);\n      R";
-        for (int v = 0; v < 16; ++v) o << (v ? ", " : " ") << "ar" << v << ", ai" << v;
-        o << ";\n";
-        uint32_t rj[16];
-        int nm[16];
-        for (int j = 0; j < 16; ++j) {
-            uint32_t r = 0;
-            for (int q = 0; q < FUSE_REG; ++q)
-                if ((j >> q) & 1) r |= 1u << S.rmap[q];
-            rj[j] = swz_h(r);
-            nm[j] = j;
-            o << "      { const cx t = tile[so ^ " << rj[j] << "u]; ar" << j << " = t.re; ai" << j << " = t.im; }\n";
-        }
-        for (uint32_t oi = S.op_begin; oi < S.op_end; ++oi) {
-            const FOp& op = P.ops[oi];
-            if (op.type == FOP_MONO) {
-                // constant x thread-bit factors (ascending bit) x terms, as in the generic kernel
-                o << "      {\n      double gr = " << opv(oi, 0) << ", gi = " << opv(oi, 1) << ";\n";
-                for (int t = 0; t < TB; ++t)
-                    if ((op.tctl >> t) & 1) {
-                        o << "      if (tidv & " << (1u << t) << "u)\n";
-                        emit_mul_g(o, alphav(op.alpha_begin + 2 * t), alphav(op.alpha_begin + 2 * t + 1));
-                    }
-                for (uint32_t ti = op.term_begin; ti < op.term_end; ++ti) {
-                    const FTerm& T = P.terms[ti];
-                    std::string cond = "((toff & " + std::to_string(T.tmask) + "u) == " + std::to_string(T.tmask) + "u)";
-                    if (T.slot < 0 && T.emask) cond += " && ((base & " + u64(T.emask) + ") == " + u64(T.emask) + ")";
-                    o << "      if (" << cond << ")\n";
-                    if (T.slot >= 0) {
-                        const std::string sv = "slotv[" + std::to_string(T.slot) + "]";
-                        emit_mul_g(o, sv + ".c", sv + ".s");
-                    } else {
-                        emit_mul_g(o, termv(ti, 0), termv(ti, 1));
-                    }
-                }
-                o << "      const R g_r = (R)gr, g_i = (R)gi;\n";
-                for (int j = 0; j < 16; ++j)
-                    if (((uint32_t)j & op.rctl) == op.rctl) {
-                        const std::string v = std::to_string(nm[j]);
-                        o << "      { const R x = ar" << v << ", y = ai" << v << "; ar" << v << " = g_r * x - g_i * y; ai"
-                          << v << " = g_r * y + g_i * x; }\n";
-                    }
-                o << "      }\n";
-                continue;
-            }
-            const bool guard = op.tctl != 0 || op.ectl != 0;
-            const std::string cond = "((toff & " + std::to_string(op.tctl) + "u) == " + std::to_string(op.tctl) +
-                                     "u) && ((base & " + u64(op.ectl) + ") == " + u64(op.ectl) + ")";
-            const int pp = (int)op.p;
-            std::vector<std::pair<int, int>> prs;
-            for (int jj = 0; jj < 8; ++jj) {
-                const int j0 = ((jj >> pp) << (pp + 1)) | (jj & ((1 << pp) - 1));
-                // REAL, RXLIKE and H are uncontrolled (rctl = 0); a failed register
-                // control of DENSE/XSWAP is the identity, i.e. nothing to emit.
-                if (((uint32_t)j0 & op.rctl) == op.rctl) prs.push_back({j0, j0 | (1 << pp)});
-            }
-            if (op.type == FOP_XSWAP) {
-                if (!guard) {
-                    for (auto& pr : prs) std::swap(nm[pr.first], nm[pr.second]);
-                    continue;
-                }
-                o << "      { const bool c = " << cond << ";\n";
-                for (auto& pr : prs) {
-                    const std::string a = std::to_string(nm[pr.first]), b = std::to_string(nm[pr.second]);
-                    o << "      { const R x0 = ar" << a << ", y0 = ai" << a << ", x1 = ar" << b << ", y1 = ai" << b << ";\n"
-                      << "        ar" << a << " = c ? x1 : x0; ai" << a << " = c ? y1 : y0; ar" << b << " = c ? x0 : x1; ai"
-                      << b << " = c ? y0 : y1; }\n";
-                }
-                o << "      }\n";
-                continue;
-            }
-            o << (guard ? "      if (" + cond + ") {\n" : std::string("      {\n"));
-            if (toff && op.type != FOP_H) {
-                o << "      const R";
-                for (int j = 0; j < 8; ++j) o << (j ? ", " : " ") << "c" << j << " = (R)" << opv(oi, j);
-                o << ";\n";
-            }
-            for (auto& pr : prs) {
-                if (toff && op.type != FOP_H) {
-                    // the same products and sums as below, with every coefficient from the table
-                    const std::string a = std::to_string(nm[pr.first]), b = std::to_string(nm[pr.second]);
-                    o << "      { const R x0 = ar" << a << ", y0 = ai" << a << ", x1 = ar" << b << ", y1 = ai" << b << ";\n";
-                    if (op.type == FOP_REAL)
-                        o << "        ar" << a << " = c0 * x0 + c2 * x1; ai" << a << " = c0 * y0 + c2 * y1;\n        ar"
-                          << b << " = c4 * x0 + c6 * x1; ai" << b << " = c4 * y0 + c6 * y1; }\n";
-                    else if (op.type == FOP_RXLIKE)
-                        o << "        ar" << a << " = c0 * x0 - c3 * y1; ai" << a << " = c0 * y0 + c3 * x1;\n        ar"
-                          << b << " = c6 * x1 - c5 * y0; ai" << b << " = c6 * y1 + c5 * x0; }\n";
-                    else
-                        o << "        ar" << a << " = (c0 * x0 - c1 * y0) + (c2 * x1 - c3 * y1); ai" << a
-                          << " = (c0 * y0 + c1 * x0) + (c2 * y1 + c3 * x1);\n        ar" << b
-                          << " = (c4 * x0 - c5 * y0) + (c6 * x1 - c7 * y1); ai" << b
-                          << " = (c4 * y0 + c5 * x0) + (c6 * y1 + c7 * x1); }\n";
-                    continue;
-                }
-                const std::string a = std::to_string(nm[pr.first]), b = std::to_string(nm[pr.second]);
-                o << "      { const R x0 = ar" << a << ", y0 = ai" << a << ", x1 = ar" << b << ", y1 = ai" << b << ";\n";
-                switch (op.type) {
-                    case FOP_H:
-                        o << "        ar" << a << " = H * (x0 + x1); ai" << a << " = H * (y0 + y1); ar" << b
-                          << " = H * (x0 - x1); ai" << b << " = H * (y0 - y1); }\n";
-                        break;
-                    case FOP_REAL:
-                        o << "        ar" << a << " = " << lin({{op.m[0], "x0"}, {op.m[2], "x1"}}) << "; ai" << a << " = "
-                          << lin({{op.m[0], "y0"}, {op.m[2], "y1"}}) << ";\n        ar" << b << " = "
-                          << lin({{op.m[4], "x0"}, {op.m[6], "x1"}}) << "; ai" << b << " = "
-                          << lin({{op.m[4], "y0"}, {op.m[6], "y1"}}) << "; }\n";
-                        break;
-                    case FOP_RXLIKE:
-                        o << "        ar" << a << " = " << lin({{op.m[0], "x0"}, {-op.m[3], "y1"}}) << "; ai" << a << " = "
-                          << lin({{op.m[0], "y0"}, {op.m[3], "x1"}}) << ";\n        ar" << b << " = "
-                          << lin({{op.m[6], "x1"}, {-op.m[5], "y0"}}) << "; ai" << b << " = "
-                          << lin({{op.m[6], "y1"}, {op.m[5], "x0"}}) << "; }\n";
-                        break;
-                    default:  // FOP_DENSE: (m00 v0) + (m01 v1), (m10 v0) + (m11 v1)
-                        o << "        ar" << a << " = " << sum2(lin({{op.m[0], "x0"}, {-op.m[1], "y0"}}), lin({{op.m[2], "x1"}, {-op.m[3], "y1"}}))
-                          << "; ai" << a << " = " << sum2(lin({{op.m[0], "y0"}, {op.m[1], "x0"}}), lin({{op.m[2], "y1"}, {op.m[3], "x1"}}))
-                          << ";\n        ar" << b << " = " << sum2(lin({{op.m[4], "x0"}, {-op.m[5], "y0"}}), lin({{op.m[6], "x1"}, {-op.m[7], "y1"}}))
-                          << "; ai" << b << " = " << sum2(lin({{op.m[4], "y0"}, {op.m[5], "x0"}}), lin({{op.m[6], "y1"}, {op.m[7], "x1"}}))
-                          << "; }\n";
-                        break;
-                }
-            }
-            o << "      }\n";
-        }
-        for (int j = 0; j < 16; ++j)
-            o << "      { cx t; t.re = ar" << nm[j] << "; t.im = ai" << nm[j] << "; tile[so ^ " << rj[j] << "u] = t; }\n";
-        o << "      __syncthreads();\n    }\n";
-    }
-    for (int i = 0; i < 16; ++i) {
-        uint64_t g = 0;
-        uint32_t s = 0;
-        for (int c = 0; c < 4; ++c)
-            if ((i >> c) & 1) {
-                g |= 1ULL << fp.so_q[TB + c];
-                s |= 1u << fp.so_l[TB + c];
-            }
-        o << "    stnt(dst + (obase | ot | " << u64(g) << "), tile[swz_st ^ " << swz_h(s) << "u]);\n";
-    }
+    emit_stages();
+    emit_store("obase");
     o << "    __syncthreads();\n  }\n}\n\n";
 }
 
@@ -312,8 +362,9 @@ std::string jit_kernel_name(int pass, bool table) {
 
 // HIP source of every pass of plan P (kernel qse_jit_pass<i> for pass i).
 // With table_off (the offsets of P's serialized program) the coefficients are
-// read from the program buffer passed as the kernels' last argument.
-std::string jit_source(const HostPlan& P, int prec, const size_t* table_off) {
+// read from the program buffer passed as the kernels' last argument. Complex128
+// passes p with (*paired)[p] set process tiles in adjacent pairs.
+std::string jit_source(const HostPlan& P, int prec, const size_t* table_off, const std::vector<bool>* paired) {
     std::ostringstream o;
     o << "typedef " << (prec == QSE_C128 ? "double" : "float") << " R;\n"
       << "struct __attribute__((aligned(2 * sizeof(R)))) cx { R re, im; };\n"
@@ -322,8 +373,7 @@ std::string jit_source(const HostPlan& P, int prec, const size_t* table_off) {
          "__device__ __forceinline__ cx cmul(cx a, cx b) { cx r; r.re = a.re * b.re - a.im * b.im; "
          "r.im = a.re * b.im + a.im * b.re; return r; }\n"
          "__device__ __forceinline__ cx cadd(cx a, cx b) { cx r; r.re = a.re + b.re; r.im = a.im + b.im; return r; }\n"
-         "__device__ __forceinline__ unsigned swz(unsigned li) { return li ^ (((li >> 4) ^ (li >> 8) ^ (li >> 12)) & "
-         "15u); }\n"
+         "__device__ __forceinline__ unsigned swz(unsigned li) { return li + (li >> 4); }\n"
          "#ifndef QSE_OPAQUE\n#define QSE_OPAQUE(v) __asm__ volatile(\"\" : \"+v\"(v))\n#endif\n";
     // complex128 state loads and stores are non-temporal (streamed past the caches): every amplitude is
     // read and written once per pass, and the hint made 30-qubit QFT passes 2-5% faster; for complex64
@@ -339,7 +389,8 @@ std::string jit_source(const HostPlan& P, int prec, const size_t* table_off) {
         o << "__device__ __forceinline__ cx ldnt(const cx* p) { return *p; }\n"
              "__device__ __forceinline__ void stnt(cx* p, cx a) { *p = a; }\n\n";
     for (int p = 0; p < (int)P.passes.size(); ++p)
-        emit_pass(o, P, p, jit_kernel_name(p, table_off != nullptr), prec == QSE_C64, table_off);
+        emit_pass(o, P, p, jit_kernel_name(p, table_off != nullptr), prec == QSE_C64, table_off,
+                  prec == QSE_C128 && paired && p < (int)paired->size() && (*paired)[p]);
     return o.str();
 }
 

@@ -264,10 +264,12 @@ extern "C" int emu_plan_stats(int n, int k, int L, const qse_op* ops, int nops, 
 // emu_jit_prepare plans like the library (relabelling plan, in-place plan when
 // it is not realisable), keeps the plan for the calls below and returns the
 // length of the generated HIP source, copied into `out` when `cap` allows.
-namespace qse {
-std::string jit_source(const HostPlan& P, int prec, const size_t* table_off);
-}
 static HostPlan g_jit_plan;
+static bool g_jit_paired = false;
+
+// Generate complex128 passes with paired tiles (the library pairs passes of
+// 12 tile-index bits or more whose paired kernel does not spill registers).
+extern "C" void emu_jit_set_pairing(int on) { g_jit_paired = on != 0; }
 static Serialized g_jit_prog;
 static int g_jit_n = 0;
 
@@ -280,7 +282,8 @@ extern "C" long emu_jit_prepare2(int n, int k, int L, const qse_op* ops, int nop
     if (rc) return -rc;
     g_jit_n = n;
     serialize(g_jit_plan, g_jit_prog);
-    const std::string src = jit_source(g_jit_plan, prec, table ? g_jit_prog.off : nullptr);
+    const std::vector<bool> paired(g_jit_plan.passes.size(), g_jit_paired);
+    const std::string src = jit_source(g_jit_plan, prec, table ? g_jit_prog.off : nullptr, &paired);
     if (out && cap > (long)src.size()) std::memcpy(out, src.c_str(), src.size() + 1);
     return (long)src.size();
 }

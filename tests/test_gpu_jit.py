@@ -175,3 +175,50 @@ def test_autotune_measures_and_keeps_the_fastest_shape(n, make):
         st.close()
     finally:
         c.close()
+
+
+def check_vs_generic(ctx, n, ops, params=None, seed=5):
+    psi = G.random_state(n, seed)
+    jit = run_mode(ctx, capi.JIT_ALWAYS, n, ops, psi, params)
+    generic = run_mode(ctx, capi.JIT_OFF, n, ops, psi, params)
+    assert np.max(np.abs(jit - generic)) < 1e-13
+    return jit, psi
+
+
+def test_paired_tiles_from_12_tile_index_bits(ctx):
+    """Complex128 passes with 12 or more tile-index bits take tiles in adjacent
+    pairs (when the paired kernel fits the registers): QFT-24 (with its
+    out-of-place bit reversal), a layered circuit, 10-qubit tiles with
+    external-qubit slots, against the generic kernel; one short layered
+    circuit against the oracle at 24 qubits."""
+    check_vs_generic(ctx, 24, G.qft(24).ops())
+    check_vs_generic(ctx, 25, G.random_layered(25, 6, seed=2).ops())
+    ctx.set_fusion(capi.C128, 10, 3)
+    try:
+        edges = G.random_regular_graph(22, 3, seed=8)
+        q = G.qaoa_maxcut(22, edges, 2).ops()
+        check_vs_generic(ctx, 22, q, np.array([0.3, 1.1, 0.7, 0.4]))
+    finally:
+        ctx.set_fusion(capi.C128, 0, 0)
+    ops = G.random_layered(24, 2, seed=6).ops()
+    jit, psi = check_vs_generic(ctx, 24, ops)
+    assert np.max(np.abs(jit - O.run_ops(psi, 24, ops))) < 1e-12
+
+
+def test_paired_structure_keyed_kernels_in_the_adjoint():
+    """The adjoint's internal runs at 24 qubits (structure-keyed kernels,
+    paired) against the generic kernel: QAOA p = 1 gradients."""
+    ctx = capi.Context(0)
+    try:
+        n = 24
+        edges = G.random_regular_graph(n, 3, seed=12)
+        ops = G.qaoa_maxcut(n, edges, 1).ops()
+        masks, w = G.maxcut_observable(edges)
+        params = np.array([[0.37, 0.81]])
+        ctx.set_jit(capi.JIT_ALWAYS)
+        E1, G1 = ctx.adjoint_expect(n, capi.C128, ops, params, masks, w)
+        ctx.set_jit(capi.JIT_OFF)
+        E0, G0 = ctx.adjoint_expect(n, capi.C128, ops, params, masks, w)
+        assert abs(E1[0] - E0[0]) < 1e-11 and np.max(np.abs(G1 - G0)) < 1e-11
+    finally:
+        ctx.close()

@@ -195,3 +195,31 @@ def test_structure_key_ignores_values():
     assert len(other) == len(q)
     assert lib.emu_same_structure(n, 9, 3, q.ctypes.data, p1.ctypes.data, other.ctypes.data, p1.ctypes.data,
                                   len(q)) == 0
+
+
+@pytest.fixture
+def paired():
+    """Generate complex128 passes with paired tiles (two tiles per workgroup
+    iteration, loads of both issued together)."""
+    lib = emu()
+    lib.emu_jit_set_pairing.argtypes = [C.c_int]
+    lib.emu_jit_set_pairing(1)
+    yield
+    lib.emu_jit_set_pairing(0)
+
+
+def test_paired_tiles(paired):
+    """Paired-tile passes: QFT with and without the out-of-place relabelling
+    pass, a layered circuit, QAOA with per-tile slots and terms, every gate
+    kind in the structure-keyed form; complex64 passes stay unpaired."""
+    n = 12
+    src = _prepare(n, 8, 3, G.qft(n).ops(), None, -1, 0)
+    assert "ntiles / 2" in src and "lb15" in src
+    check(n, 8, 3, G.qft(n).ops())
+    check(n, 9, 2, G.random_layered(n, 6, seed=4).ops(), relabels=(-1,))
+    edges = G.random_regular_graph(n, 3, seed=1)
+    q = G.qaoa_maxcut(n, edges, 3).ops()
+    params = np.random.default_rng(2).random(6) * math.pi
+    check(n, 8, 2, q, params, relabels=(-1,))
+    check(n, 8, 4, G.random_mixed(n, 300, 5).ops(), seed=6, relabels=(-1,), table=1)
+    assert "ntiles / 2" not in _prepare(n, 8, 3, G.qft(n).ops(), None, -1, 1)
