@@ -60,8 +60,9 @@ int jit_compile(const std::string& src, int npasses, bool table, hipModule_t* mo
 }
 
 // Tile-index bits from which complex128 passes are first generated with
-// paired tiles (2048 pairs or more keep every CU busy).
-constexpr int JIT_PAIR_MIN_NEXT = 12;
+// paired tiles: 4096 pairs or more (at 12 bits, the QAOA-24 adjoint's passes
+// ran 7% slower paired than unpaired).
+constexpr int JIT_PAIR_MIN_NEXT = 13;
 
 // Compile plan P's passes (coefficients from the program at toff when given).
 // Complex128 passes with enough tiles are generated with paired tiles; a pass

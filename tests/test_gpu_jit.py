@@ -185,32 +185,32 @@ def check_vs_generic(ctx, n, ops, params=None, seed=5):
     return jit, psi
 
 
-def test_paired_tiles_from_12_tile_index_bits(ctx):
-    """Complex128 passes with 12 or more tile-index bits take tiles in adjacent
-    pairs (when the paired kernel fits the registers): QFT-24 (with its
-    out-of-place bit reversal), a layered circuit, 10-qubit tiles with
+def test_paired_tiles_from_13_tile_index_bits(ctx):
+    """Complex128 passes with 13 or more tile-index bits take tiles in adjacent
+    pairs (when the paired kernel fits the registers): QFT-25 (with its
+    out-of-place bit reversal), a layered circuit, 11-qubit tiles with
     external-qubit slots, against the generic kernel; one short layered
-    circuit against the oracle at 24 qubits."""
-    check_vs_generic(ctx, 24, G.qft(24).ops())
+    circuit against the oracle at 25 qubits."""
+    check_vs_generic(ctx, 25, G.qft(25).ops())
     check_vs_generic(ctx, 25, G.random_layered(25, 6, seed=2).ops())
-    ctx.set_fusion(capi.C128, 10, 3)
+    ctx.set_fusion(capi.C128, 11, 3)
     try:
-        edges = G.random_regular_graph(22, 3, seed=8)
-        q = G.qaoa_maxcut(22, edges, 2).ops()
-        check_vs_generic(ctx, 22, q, np.array([0.3, 1.1, 0.7, 0.4]))
+        edges = G.random_regular_graph(24, 3, seed=8)
+        q = G.qaoa_maxcut(24, edges, 2).ops()
+        check_vs_generic(ctx, 24, q, np.array([0.3, 1.1, 0.7, 0.4]))
     finally:
         ctx.set_fusion(capi.C128, 0, 0)
-    ops = G.random_layered(24, 2, seed=6).ops()
-    jit, psi = check_vs_generic(ctx, 24, ops)
-    assert np.max(np.abs(jit - O.run_ops(psi, 24, ops))) < 1e-12
+    ops = G.random_layered(25, 2, seed=6).ops()
+    jit, psi = check_vs_generic(ctx, 25, ops)
+    assert np.max(np.abs(jit - O.run_ops(psi, 25, ops))) < 1e-12
 
 
 def test_paired_structure_keyed_kernels_in_the_adjoint():
-    """The adjoint's internal runs at 24 qubits (structure-keyed kernels,
+    """The adjoint's internal runs at 26 qubits (structure-keyed kernels,
     paired) against the generic kernel: QAOA p = 1 gradients."""
     ctx = capi.Context(0)
     try:
-        n = 24
+        n = 26
         edges = G.random_regular_graph(n, 3, seed=12)
         ops = G.qaoa_maxcut(n, edges, 1).ops()
         masks, w = G.maxcut_observable(edges)
