@@ -234,19 +234,22 @@ __device__ __forceinline__ constexpr int pair_group(int s, int j) {
 __device__ __forceinline__ constexpr int pair_tbit(int s, int i) {
     return s == 0 ? (i < 3 ? i : i + 4) : s == 1 ? (i < 7 ? i : 11) : 3 + i;
 }
-__device__ __forceinline__ uint32_t pair_local(int s, uint32_t t, int r) {
+// LDS place of local index li: one element of padding after every 16, so that
+// the 16 lanes of each quarter-wave hit 16 different 16-byte bank groups in all
+// three layouts. The padding is additive over disjoint bits, so a stage's 16
+// places are one per-thread base plus constant offsets (no address registers).
+__device__ __forceinline__ constexpr uint32_t pair_pad(uint32_t li) { return li + (li >> 4); }
+constexpr int PAIR_LDS = (1 << PAIR_K) + (1 << (PAIR_K - 4));
+__device__ __forceinline__ uint32_t pair_tpart(int s, uint32_t t) {
     uint32_t li = 0;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) li |= (uint32_t)((r >> j) & 1) << pair_group(s, j);
 #pragma unroll
     for (int i = 0; i < 8; ++i) li |= ((t >> i) & 1u) << pair_tbit(s, i);
     return li;
 }
-// LDS place of local index li: low bits XORed with bits 4..7, so that the 16
-// lanes of each quarter-wave hit 16 different 16-byte bank groups in all three
-// layouts
-__device__ __forceinline__ uint32_t pair_swz(uint32_t li) {
-    return li ^ ((li >> 4) & 7u) ^ (((li >> 7) & 1u) << 3);
+__device__ __forceinline__ constexpr uint32_t pair_rpart(int s, int r) {
+    uint32_t li = 0;
+    for (int j = 0; j < 4; ++j) li |= (uint32_t)((r >> j) & 1) << pair_group(s, j);
+    return li;
 }
 
 template <class R, int S>
@@ -271,30 +274,25 @@ __device__ __forceinline__ void pair_terms(const cx<R> (&p)[16], const cx<R> (&l
     }
 }
 
-template <class R>
-__device__ __forceinline__ void pair_exchange(cx<R> (&v)[16], cx<R>* lds, uint32_t t, int from) {
+template <class R, int FROM>
+__device__ __forceinline__ void pair_exchange(cx<R> (&v)[16], cx<R>* lds, uint32_t t) {
+    __syncthreads();
+    const uint32_t wb = pair_pad(pair_tpart(FROM, t)), rb = pair_pad(pair_tpart(FROM + 1, t));
+#pragma unroll
+    for (int r = 0; r < 16; ++r) lds[wb + pair_pad(pair_rpart(FROM, r))] = v[r];
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const uint32_t li = from == 0 ? pair_local(0, t, r) : pair_local(1, t, r);
-        lds[pair_swz(li)] = v[r];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const uint32_t li = from == 0 ? pair_local(1, t, r) : pair_local(2, t, r);
-        v[r] = lds[pair_swz(li)];
-    }
+    for (int r = 0; r < 16; ++r) v[r] = lds[rb + pair_pad(pair_rpart(FROM + 1, r))];
 }
 
 template <class R>
 __global__ __launch_bounds__(PAIR_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_grad_pair(const cx<R>* __restrict__ psi,
                                                             const cx<R>* __restrict__ lam, uint64_t ntiles,
                                                             PairPass P, double* __restrict__ part) {
-    __shared__ cx<R> lds[1 << PAIR_K];
+    __shared__ cx<R> lds[PAIR_LDS];
     __shared__ double red[PAIR_THREADS / 64][PAIR_COLS];
-    double* hw = reinterpret_cast<double*>(lds);  // W[u] of thread t at hw[u * 256 + t] (stage 0)
-    static_assert(sizeof(cx<R>) * (1 << PAIR_K) >= sizeof(double) * 16 * PAIR_THREADS, "LDS too small for W");
+    double* hw = reinterpret_cast<double*>(lds);  // W[u] of thread t at hw[u * 256 + t] (last stage)
+    static_assert(sizeof(cx<R>) * PAIR_LDS >= sizeof(double) * 16 * PAIR_THREADS, "LDS too small for W");
     const uint32_t t = threadIdx.x;
     double acc[PAIR_K], zacc[PAIR_Z];
 #pragma unroll
@@ -332,13 +330,13 @@ __global__ __launch_bounds__(PAIR_THREADS) __attribute__((amdgpu_waves_per_eu(2)
         }
         pair_terms<R, 0>(p, l, P, acc);
         if (P.nstages > 1) {
-            pair_exchange<R>(p, lds, t, 0);
-            pair_exchange<R>(l, lds, t, 0);
+            pair_exchange<R, 0>(p, lds, t);
+            pair_exchange<R, 0>(l, lds, t);
             pair_terms<R, 1>(p, l, P, acc);
         }
         if (P.nstages > 2) {
-            pair_exchange<R>(p, lds, t, 1);
-            pair_exchange<R>(l, lds, t, 1);
+            pair_exchange<R, 1>(p, lds, t);
+            pair_exchange<R, 1>(l, lds, t);
             pair_terms<R, 2>(p, l, P, acc);
         }
         if (P.nz > 0) {
