@@ -188,7 +188,7 @@ def uneven_batch_leg(ctx) -> dict:
 
 
 def layered_leg(ctx, n: int = 30, layers: int = 20) -> dict:
-    """The survey's plain-method circuit at 30 qubits complex128: per layer one
+    """The plain-method random circuit at 30 qubits complex128: per layer one
     of H/RX/RY/RZ/phase per qubit (angles uniform in [0, 2pi)), then a CNOT
     brickwork; executed as fused passes on a resident seeded random state.
     Per-pass HBM rate from the library's HIP events (2 x 16 x 2^n bytes per
@@ -295,7 +295,7 @@ def main() -> int:
                     help="only run X on qubit 0 (one read + one write of every amplitude) for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+        print(f"bench.py measures one GPU only: the library drives one GPU per process and this benchmark measures one (asked for --gpus {args.gpus})",
               file=sys.stderr)
         return 2
 

@@ -93,8 +93,8 @@ def inverse_ops(ops):
 
 
 def test_random_layered_30q_round_trip():
-    """The survey's plain-method circuit (one of H/RX/RY/RZ/phase per qubit
-    per layer, CNOT brickwork) at 30 qubits complex128, 20 layers (29 fused
+    """The plain-method random circuit (one of H/RX/RY/RZ/phase per qubit per
+    layer, CNOT brickwork) at 30 qubits complex128, 20 layers (29 fused
     passes): the FP64 norm stays 1 and U then U^dagger returns the seeded
     state at 4096 sampled amplitudes (size-independent properties; the
     oracle comparison of this circuit runs at 20 qubits)."""
