@@ -345,6 +345,16 @@ int qse_batch_run_small(qse_ctx* ctx, int precision, int ncirc,
                         const int64_t* par_off, const double* params,
                         const uint64_t* obs_masks, const double* obs_weights, int nterms,
                         double* energies, double* grads);
+/* Measured tile shape for qse_batch_run_small (no counterpart in the
+ * reference): runs the given circuits once per candidate shape (tiles of 12,
+ * 11 or 10 qubits, 2 to 5 contiguous low qubits; pass a sample of a large
+ * batch) and keeps the fastest in the context for later batches of this
+ * precision. with_grads: time energies plus gradients (par_off needed) or
+ * energies only. Outputs the chosen shape and its time (may be null). */
+int qse_batch_autotune(qse_ctx* ctx, int precision, int ncirc, const int* nq, const int64_t* op_off,
+                       const qse_op* ops, const int64_t* par_off, const double* params,
+                       const uint64_t* obs_masks, const double* obs_weights, int nterms, int with_grads,
+                       int* k_out, int* L_out, double* ms_out);
 
 /* ---- batched Pauli-noise trajectories (Noise.cpp semantics) ----
  * For ntraj frames (xmask[t], zmask[t]) applied to the state, draw

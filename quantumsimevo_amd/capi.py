@@ -44,7 +44,7 @@ EXPORTS = [
     "qse_measure", "qse_norm2", "qse_prob_one", "qse_collapse_qubit", "qse_expect_zstrings",
     "qse_adjoint_expect", "qse_batch_expect", "qse_batch_run_small", "qse_sample_frames",
     "qse_compose_dense", "qse_apply_dense", "qse_state_renormalize", "qse_ctx_set_renormalize",
-    "qse_ctx_set_adjoint_terms",
+    "qse_ctx_set_adjoint_terms", "qse_batch_autotune",
 ]
 
 
@@ -122,6 +122,8 @@ def _sigs(lib):
         "qse_state_renormalize": ([_vp, _vp, _vp], _i),
         "qse_ctx_set_renormalize": ([_vp, _i], _i),
         "qse_ctx_set_adjoint_terms": ([_vp, _i], _i),
+        "qse_batch_autotune": ([_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, C.POINTER(_i), C.POINTER(_i),
+                                _pd], _i),
         "qse_apply_dense": ([_vp, _vp, _i, _vp], _i),
         "qse_sample": ([_vp, _vp, _vp, _i64, _vp], _i),
         "qse_measure": ([_vp, _vp, _d, _pu64], _i),
@@ -326,6 +328,21 @@ class Context:
         if not grads:
             return E
         return E, [G[par_off[i]:par_off[i + 1]].copy() for i in range(len(circuits))]
+    def batch_autotune(self, precision, circuits, obs_masks, obs_weights, grads: bool = True) -> dict:
+        """Time the batch engine's tile shapes on these circuits (a sample of a
+        larger batch) and keep the fastest for later batch_run_small calls."""
+        from .circuits import pack_batch
+        nq, op_off, ops, par_off, params = pack_batch(circuits)
+        ops = as_ops(ops) if len(ops) else np.zeros(0, dtype=OP_DTYPE)
+        m = np.ascontiguousarray(obs_masks, dtype=np.uint64)
+        w = np.ascontiguousarray(obs_weights, dtype=np.float64)
+        k, L, ms = C.c_int(0), C.c_int(0), C.c_double(0)
+        check(self.lib.qse_batch_autotune(self.h, precision, len(circuits), _ptr(nq), _ptr(op_off), _ptr(ops),
+                                          _ptr(par_off), _ptr(params) if len(params) else None, _ptr(m), _ptr(w),
+                                          len(m), int(grads), C.byref(k), C.byref(L), C.byref(ms)),
+              "qse_batch_autotune")
+        return {"tile_qubits": k.value, "low_qubits": L.value, "ms": ms.value}
+
     def adjoint_expect(self, nqubits, precision, ops, params, obs_masks, obs_weights, ckpt_every=0):
         a = as_ops(ops)
         P = np.ascontiguousarray(params, dtype=np.float64)

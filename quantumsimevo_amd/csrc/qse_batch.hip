@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <vector>
 
@@ -418,9 +419,12 @@ extern "C" int qse_batch_run_small(qse_ctx* ctx, int precision, int ncirc, const
         !energies || (grads && !par_off))
         return set_error(QSE_ERR_INVALID_ARG, "bad argument");
     QSE_HIP(hipSetDevice(ctx->device));
+    const int pi = precision == QSE_C128 ? 0 : 1;
+    const int kmax = ctx->batch_k[pi] ? ctx->batch_k[pi] : BATCH_KMAX;
+    const int low = ctx->batch_L[pi] ? ctx->batch_L[pi] : (precision == QSE_C128 ? 3 : 4);
     BatchPlan B;
     int rc = build_batch_plan(precision, ncirc, nq, op_off, ops, par_off, params, obs_masks, nterms, grads != nullptr,
-                              BATCH_KMAX, precision == QSE_C128 ? 3 : 4, B);
+                              kmax, low, B);
     if (rc) return rc;
     // circuits of BATCH_TEAM_MIN_QUBITS or more go to teams, largest first
     constexpr int BATCH_TEAM_MIN_QUBITS = 18;
@@ -514,5 +518,61 @@ extern "C" int qse_batch_run_small(qse_ctx* ctx, int precision, int ncirc, const
     if (e != hipSuccess) return fail(hip_error(e, "batch readback"));
     if (herr) return set_error(QSE_ERR_HIP, "batch: a team barrier timed out");
     batch_results(B, vals.data(), par_off, energies, grads);
+    return QSE_OK;
+}
+
+// Measured choice of the batch engine's tile shape: the given circuits (a
+// sample of a larger batch, typically) run once per candidate (tiles of 12,
+// 11 or 10 qubits; 2 to 5 contiguous low qubits), and the fastest candidate
+// is kept in the context for later qse_batch_run_small calls of this
+// precision. Candidates that fail are skipped.
+extern "C" int qse_batch_autotune(qse_ctx* ctx, int precision, int ncirc, const int* nq, const int64_t* op_off,
+                                  const qse_op* ops, const int64_t* par_off, const double* params,
+                                  const uint64_t* obs_masks, const double* obs_weights, int nterms, int with_grads,
+                                  int* k_out, int* L_out, double* ms_out) {
+    if (!ctx || (precision != QSE_C128 && precision != QSE_C64) || ncirc < 1 || !nq || !op_off)
+        return set_error(QSE_ERR_INVALID_ARG, "bad argument");
+    const int pi = precision == QSE_C128 ? 0 : 1;
+    std::vector<double> en(ncirc);
+    std::vector<double> gr(with_grads && par_off ? (size_t)std::max<int64_t>(par_off[ncirc], 1) : 1);
+    double* gp = with_grads && par_off ? gr.data() : nullptr;
+    const int keep_k = ctx->batch_k[pi], keep_L = ctx->batch_L[pi];
+    auto run = [&](int k, int L, double* ms) -> int {
+        ctx->batch_k[pi] = k;
+        ctx->batch_L[pi] = L;
+        const auto t0 = std::chrono::steady_clock::now();
+        const int rc = qse_batch_run_small(ctx, precision, ncirc, nq, op_off, ops, par_off, params, obs_masks,
+                                           obs_weights, nterms, en.data(), gp);
+        *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return rc;
+    };
+    double ms = 0.0;
+    int rc = run(BATCH_KMAX, precision == QSE_C128 ? 3 : 4, &ms);  // warm-up: device buffer and code objects
+    if (rc) {
+        ctx->batch_k[pi] = keep_k;
+        ctx->batch_L[pi] = keep_L;
+        return rc;
+    }
+    int best_k = 0, best_L = 0;
+    double best = 1e300;
+    for (int k = BATCH_KMAX; k >= 10; --k)
+        for (int L = 2; L <= 5 && L <= k - FUSE_REG; ++L) {
+            if (run(k, L, &ms) != QSE_OK) continue;
+            if (ms < best) {
+                best = ms;
+                best_k = k;
+                best_L = L;
+            }
+        }
+    if (!best_k) {
+        ctx->batch_k[pi] = keep_k;
+        ctx->batch_L[pi] = keep_L;
+        return set_error(QSE_ERR_HIP, "batch autotune: no candidate ran");
+    }
+    ctx->batch_k[pi] = best_k;
+    ctx->batch_L[pi] = best_L;
+    if (k_out) *k_out = best_k;
+    if (L_out) *L_out = best_L;
+    if (ms_out) *ms_out = best;
     return QSE_OK;
 }

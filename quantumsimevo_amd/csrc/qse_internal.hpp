@@ -149,6 +149,11 @@ struct qse_ctx {
     size_t pinned_bytes = 0;
     // How the adjoint evaluates X/Y generator terms (QSE_ADJ_TERMS_*).
     int adjoint_terms = 0;
+    // Tile shape of qse_batch_run_small per precision (0: 12-qubit tiles with
+    // 3 (complex128) or 4 (complex64) contiguous low qubits); set by
+    // qse_batch_autotune.
+    int batch_k[2] = {0, 0};
+    int batch_L[2] = {0, 0};
     // Renormalise states after this many passes of qse_run calls (0: never).
     int64_t renorm_every = 0;
     // Fusion shape per precision (tile qubits k, contiguous low qubits L).

@@ -78,3 +78,24 @@ def test_team_circuits_vs_host_adjoint(ctx):
     # deterministic: the same batch twice gives identical values
     E2, G2 = ctx.batch_run_small(capi.C128, circs, masks, weights)
     assert np.array_equal(E, E2) and all(np.array_equal(a, b) for a, b in zip(Gr, G2))
+
+
+@pytest.mark.parametrize("prec,tol_e,tol_g", [(capi.C128, 1e-11, 1e-10), (capi.C64, 1e-4, 1e-3)])
+def test_autotuned_shape_keeps_results(ctx, prec, tol_e, tol_g):
+    # the measured tile shape is one of the candidates and is kept by the context;
+    # results under it agree with the default shape's (and so with the oracle's)
+    circs = G.uneven_ansatz_batch(32, seed=5, n_range=(10, 16), depth_range=(3, 6))
+    masks = [1 << q for q in range(16)]
+    weights = [1.0] * 16
+    E0, G0 = ctx.batch_run_small(prec, circs, masks, weights)
+    tune = ctx.batch_autotune(prec, circs[:16], masks, weights)
+    print(f"batch autotune prec={prec}: {tune}", flush=True)
+    assert 10 <= tune["tile_qubits"] <= 12
+    assert 2 <= tune["low_qubits"] <= min(5, tune["tile_qubits"] - 4)
+    assert tune["ms"] > 0
+    E1, G1 = ctx.batch_run_small(prec, circs, masks, weights)
+    assert np.max(np.abs(E1 - E0)) < tol_e
+    assert max(np.max(np.abs(a - b)) for a, b in zip(G1, G0)) < tol_g
+    n, ops, par = circs[3]
+    zm, zw = G.zsum_observable(n)
+    assert abs(E1[3] - oracle_energy(n, ops, par, zm, zw)) < (1e-11 if prec == capi.C128 else 1e-4)
