@@ -201,8 +201,12 @@ __device__ void batch_pass(cx<R>* s, int n, const FPass& P, const BArgs& A, cx<R
     }
 }
 
+// Inlined at its one call site: there the compiler sees that the state and the
+// program live in global memory, and reads the program's uniform fields with
+// scalar loads (an outlined call takes generic pointers and reads every field
+// with a per-lane flat load).
 template <class R>
-__device__ void batch_pass_k(int k, cx<R>* s, int n, const FPass& P, const BArgs& A, cx<R>* tile, SlotVal* slotv,
+__device__ __attribute__((always_inline)) void batch_pass_k(int k, cx<R>* s, int n, const FPass& P, const BArgs& A, cx<R>* tile, SlotVal* slotv,
                              int r, int T) {
     switch (k) {
         case 8: batch_pass<R, 8>(s, n, P, A, tile, slotv, r, T); break;
@@ -290,7 +294,7 @@ struct BKernelArgs {
 // One circuit's step list by a team (or a single workgroup). False: a team
 // barrier failed.
 template <class R>
-__device__ bool run_circuit(const BCirc& C, Team& tm, const BKernelArgs& K, cx<R>* tile, SlotVal* slotv,
+__device__ __attribute__((always_inline)) bool run_circuit(const BCirc& C, Team& tm, const BKernelArgs& K, cx<R>* tile, SlotVal* slotv,
                             double* red, int* s_flag) {
     const int tid = threadIdx.x, nth = blockDim.x;
     cx<R>* arena = (cx<R>*)K.arena_d;
@@ -305,16 +309,15 @@ __device__ bool run_circuit(const BCirc& C, Team& tm, const BKernelArgs& K, cx<R
         if (kind == BS_BASIS) {
             for (uint64_t i = first; i < N; i += stride) psi[i] = {(R)(i == 0 ? 1 : 0), (R)0};
         } else if (kind == BS_PROG) {
-            for (int p = S.pass_begin; p < S.pass_end; ++p) {
-                const FPass& P = K.prog.passes[p];
-                if (S.target & 1) {
-                    batch_pass_k<R>(C.k, psi, C.n, P, K.prog, tile, slotv, tm.r, tm.T);
-                    if (!team_sync(tm, s_flag)) return false;
-                }
-                if (S.target & 2) {
-                    batch_pass_k<R>(C.k, lam, C.n, P, K.prog, tile, slotv, tm.r, tm.T);
-                    if (!team_sync(tm, s_flag)) return false;
-                }
+            // pass p on psi (target bit 0), then on lambda (bit 1): one call site
+            const int nt = S.pass_end > S.pass_begin ? 2 * (S.pass_end - S.pass_begin) : 0;
+#pragma unroll 1
+            for (int it = 0; it < nt; ++it) {
+                const int which = it & 1;
+                if (!((S.target >> which) & 1)) continue;
+                const FPass& P = K.prog.passes[S.pass_begin + (it >> 1)];
+                batch_pass_k<R>(C.k, which ? lam : psi, C.n, P, K.prog, tile, slotv, tm.r, tm.T);
+                if (!team_sync(tm, s_flag)) return false;
             }
             continue;
         } else if (kind == BS_OBS) {
@@ -380,27 +383,33 @@ __global__ __launch_bounds__(BATCH_THREADS) __attribute__((amdgpu_waves_per_eu(2
     __shared__ double red[BATCH_THREADS / 64 * BATCH_DIAG];
     __shared__ int s_next, s_flag;
     const int tid = threadIdx.x;
-    if (T > 1 && nbig > 0) {
-        Team tm{(int)blockIdx.x / T, (int)blockIdx.x % T, T, team_bar + blockIdx.x / T, 0, err};
-        for (;;) {
+    // Team circuits first (queue 0), then this workgroup alone (queue 1); one
+    // call site of run_circuit serves both phases.
+    bool in_team = T > 1 && nbig > 0;
+    Team tm{(int)blockIdx.x / T, (int)blockIdx.x % T, T, team_bar + blockIdx.x / T, 0, err};
+    if (!in_team) tm = Team{0, 0, 1, nullptr, 0, err};
+    const int first = (T > 1) ? nbig : 0;
+    for (;;) {
+        int qi;
+        if (in_team) {
             if (tm.r == 0 && tid == 0) team_cur[tm.id] = atomicAdd(&qctr[0], 1);
             if (!team_sync(tm, &s_flag)) return;
             if (tid == 0) s_next = __hip_atomic_load(&team_cur[tm.id], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __syncthreads();
-            const int qi = s_next;
-            if (qi >= nbig) break;
-            if (!run_circuit<R>(K.circs[K.order[qi]], tm, K, tile, slotv, red, &s_flag)) return;
+            qi = s_next;
+            if (qi >= nbig) {
+                in_team = false;
+                tm = Team{0, 0, 1, nullptr, 0, err};
+                continue;
+            }
+        } else {
+            __syncthreads();
+            if (tid == 0) s_next = first + atomicAdd(&qctr[1], 1);
+            __syncthreads();
+            qi = s_next;
+            if (qi >= ncirc) break;
         }
-    }
-    Team solo{0, 0, 1, nullptr, 0, err};
-    const int first = (T > 1) ? nbig : 0;
-    for (;;) {
-        __syncthreads();
-        if (tid == 0) s_next = first + atomicAdd(&qctr[1], 1);
-        __syncthreads();
-        const int qi = s_next;
-        if (qi >= ncirc) break;
-        run_circuit<R>(K.circs[K.order[qi]], solo, K, tile, slotv, red, &s_flag);
+        if (!run_circuit<R>(K.circs[K.order[qi]], tm, K, tile, slotv, red, &s_flag)) return;
     }
 }
 
