@@ -44,11 +44,28 @@ struct BArgs {
     const double* __restrict__ alphas;
 };
 
+// The program (passes, stages, ops, terms, factors) is read-only for the whole
+// launch: read through the constant address space, its uniform fields come in
+// with scalar loads. (The host pass of the compiler, which never runs this
+// code, knows no such address space and keeps the pointer as it is.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* kc(const T* p) {
+    return (const __attribute__((address_space(4))) T*)p;
+}
+#else
+template <class T>
+__device__ __forceinline__ const T* kc(const T* p) {
+    return p;
+}
+#endif
+
 // One in-place pass of tile size K over the 2^n amplitudes at s: tiles r, r + T, ...
 // (this workgroup's share when T workgroups work on the circuit).
 template <class R, int K>
-__device__ void batch_pass(cx<R>* s, int n, const FPass& P, const BArgs& A, cx<R>* tile, SlotVal* slotv, int r,
+__device__ void batch_pass(cx<R>* s, int n, const FPass& Pg, const BArgs& A, cx<R>* tile, SlotVal* slotv, int r,
                            int T) {
+    const auto& P = *kc(&Pg);
     constexpr int NT = 1 << (K - 4);
     constexpr int TB = K - 4;
     const int tid = threadIdx.x;
@@ -104,7 +121,7 @@ __device__ void batch_pass(cx<R>* s, int n, const FPass& P, const BArgs& A, cx<R
         }
         __syncthreads();
         for (int sg = 0; sg < P.nstages; ++sg) {
-            const FStage& S = A.stages[P.stage_begin + sg];
+            const auto& S = *kc(A.stages + P.stage_begin + sg);
             if (act) {
                 uint32_t toff = 0;
 #pragma unroll
@@ -119,10 +136,10 @@ __device__ void batch_pass(cx<R>* s, int n, const FPass& P, const BArgs& A, cx<R
                     a[j] = tile[st_off ^ ((j & 1) ? r0 : 0u) ^ ((j & 2) ? r1 : 0u) ^ ((j & 4) ? r2 : 0u) ^
                                 ((j & 8) ? r3 : 0u)];
                 for (uint32_t oi = S.op_begin; oi < S.op_end; ++oi) {
-                    const FOp& op = A.ops[oi];
+                    const FOp op = *kc(A.ops + oi);
                     if (op.type == FOP_MONO) {
                         double gr, gi;
-                        mono_factor<TB>(op, A.terms, A.alphas, tid, toff, base, slotv, gr, gi);
+                        mono_factor<TB>(op, kc(A.terms), kc(A.alphas), tid, toff, base, slotv, gr, gi);
                         const cx<R> g = {(R)gr, (R)gi};
                         const uint32_t rctl = op.rctl;
 #pragma unroll

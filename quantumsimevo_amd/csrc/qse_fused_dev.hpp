@@ -128,15 +128,17 @@ struct SlotVal {
 // Factor of a MONO op for this thread: constant x product over the thread-id
 // bits set in both tid and op.tctl of their factor x product of the terms whose
 // thread/external conditions hold (external ones from the per-tile slots).
-template <int TB>
-__device__ __forceinline__ void mono_factor(const FOp& op, const FTerm* __restrict__ terms,
-                                            const double* __restrict__ alphas, int tid, uint32_t toff,
-                                            uint64_t base, const SlotVal* slotv, double& gr, double& gi) {
+// TermPtr / AlphaPtr: generic pointers, or pointers into the constant address
+// space (uniform reads then become scalar loads).
+template <int TB, class TermPtr, class AlphaPtr>
+__device__ __forceinline__ void mono_factor(const FOp& op, TermPtr __restrict__ terms, AlphaPtr __restrict__ alphas,
+                                            int tid, uint32_t toff, uint64_t base, const SlotVal* slotv, double& gr,
+                                            double& gi) {
     gr = op.m[0];
     gi = op.m[1];
     uint32_t am = op.tctl;
     if (am) {
-        const double* al = alphas + op.alpha_begin;
+        const auto al = alphas + op.alpha_begin;
         while (am) {
             const int t = __builtin_ctz(am);
             am &= am - 1;
@@ -148,7 +150,7 @@ __device__ __forceinline__ void mono_factor(const FOp& op, const FTerm* __restri
         }
     }
     for (uint32_t t = op.term_begin; t < op.term_end; ++t) {
-        const FTerm& T = terms[t];
+        const auto& T = terms[t];
         double dr, di;
         bool on = (toff & T.tmask) == T.tmask;
         if (T.slot >= 0) {
