@@ -347,14 +347,16 @@ int qse_batch_run_small(qse_ctx* ctx, int precision, int ncirc,
                         double* energies, double* grads);
 /* Measured tile shape for qse_batch_run_small (no counterpart in the
  * reference): runs the given circuits once per candidate shape (tiles of 12,
- * 11 or 10 qubits, 2 to 5 contiguous low qubits; pass a sample of a large
- * batch) and keeps the fastest in the context for later batches of this
- * precision. with_grads: time energies plus gradients (par_off needed) or
- * energies only. Outputs the chosen shape and its time (may be null). */
+ * 11 or 10 qubits, 2 to 5 contiguous low qubits, with teams of 16
+ * workgroups), then with the best shape once per team size (4, 8, 32
+ * workgroups per circuit of 18 or more qubits), and keeps the fastest in the
+ * context for later batches of this precision (pass a sample of a large
+ * batch). with_grads: time energies plus gradients (par_off needed) or
+ * energies only. Outputs the chosen shape, team size and time (may be null). */
 int qse_batch_autotune(qse_ctx* ctx, int precision, int ncirc, const int* nq, const int64_t* op_off,
                        const qse_op* ops, const int64_t* par_off, const double* params,
                        const uint64_t* obs_masks, const double* obs_weights, int nterms, int with_grads,
-                       int* k_out, int* L_out, double* ms_out);
+                       int* k_out, int* L_out, int* team_out, double* ms_out);
 
 /* ---- batched Pauli-noise trajectories (Noise.cpp semantics) ----
  * For ntraj frames (xmask[t], zmask[t]) applied to the state, draw

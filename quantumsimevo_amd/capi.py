@@ -123,7 +123,7 @@ def _sigs(lib):
         "qse_ctx_set_renormalize": ([_vp, _i], _i),
         "qse_ctx_set_adjoint_terms": ([_vp, _i], _i),
         "qse_batch_autotune": ([_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, C.POINTER(_i), C.POINTER(_i),
-                                _pd], _i),
+                                C.POINTER(_i), _pd], _i),
         "qse_apply_dense": ([_vp, _vp, _i, _vp], _i),
         "qse_sample": ([_vp, _vp, _vp, _i64, _vp], _i),
         "qse_measure": ([_vp, _vp, _d, _pu64], _i),
@@ -336,12 +336,12 @@ class Context:
         ops = as_ops(ops) if len(ops) else np.zeros(0, dtype=OP_DTYPE)
         m = np.ascontiguousarray(obs_masks, dtype=np.uint64)
         w = np.ascontiguousarray(obs_weights, dtype=np.float64)
-        k, L, ms = C.c_int(0), C.c_int(0), C.c_double(0)
+        k, L, T, ms = C.c_int(0), C.c_int(0), C.c_int(0), C.c_double(0)
         check(self.lib.qse_batch_autotune(self.h, precision, len(circuits), _ptr(nq), _ptr(op_off), _ptr(ops),
                                           _ptr(par_off), _ptr(params) if len(params) else None, _ptr(m), _ptr(w),
-                                          len(m), int(grads), C.byref(k), C.byref(L), C.byref(ms)),
+                                          len(m), int(grads), C.byref(k), C.byref(L), C.byref(T), C.byref(ms)),
               "qse_batch_autotune")
-        return {"tile_qubits": k.value, "low_qubits": L.value, "ms": ms.value}
+        return {"tile_qubits": k.value, "low_qubits": L.value, "team_workgroups": T.value, "ms": ms.value}
 
     def adjoint_expect(self, nqubits, precision, ops, params, obs_masks, obs_weights, ckpt_every=0):
         a = as_ops(ops)

@@ -464,7 +464,7 @@ extern "C" int qse_batch_run_small(qse_ctx* ctx, int precision, int ncirc, const
         &per_cu, precision == QSE_C128 ? (const void*)k_batch<double> : (const void*)k_batch<float>, BATCH_THREADS,
         0));
     int grid = std::max(1, per_cu) * ctx->num_cus;
-    int T = 16;
+    int T = ctx->batch_team[pi] ? ctx->batch_team[pi] : 16;
     if (nbig == 0 || grid < 2 * T) T = 1;
     if (T > 1) grid = grid / T * T;
     grid = std::min(grid, std::max(ncirc, T > 1 ? grid : 1));
@@ -555,17 +555,23 @@ extern "C" int qse_batch_run_small(qse_ctx* ctx, int precision, int ncirc, const
 extern "C" int qse_batch_autotune(qse_ctx* ctx, int precision, int ncirc, const int* nq, const int64_t* op_off,
                                   const qse_op* ops, const int64_t* par_off, const double* params,
                                   const uint64_t* obs_masks, const double* obs_weights, int nterms, int with_grads,
-                                  int* k_out, int* L_out, double* ms_out) {
+                                  int* k_out, int* L_out, int* team_out, double* ms_out) {
     if (!ctx || (precision != QSE_C128 && precision != QSE_C64) || ncirc < 1 || !nq || !op_off)
         return set_error(QSE_ERR_INVALID_ARG, "bad argument");
     const int pi = precision == QSE_C128 ? 0 : 1;
     std::vector<double> en(ncirc);
     std::vector<double> gr(with_grads && par_off ? (size_t)std::max<int64_t>(par_off[ncirc], 1) : 1);
     double* gp = with_grads && par_off ? gr.data() : nullptr;
-    const int keep_k = ctx->batch_k[pi], keep_L = ctx->batch_L[pi];
-    auto run = [&](int k, int L, double* ms) -> int {
+    const int keep_k = ctx->batch_k[pi], keep_L = ctx->batch_L[pi], keep_T = ctx->batch_team[pi];
+    auto restore = [&] {
+        ctx->batch_k[pi] = keep_k;
+        ctx->batch_L[pi] = keep_L;
+        ctx->batch_team[pi] = keep_T;
+    };
+    auto run = [&](int k, int L, int team, double* ms) -> int {
         ctx->batch_k[pi] = k;
         ctx->batch_L[pi] = L;
+        ctx->batch_team[pi] = team;
         const auto t0 = std::chrono::steady_clock::now();
         const int rc = qse_batch_run_small(ctx, precision, ncirc, nq, op_off, ops, par_off, params, obs_masks,
                                            obs_weights, nterms, en.data(), gp);
@@ -573,17 +579,17 @@ extern "C" int qse_batch_autotune(qse_ctx* ctx, int precision, int ncirc, const 
         return rc;
     };
     double ms = 0.0;
-    int rc = run(BATCH_KMAX, precision == QSE_C128 ? 3 : 4, &ms);  // warm-up: device buffer and code objects
+    int rc = run(BATCH_KMAX, precision == QSE_C128 ? 3 : 4, 16, &ms);  // warm-up: device buffer and code objects
     if (rc) {
-        ctx->batch_k[pi] = keep_k;
-        ctx->batch_L[pi] = keep_L;
+        restore();
         return rc;
     }
-    int best_k = 0, best_L = 0;
+    // tile shape with teams of 16, then the team size with that shape
+    int best_k = 0, best_L = 0, best_T = 16;
     double best = 1e300;
     for (int k = BATCH_KMAX; k >= 10; --k)
         for (int L = 2; L <= 5 && L <= k - FUSE_REG; ++L) {
-            if (run(k, L, &ms) != QSE_OK) continue;
+            if (run(k, L, 16, &ms) != QSE_OK) continue;
             if (ms < best) {
                 best = ms;
                 best_k = k;
@@ -591,14 +597,22 @@ extern "C" int qse_batch_autotune(qse_ctx* ctx, int precision, int ncirc, const 
             }
         }
     if (!best_k) {
-        ctx->batch_k[pi] = keep_k;
-        ctx->batch_L[pi] = keep_L;
+        restore();
         return set_error(QSE_ERR_HIP, "batch autotune: no candidate ran");
+    }
+    for (int team : {4, 8, 32}) {
+        if (run(best_k, best_L, team, &ms) != QSE_OK) continue;
+        if (ms < best) {
+            best = ms;
+            best_T = team;
+        }
     }
     ctx->batch_k[pi] = best_k;
     ctx->batch_L[pi] = best_L;
+    ctx->batch_team[pi] = best_T;
     if (k_out) *k_out = best_k;
     if (L_out) *L_out = best_L;
+    if (team_out) *team_out = best_T;
     if (ms_out) *ms_out = best;
     return QSE_OK;
 }

@@ -92,6 +92,7 @@ def test_autotuned_shape_keeps_results(ctx, prec, tol_e, tol_g):
     print(f"batch autotune prec={prec}: {tune}", flush=True)
     assert 10 <= tune["tile_qubits"] <= 12
     assert 2 <= tune["low_qubits"] <= min(5, tune["tile_qubits"] - 4)
+    assert tune["team_workgroups"] in (4, 8, 16, 32)
     assert tune["ms"] > 0
     E1, G1 = ctx.batch_run_small(prec, circs, masks, weights)
     assert np.max(np.abs(E1 - E0)) < tol_e
