@@ -349,7 +349,7 @@ int qse_batch_run_small(qse_ctx* ctx, int precision, int ncirc,
  * reference): runs the given circuits once per candidate shape (tiles of 12,
  * 11 or 10 qubits, 2 to 5 contiguous low qubits, with teams of 16
  * workgroups), then with the best shape once per team size (4, 8, 32
- * workgroups per circuit of 18 or more qubits), and keeps the fastest in the
+ * workgroups per circuit of 16 or more qubits), and keeps the fastest in the
  * context for later batches of this precision (pass a sample of a large
  * batch). with_grads: time energies plus gradients (par_off needed) or
  * energies only. Outputs the chosen shape, team size and time (may be null). */

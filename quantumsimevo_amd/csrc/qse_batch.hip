@@ -6,7 +6,7 @@
 // work) and each circuit's whole step list (qse_batch.hpp) is executed by one
 // team: fused passes over the tiles of the circuit's state in HBM (the
 // LDS-tiled pass of qse_fused.hip, in place), the observable, and FP64
-// reductions for the energy and the generator terms. Circuits of 18 qubits or
+// reductions for the energy and the generator terms. Circuits of 16 qubits or
 // more are taken by teams of 16 workgroups that split every pass's tiles and
 // every reduction's indices and meet at team barriers (global counters); the
 // others by single workgroups, whose own barriers are the only
@@ -453,7 +453,7 @@ extern "C" int qse_batch_run_small(qse_ctx* ctx, int precision, int ncirc, const
                               kmax, low, B);
     if (rc) return rc;
     // circuits of BATCH_TEAM_MIN_QUBITS or more go to teams, largest first
-    constexpr int BATCH_TEAM_MIN_QUBITS = 18;
+    constexpr int BATCH_TEAM_MIN_QUBITS = 16;
     std::stable_partition(B.order.begin(), B.order.end(),
                           [&](int c) { return B.circs[c].n >= BATCH_TEAM_MIN_QUBITS; });
     int nbig = 0;

@@ -154,7 +154,7 @@ struct qse_ctx {
     // qse_batch_autotune.
     int batch_k[2] = {0, 0};
     int batch_L[2] = {0, 0};
-    int batch_team[2] = {0, 0};  // workgroups per team for circuits of 18+ qubits (0: 16)
+    int batch_team[2] = {0, 0};  // workgroups per team for circuits of 16+ qubits (0: 16)
     // Renormalise states after this many passes of qse_run calls (0: never).
     int64_t renorm_every = 0;
     // Fusion shape per precision (tile qubits k, contiguous low qubits L).

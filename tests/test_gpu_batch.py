@@ -63,7 +63,7 @@ def test_batch_rejects_bad_input(ctx):
 
 
 def test_team_circuits_vs_host_adjoint(ctx):
-    # circuits of 18 qubits and more run on teams of workgroups; mixed with small ones
+    # circuits of 16 qubits and more run on teams of workgroups; mixed with small ones
     circs = G.uneven_ansatz_batch(6, seed=21, n_range=(18, 20), depth_range=(3, 5))
     circs += G.uneven_ansatz_batch(10, seed=22, n_range=(10, 14), depth_range=(2, 4))
     masks = [1 << q for q in range(20)]
