@@ -33,6 +33,14 @@ namespace {
 // LDS per workgroup, two workgroups per CU.
 constexpr int BATCH_KMAX = 12;
 constexpr int BATCH_THREADS = 1 << (BATCH_KMAX - 4);
+// External (tile-index) qubits are deposited in groups of BATCH_DEP_BITS from
+// per-pass LDS tables: 4 groups cover the 22 external qubits of a 30-qubit
+// circuit with 8-qubit tiles.
+constexpr int BATCH_DEP_BITS = 6;
+constexpr int BATCH_DEP_LEVELS = 4;
+constexpr int BATCH_DEP_HALF = BATCH_DEP_LEVELS << BATCH_DEP_BITS;
+static_assert(30 - FUSE_MIN_K <= BATCH_DEP_LEVELS * BATCH_DEP_BITS,
+              "the deposit tables cover the external qubits of the largest batch circuit");
 
 struct BArgs {
     const FPass* __restrict__ passes;
@@ -63,8 +71,8 @@ __device__ __forceinline__ const T* kc(const T* p) {
 // One in-place pass of tile size K over the 2^n amplitudes at s: tiles r, r + T, ...
 // (this workgroup's share when T workgroups work on the circuit).
 template <class R, int K>
-__device__ void batch_pass(cx<R>* s, int n, const FPass& Pg, const BArgs& A, cx<R>* tile, SlotVal* slotv, int r,
-                           int T) {
+__device__ void batch_pass(cx<R>* s, int n, const FPass& Pg, const BArgs& A, cx<R>* tile, SlotVal* slotv,
+                           uint64_t* dep, int r, int T) {
     const auto& P = *kc(&Pg);
     constexpr int NT = 1 << (K - 4);
     constexpr int TB = K - 4;
@@ -83,15 +91,31 @@ __device__ void batch_pass(cx<R>* s, int n, const FPass& Pg, const BArgs& A, cx<
     }
     const uint32_t swz_tid = swz((uint32_t)tid), swz_st = swz(st_t);
     const uint64_t ntiles = 1ULL << (n - K);
+    // Tile index -> source / destination base addresses: the external qubits in
+    // groups of 6 tile-index bits, each group's 64 deposits tabulated once per
+    // pass in LDS (dep[0..255]: source, dep[256..511]: destination). The
+    // previous pass's readers have all passed a workgroup barrier.
+    const int nlev = (P.next + BATCH_DEP_BITS - 1) / BATCH_DEP_BITS;
+    for (int idx = tid; idx < nlev << BATCH_DEP_BITS; idx += blockDim.x) {
+        const int lvl = idx >> BATCH_DEP_BITS, j = idx & ((1 << BATCH_DEP_BITS) - 1);
+        uint64_t b = 0, o = 0;
+        for (int u = 0; u < BATCH_DEP_BITS; ++u) {
+            const int e = lvl * BATCH_DEP_BITS + u;
+            if (e < P.next && ((j >> u) & 1)) {
+                b |= 1ULL << P.epos[e];
+                o |= 1ULL << P.eout[e];
+            }
+        }
+        dep[idx] = b;
+        dep[BATCH_DEP_HALF + idx] = o;
+    }
+    __syncthreads();
     for (uint64_t ti = (uint64_t)r; ti < ntiles; ti += (uint64_t)T) {
         uint64_t base = 0, obase = 0;
-        {
-            uint64_t t = ti;
-            for (int e = 0; e < P.next; ++e) {
-                base |= (t & 1ULL) << P.epos[e];
-                obase |= (t & 1ULL) << P.eout[e];
-                t >>= 1;
-            }
+        for (int lvl = 0; lvl < nlev; ++lvl) {
+            const int j = (int)((ti >> (lvl * BATCH_DEP_BITS)) & ((1 << BATCH_DEP_BITS) - 1)) + (lvl << BATCH_DEP_BITS);
+            base |= dep[j];
+            obase |= dep[BATCH_DEP_HALF + j];
         }
         cx<R> ld[16];
         if (act) {
@@ -223,14 +247,14 @@ __device__ void batch_pass(cx<R>* s, int n, const FPass& Pg, const BArgs& A, cx<
 // scalar loads (an outlined call takes generic pointers and reads every field
 // with a per-lane flat load).
 template <class R>
-__device__ __attribute__((always_inline)) void batch_pass_k(int k, cx<R>* s, int n, const FPass& P, const BArgs& A, cx<R>* tile, SlotVal* slotv,
+__device__ __attribute__((always_inline)) void batch_pass_k(int k, cx<R>* s, int n, const FPass& P, const BArgs& A, cx<R>* tile, SlotVal* slotv, uint64_t* dep,
                              int r, int T) {
     switch (k) {
-        case 8: batch_pass<R, 8>(s, n, P, A, tile, slotv, r, T); break;
-        case 9: batch_pass<R, 9>(s, n, P, A, tile, slotv, r, T); break;
-        case 10: batch_pass<R, 10>(s, n, P, A, tile, slotv, r, T); break;
-        case 11: batch_pass<R, 11>(s, n, P, A, tile, slotv, r, T); break;
-        default: batch_pass<R, 12>(s, n, P, A, tile, slotv, r, T); break;
+        case 8: batch_pass<R, 8>(s, n, P, A, tile, slotv, dep, r, T); break;
+        case 9: batch_pass<R, 9>(s, n, P, A, tile, slotv, dep, r, T); break;
+        case 10: batch_pass<R, 10>(s, n, P, A, tile, slotv, dep, r, T); break;
+        case 11: batch_pass<R, 11>(s, n, P, A, tile, slotv, dep, r, T); break;
+        default: batch_pass<R, 12>(s, n, P, A, tile, slotv, dep, r, T); break;
     }
 }
 
@@ -312,7 +336,7 @@ struct BKernelArgs {
 // barrier failed.
 template <class R>
 __device__ __attribute__((always_inline)) bool run_circuit(const BCirc& C, Team& tm, const BKernelArgs& K, cx<R>* tile, SlotVal* slotv,
-                            double* red, int* s_flag) {
+                            uint64_t* dep, double* red, int* s_flag) {
     const int tid = threadIdx.x, nth = blockDim.x;
     cx<R>* arena = (cx<R>*)K.arena_d;
     cx<R>* psi = arena + C.psi_off;
@@ -333,7 +357,7 @@ __device__ __attribute__((always_inline)) bool run_circuit(const BCirc& C, Team&
                 const int which = it & 1;
                 if (!((S.target >> which) & 1)) continue;
                 const FPass& P = K.prog.passes[S.pass_begin + (it >> 1)];
-                batch_pass_k<R>(C.k, which ? lam : psi, C.n, P, K.prog, tile, slotv, tm.r, tm.T);
+                batch_pass_k<R>(C.k, which ? lam : psi, C.n, P, K.prog, tile, slotv, dep, tm.r, tm.T);
                 if (!team_sync(tm, s_flag)) return false;
             }
             continue;
@@ -397,6 +421,7 @@ __global__ __launch_bounds__(BATCH_THREADS) __attribute__((amdgpu_waves_per_eu(2
     BKernelArgs K, int ncirc, int nbig, int T, int* qctr, int* team_bar, int* team_cur, int* err) {
     __shared__ cx<R> tile[1 << BATCH_KMAX];
     __shared__ SlotVal slotv[FUSE_MAX_SLOTS];
+    __shared__ uint64_t dep[2 * BATCH_DEP_HALF];
     __shared__ double red[BATCH_THREADS / 64 * BATCH_DIAG];
     __shared__ int s_next, s_flag;
     const int tid = threadIdx.x;
@@ -426,7 +451,7 @@ __global__ __launch_bounds__(BATCH_THREADS) __attribute__((amdgpu_waves_per_eu(2
             qi = s_next;
             if (qi >= ncirc) break;
         }
-        if (!run_circuit<R>(K.circs[K.order[qi]], tm, K, tile, slotv, red, &s_flag)) return;
+        if (!run_circuit<R>(K.circs[K.order[qi]], tm, K, tile, slotv, dep, red, &s_flag)) return;
     }
 }
 
