@@ -14,6 +14,7 @@
  *   mid-circuit collapse/reset  StateVector.cpp:158-221
  *   noise     QuantumSimEvo/Noise.cpp:3-44, draw order Circuit.cpp:64-120
  *   dispatch  QuantumSimEvo/Circuit.cpp:73-112
+ *   adjoint gradients (no reference counterpart): restated from the mathematics
  * The reference draws its readout randomness from std::random_device
  * (StateVector.cpp:98-99,127-128); here the canonical uniforms are injected so
  * outcomes are reproducible: target = u * total, exactly what
@@ -35,7 +36,10 @@
  *   noise       : std::mt19937 standard known answer (10000th output of the
  *                 default-seeded engine = 4123659995) and flip patterns written
  *                 by the reference's own Noise class (oracle/_ref);
- *   expectation : closed forms on basis/product states, numpy brute force.
+ *   expectation : closed forms on basis/product states, numpy brute force;
+ *   adjoint     : the oracle's own parameter-shift gradients (built on the
+ *                 pinned gate sweeps and expectation) for every parameterised
+ *                 kind, and central finite differences.
  */
 #include <algorithm>
 #include <cmath>
@@ -322,6 +326,92 @@ double orc_expect_zstrings(const double* psi, int n, const uint64_t* masks, cons
         acc += norm2(s[i]) * f;
     }
     return acc;
+}
+
+/* The inverse of one op, applied in place (adjoint sweep below): H, X, Y, Z,
+ * CNOT and SWAP are their own inverses, angle kinds take the negated angle,
+ * a generic 2x2 its conjugate transpose. */
+static int orc_unapply(double* psi, int n, const qse_op& op, const double* params) {
+    qse_op inv = op;
+    inv.param = -1;
+    inv.theta = -op_angle(op, params);
+    if (op.kind == QSE_GATE_MATRIX) {
+        // (G^dagger)_{rc} = conj(G_{cr})
+        const double* m = op.m;
+        const double t[8] = {m[0], -m[1], m[4], -m[5], m[2], -m[3], m[6], -m[7]};
+        std::memcpy(inv.m, t, sizeof t);
+    }
+    return orc_run_ops(psi, n, &inv, 1, nullptr);
+}
+
+/* Adjoint-method gradient (no reference counterpart; restated from the
+ * mathematics as a plain sequential check of the device reverse sweep):
+ * psi = U_L ... U_1 |0>, E = <psi|O|psi> with O = sum_t w_t Z(mask_t),
+ * lambda = O psi. Walking the gates backwards, at the output psi_k of gate k
+ * (lambda_k = U_{k+1}^dag ... U_L^dag O psi) the term of a parameterised gate
+ * is, with U = exp(-i theta G / 2) for RX/RY/RZ (G = X, Y, Z) and
+ * U = exp(i theta P) for PHASE/CPHASE/MCPHASE (P the projector on the pinned
+ * bits being 1):
+ *   dE/dtheta = 2 Re <lambda_k| dU/dtheta U^dag |psi_k>
+ *             = Im <lambda_k|G|psi_k>       (RX, RY, RZ)
+ *             = -2 Im <lambda_k|P|psi_k>     (PHASE, CPHASE, MCPHASE),
+ * added to grads[param] times the op's scale; then gate k is un-applied on
+ * psi and lambda. Returns 0, or -1 for an unsupported kind. */
+int orc_adjoint_expect(int n, const qse_op* ops, int nops, const double* params, int nparams, const uint64_t* masks,
+                       const double* w, int nterms, double* energy, double* grads) {
+    const uint64_t N = 1ULL << n;
+    std::vector<cplx> psi(N, cplx(0.0, 0.0)), lam(N);
+    psi[0] = 1.0;
+    double* ps = reinterpret_cast<double*>(psi.data());
+    double* ls = reinterpret_cast<double*>(lam.data());
+    if (orc_run_ops(ps, n, ops, nops, params) != 0) return -1;
+    double e = 0.0;
+    for (uint64_t i = 0; i < N; ++i) {
+        double f = 0.0;
+        for (int k = 0; k < nterms; ++k) f += (__builtin_popcountll(i & masks[k]) & 1) ? -w[k] : w[k];
+        e += norm2(psi[i]) * f;
+        lam[i] = f * psi[i];
+    }
+    *energy = e;
+    for (int j = 0; j < nparams; ++j) grads[j] = 0.0;
+    const cplx I(0.0, 1.0);
+    for (int k = nops - 1; k >= 0; --k) {
+        const qse_op& op = ops[k];
+        if (op.param >= 0 && op.param < nparams) {
+            const uint64_t m = 1ULL << op.target;
+            cplx z(0.0, 0.0);
+            double d = 0.0;
+            switch (op.kind) {
+                case QSE_GATE_RX:
+                    for (uint64_t i = 0; i < N; ++i) z += std::conj(lam[i]) * psi[i ^ m];
+                    d = z.imag();
+                    break;
+                case QSE_GATE_RY:  // (Y psi)_i = -i psi_{i|m} (bit 0), +i psi_{i&~m} (bit 1)
+                    for (uint64_t i = 0; i < N; ++i) z += std::conj(lam[i]) * ((i & m) ? I : -I) * psi[i ^ m];
+                    d = z.imag();
+                    break;
+                case QSE_GATE_RZ:
+                    for (uint64_t i = 0; i < N; ++i) z += std::conj(lam[i]) * ((i & m) ? -psi[i] : psi[i]);
+                    d = z.imag();
+                    break;
+                case QSE_GATE_PHASE:
+                case QSE_GATE_CPHASE:
+                case QSE_GATE_MCPHASE: {
+                    uint64_t all = m;
+                    if (op.kind == QSE_GATE_CPHASE) all |= 1ULL << op.control;
+                    if (op.kind == QSE_GATE_MCPHASE) all |= op.controls;
+                    for (uint64_t i = 0; i < N; ++i)
+                        if ((i & all) == all) z += std::conj(lam[i]) * psi[i];
+                    d = -2.0 * z.imag();
+                    break;
+                }
+                default: return -1;
+            }
+            grads[op.param] += op.scale * d;
+        }
+        if (orc_unapply(ps, n, op, params) != 0 || orc_unapply(ls, n, op, params) != 0) return -1;
+    }
+    return 0;
 }
 
 /* Pauli frame: X on every bit of xmask, then Z on every bit of zmask

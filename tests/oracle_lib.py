@@ -48,6 +48,8 @@ def oracle():
         lib.orc_norm2.restype = _d
         lib.orc_expect_zstrings.argtypes = [_vp, _i, _vp, _vp, _i]
         lib.orc_expect_zstrings.restype = _d
+        lib.orc_adjoint_expect.argtypes = [_i, _vp, _i, _vp, _i, _vp, _vp, _i, _vp, _vp]
+        lib.orc_adjoint_expect.restype = _i
         lib.orc_noise_create.argtypes = [_i, _u32, _d]
         lib.orc_noise_create.restype = _vp
         lib.orc_noise_destroy.argtypes = [_vp]
@@ -131,6 +133,19 @@ def expect_zstrings(psi, n, masks, weights) -> float:
     m = np.ascontiguousarray(masks, dtype=np.uint64)
     w = np.ascontiguousarray(weights, dtype=np.float64)
     return oracle().orc_expect_zstrings(_p(np.ascontiguousarray(psi, dtype=np.complex128)), n, _p(m), _p(w), len(m))
+
+
+def adjoint_expect(n, ops, params, masks, weights):
+    """Energy and adjoint gradient of sum_t w_t Z(m_t) after ops on |0...0>
+    (plain sequential restatement, orc_adjoint_expect)."""
+    m = np.ascontiguousarray(masks, dtype=np.uint64)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    g = np.zeros(len(p))
+    e = C.c_double(0.0)
+    rc = oracle().orc_adjoint_expect(n, _p(ops), len(ops), _p(p), len(p), _p(m), _p(w), len(m), C.addressof(e), _p(g))
+    assert rc == 0
+    return e.value, g
 
 
 def pauli_frame(psi, n, x, z):

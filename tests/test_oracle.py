@@ -180,7 +180,7 @@ def test_oracle_matches_reference_golden_circuits():
     assert np.max(np.abs(got - GOLD["qft_out"])) < 1e-12
 
 
-@pytest.mark.skipif(O.ref() is None, reason="reference CPU backend (oracle/_ref) not built here")
+@pytest.mark.skipif(not O.REF_PATH.exists(), reason="reference CPU backend (oracle/_ref) not built here")
 def test_oracle_matches_live_reference():
     n = 11
     ops = C.random_layered(n, 12, seed=21).ops()
@@ -296,3 +296,101 @@ def test_product_noise_frames_match_reference_draws():
         x, z = Q.noise_frames(n, ops["kind"], ops["target"], np.arange(40, dtype=np.uint32), rate, mult)
         xo, zo = O.noise_trajectories(n, ops, 0, 40, rate, mult)
         assert np.array_equal(x, xo) and np.array_equal(z, zo)
+
+
+def adjoint_test_circuit(n, seed):
+    """Every parameterised kind (RX, RY, RZ, PHASE, CPHASE, MCPHASE) between
+    fixed H, X, Y, Z, CNOT, SWAP and a generic complex 2x2, some parameters
+    shared by several gates with different scales."""
+    rng = np.random.default_rng(seed)
+    g = C.GateList(n)
+    for q in range(n):
+        g.hadamard(q)
+    p = 0
+    for layer in range(2):
+        for q in range(n):
+            k = (q + layer) % 6
+            sc = float(rng.choice([1.0, -0.5, 2.0]))
+            if k == 0: g.rotate_x(0.2, q, param=p, scale=sc)
+            elif k == 1: g.rotate_y(-0.3, q, param=p, scale=sc)
+            elif k == 2: g.rotate_z(0.1, q, param=p, scale=sc)
+            elif k == 3: g.phase(0.4, q, param=p, scale=sc)
+            elif k == 4: g.cphase(-0.2, q, (q + 2) % n, param=p, scale=sc)
+            else: g._add(C.MCPHASE, q, theta=0.3, param=p, scale=sc, controls=(1 << ((q + 1) % n)) | (1 << ((q + 3) % n)))
+            p += 1
+        g.cnot(0, n - 1); g.rotate_z(0.0, n - 1, param=0, scale=1.5); g.cnot(0, n - 1)
+        g.pauli_x(1); g.pauli_y(2); g.pauli_z(3); g.swap(0, n - 2)
+        a, b = rng.standard_normal(2) + 1j * rng.standard_normal(2)
+        nrm = math.sqrt(abs(a) ** 2 + abs(b) ** 2)
+        a, b = a / nrm, b / nrm
+        g.gate(np.array([[a, -np.conj(b)], [b, np.conj(a)]]), layer + 1)
+        for q in range(layer % 2, n - 1, 2):
+            g.cnot(q, q + 1)
+    return g.ops(), rng.random(p) * 2 * PI
+
+
+def shift_gradient(n, ops, params, masks, w):
+    """Parameter shift over every occurrence: d/dtheta_k = [E(+pi/2) - E(-pi/2)]/2
+    (generators with eigenvalue gap 1), summed per parameter with the op's scale."""
+    g = np.zeros(len(params))
+    for k in np.nonzero(ops["param"] >= 0)[0]:
+        for s in (+1.0, -1.0):
+            sh = ops.copy()
+            sh["theta"][k] += s * PI / 2
+            psi = O.run_ops(O.basis(n), n, sh, params)
+            g[int(ops[k]["param"])] += s * 0.5 * float(ops[k]["scale"]) * O.expect_zstrings(psi, n, masks, w)
+    return g
+
+
+def test_oracle_adjoint_vs_parameter_shift_and_fd():
+    """The oracle's plain adjoint sweep against its own parameter shift (built on
+    the pinned gate sweeps and expectation) and central finite differences."""
+    n = 7
+    ops, params = adjoint_test_circuit(n, 4)
+    masks = np.array([1 << q for q in range(n)] + [0b1010, 0b1100001, 0], dtype=np.uint64)
+    w = np.array([1.0] * n + [0.5, -0.75, 0.3])
+    E, g = O.adjoint_expect(n, ops, params, masks, w)
+    psi = O.run_ops(O.basis(n), n, ops, params)
+    assert abs(E - O.expect_zstrings(psi, n, masks, w)) < 1e-13
+    assert np.max(np.abs(g - shift_gradient(n, ops, params, masks, w))) < 1e-12
+    h = 1e-5
+    for j in range(len(params)):
+        pp, pm = params.copy(), params.copy()
+        pp[j] += h
+        pm[j] -= h
+        fd = (O.expect_zstrings(O.run_ops(O.basis(n), n, ops, pp), n, masks, w) -
+              O.expect_zstrings(O.run_ops(O.basis(n), n, ops, pm), n, masks, w)) / (2 * h)
+        assert abs(g[j] - fd) < 1e-8, j
+    # a planted sign error in one generator term is caught by the shift gradient
+    bad = ops.copy()
+    ry = np.nonzero(bad["kind"] == C.RY)[0][0]
+    bad["scale"][ry] = -bad["scale"][ry]
+    _, gb = O.adjoint_expect(n, bad, params, masks, w)
+    assert np.max(np.abs(gb - shift_gradient(n, ops, params, masks, w))) > 1e-3
+
+
+def test_oracle_adjoint_qaoa_closed_form_p1():
+    """QAOA p=1 Max-Cut on a triangle-free 3-regular graph has a closed form
+    (Wang, Hadfield, Jiang, Rieffel, PRA 97, 022304 (2018), Thm. 1): with
+    U_C = exp(-i g C), U_B = exp(-i b sum X), each edge gives
+    <C_uv> = 1/2 + (1/2) sin(4b) sin(g) cos^2(g). Here the edge gate
+    CNOT.RZ(4 gamma).CNOT = exp(-2i gamma Z_u Z_v) is U_C with g = -4 gamma and
+    RX(2 beta) = exp(-i beta X), so <C> = m (1/2 - (1/2) sin(4 beta) sin(4 gamma)
+    cos^2(4 gamma)); checked together with its derivatives."""
+    n = 8
+    edges = C.random_regular_graph(n, 3, seed=3)
+    es = set(edges)
+    tri = any((a, b) in es and (b, c) in es and (a, c) in es
+              for a in range(n) for b in range(a + 1, n) for c in range(b + 1, n))
+    assert not tri, "the seeded graph must be triangle-free for this closed form"
+    ops = C.qaoa_maxcut(n, edges, 1).ops()
+    masks, w = C.maxcut_observable(edges)
+
+    def closed(gm, bt):
+        return len(edges) * (0.5 - 0.5 * math.sin(4 * bt) * math.sin(4 * gm) * math.cos(4 * gm) ** 2)
+    h = 1e-6
+    for gam, bet in ((0.37, 0.91), (1.1, 0.2), (2.5, 2.9)):
+        E, g = O.adjoint_expect(n, ops, np.array([gam, bet]), masks, w)
+        assert abs(E - closed(gam, bet)) < 1e-12
+        assert abs(g[0] - (closed(gam + h, bet) - closed(gam - h, bet)) / (2 * h)) < 1e-7
+        assert abs(g[1] - (closed(gam, bet + h) - closed(gam, bet - h)) / (2 * h)) < 1e-7
