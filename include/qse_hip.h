@@ -294,6 +294,18 @@ int qse_collapse_qubit(qse_ctx* ctx, qse_state* st, int q, double u, int reset, 
  * sum_i |a_i|^2 f(i) with f(i) = sum_k w_k * (-1)^popcount(i & masks[k])
  * (a weighted sum of Z-strings). */
 int qse_expect_zstrings(qse_ctx* ctx, qse_state* st, const uint64_t* masks, const double* weights, int nterms, double* out);
+/* The same value, computed without allocating any memory (no counterpart in
+ * the reference): for callers that cannot allocate, such as tight serving
+ * loops, code under HIP graph capture and harnesses that forbid hipMalloc. The
+ * kernels and summation order are those of qse_expect_zstrings (bit-identical
+ * results); masks, weights, partial sums and the result go through a fixed
+ * 32 KiB device array set up per device at context creation. nterms <= 64
+ * (QSE_ERR_LIMIT above). The array is shared by every call on one GPU and keeps
+ * the last call's contents: calls that overlap on one GPU (several host
+ * threads, or several contexts of one device driven concurrently) race.
+ * Synchronises the context stream. */
+int qse_expect_zstrings_noalloc(qse_ctx* ctx, qse_state* st, const uint64_t* masks, const double* weights, int nterms,
+                                double* out);
 
 /* ---- adjoint-method gradients (no counterpart in the reference) ----
  * For `batch` parameter sets params[b*nparams .. +nparams), evolve |0...0> by

@@ -41,7 +41,7 @@ EXPORTS = [
     "qse_apply_hadamard", "qse_apply_x", "qse_apply_y", "qse_apply_z", "qse_apply_phase", "qse_apply_rx",
     "qse_apply_ry", "qse_apply_rz", "qse_apply_cnot", "qse_apply_cphase", "qse_apply_mcphase",
     "qse_apply_swap", "qse_pauli_frame", "qse_run", "qse_run_repeat", "qse_run_sweeps", "qse_plan_passes", "qse_plan_info", "qse_qft_probe", "qse_copy_bandwidth", "qse_sample",
-    "qse_measure", "qse_norm2", "qse_prob_one", "qse_collapse_qubit", "qse_expect_zstrings",
+    "qse_measure", "qse_norm2", "qse_prob_one", "qse_collapse_qubit", "qse_expect_zstrings", "qse_expect_zstrings_noalloc",
     "qse_adjoint_expect", "qse_batch_expect", "qse_batch_run_small", "qse_sample_frames",
     "qse_compose_dense", "qse_apply_dense", "qse_state_renormalize", "qse_ctx_set_renormalize",
     "qse_ctx_set_adjoint_terms", "qse_batch_autotune",
@@ -131,6 +131,7 @@ def _sigs(lib):
         "qse_prob_one": ([_vp, _vp, _i, _pd], _i),
         "qse_collapse_qubit": ([_vp, _vp, _i, _d, _i, C.POINTER(_i)], _i),
         "qse_expect_zstrings": ([_vp, _vp, _vp, _vp, _i, _pd], _i),
+        "qse_expect_zstrings_noalloc": ([_vp, _vp, _vp, _vp, _i, _pd], _i),
         "qse_adjoint_expect": ([_vp, _i, _i, _vp, _i, _vp, _i, _i, _vp, _vp, _i, _i, _vp, _vp], _i),
         "qse_batch_expect": ([_vp, _i, _i, _vp, _i, _vp, _i, _i, _vp, _vp, _i, _vp], _i),
         "qse_batch_run_small": ([_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp], _i),
@@ -523,4 +524,14 @@ class State:
         v = C.c_double(0)
         check(self.lib.qse_expect_zstrings(self.ctx.h, self.h, _ptr(m), _ptr(w), len(m), C.byref(v)),
               "qse_expect_zstrings")
+        return v.value
+
+    def expect_zstrings_noalloc(self, masks, weights) -> float:
+        """expect_zstrings through the allocation-free entry point (at most 64
+        terms; a fixed per-device scratch array instead of allocated buffers)."""
+        m = np.ascontiguousarray(masks, dtype=np.uint64)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        v = C.c_double(0)
+        check(self.lib.qse_expect_zstrings_noalloc(self.ctx.h, self.h, _ptr(m), _ptr(w), len(m), C.byref(v)),
+              "qse_expect_zstrings_noalloc")
         return v.value

@@ -19,6 +19,9 @@ int sample_frames(qse_ctx* ctx, qse_state* st, const uint64_t* xmask, int ntraj,
 int reduce(qse_ctx* ctx, const qse_state* st, int mode, int q, const uint64_t* masks, const double* w, int nterms,
            double* out);
 int launch_collapse(qse_ctx* ctx, qse_state* st, int q, int outcome, int reset, double nf);
+int noalloc_init(int device);
+int reduce_zstr_noalloc(qse_ctx* ctx, const qse_state* st, const uint64_t* masks, const double* w, int nterms,
+                        double* out);
 void jit_release_all(qse_ctx* ctx);
 
 static thread_local std::string g_last_error;
@@ -203,6 +206,12 @@ int qse_ctx_create(int device, qse_ctx** out) {
     if (hipGetDeviceProperties(&p, device) == hipSuccess) {
         ctx->num_cus = p.multiProcessorCount;
         ctx->lds_per_block = p.sharedMemPerBlock;
+    }
+    // Scratch address of the allocation-free entry point on this device.
+    int rc = noalloc_init(device);
+    if (rc) {
+        delete ctx;
+        return rc;
     }
     // Default-priority stream (no priority variants).
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -632,6 +641,15 @@ int qse_expect_zstrings(qse_ctx* ctx, qse_state* st, const uint64_t* masks, cons
     if (nterms < 0 || (nterms > 0 && (!masks || !weights)) || !out)
         return set_error(QSE_ERR_INVALID_ARG, "bad observable");
     return reduce(ctx, st, 2, 0, masks, weights, nterms, out);
+}
+
+int qse_expect_zstrings_noalloc(qse_ctx* ctx, qse_state* st, const uint64_t* masks, const double* weights, int nterms,
+                                double* out) {
+    int rc = check_state(ctx, st);
+    if (rc) return rc;
+    if (nterms < 0 || (nterms > 0 && (!masks || !weights)) || !out)
+        return set_error(QSE_ERR_INVALID_ARG, "bad observable");
+    return reduce_zstr_noalloc(ctx, st, masks, weights, nterms, out);
 }
 
 }  // extern "C"

@@ -23,6 +23,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace qse {
@@ -397,6 +399,75 @@ int reduce(qse_ctx* ctx, const qse_state* st, int mode, int q, const uint64_t* m
            double* out) {
     if (st->prec == QSE_C128) return reduce_t<double>(ctx, st, mode, q, masks, w, nterms, out);
     return reduce_t<float>(ctx, st, mode, q, masks, w, nterms, out);
+}
+
+}  // namespace qse
+
+// ---- allocation-free expectation values (qse_expect_zstrings_noalloc) ----
+// The scratch memory of that entry point: masks (64 x 8 B) | weights
+// (64 x 8 B) | per-workgroup partial sums (at most 2048 x 8 B) | result. One
+// copy of the array lives on every GPU the library runs on, until the process
+// ends. In host code its name is only a stand-in, not a device address:
+// noalloc_init() looks the address up once per device (with that device
+// current) and host code uses that address only.
+static constexpr int NOALLOC_MAX_TERMS = 64;
+static constexpr unsigned NOALLOC_MAX_PARTS = 2048;
+static constexpr size_t NOALLOC_BYTES = 32768;
+static_assert(8 * (2 * NOALLOC_MAX_TERMS + NOALLOC_MAX_PARTS + 1) <= NOALLOC_BYTES, "scratch layout");
+static __device__ __align__(256) unsigned char noalloc_scratch[NOALLOC_BYTES];
+
+namespace qse {
+
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+static std::mutex g_noalloc_mu;
+
+// Called by qse_ctx_create with `device` current.
+int noalloc_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return set_error(QSE_ERR_LIMIT, "device index above the library's per-device table");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return QSE_OK;
+    void* p = nullptr;
+    QSE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)));
+    if (!p) return set_error(QSE_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_noalloc_ptr[device] = p;
+    return QSE_OK;
+}
+
+// reduce(RED_ZSTR) without allocating: the same kernels, grid and summation
+// order (so bit-identical results), with masks, weights, partial sums and the
+// result in the device's scratch array instead of the context's grown scratch
+// and pinned buffers, and without the timing hooks (which grow host lists).
+int reduce_zstr_noalloc(qse_ctx* ctx, const qse_state* st, const uint64_t* masks_h, const double* w_h, int nterms,
+                        double* out_h) {
+    if (nterms > NOALLOC_MAX_TERMS)
+        return set_error(QSE_ERR_LIMIT, "qse_expect_zstrings_noalloc: at most 64 Z-string terms (got " +
+                                            std::to_string(nterms) + ")");
+    void* base = (ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES) ? g_noalloc_ptr[ctx->device] : nullptr;
+    if (!base) return set_error(QSE_ERR_HIP, "scratch array of this device was not set up at context creation");
+    const uint64_t N = 1ULL << st->n;
+    const unsigned g = std::min<unsigned>(grid_for(ctx, N, RB_THREADS, 8), NOALLOC_MAX_PARTS);
+    uint64_t* d_m = (uint64_t*)base;
+    double* d_w = (double*)(d_m + NOALLOC_MAX_TERMS);
+    double* d_part = d_w + NOALLOC_MAX_TERMS;
+    double* d_out = d_part + NOALLOC_MAX_PARTS;
+    if (nterms > 0) {
+        QSE_HIP(hipMemcpyAsync(d_m, masks_h, sizeof(uint64_t) * nterms, hipMemcpyHostToDevice, ctx->stream));
+        QSE_HIP(hipMemcpyAsync(d_w, w_h, sizeof(double) * nterms, hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (st->prec == QSE_C128)
+        k_reduce<double, RED_ZSTR><<<g, RB_THREADS, 0, ctx->stream>>>((const cx<double>*)st->d, N, 0, d_m, d_w,
+                                                                       nterms, d_part);
+    else
+        k_reduce<float, RED_ZSTR><<<g, RB_THREADS, 0, ctx->stream>>>((const cx<float>*)st->d, N, 0, d_m, d_w,
+                                                                      nterms, d_part);
+    QSE_HIP(hipGetLastError());
+    k_sum_partials<<<1, 64, 0, ctx->stream>>>(d_part, (int)g, d_out);
+    QSE_HIP(hipGetLastError());
+    QSE_HIP(hipMemcpyAsync(out_h, d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    QSE_HIP(hipStreamSynchronize(ctx->stream));
+    return QSE_OK;
 }
 
 // sum_k w_k <Z_{m_k}> with masks and weights already on the device, written to
