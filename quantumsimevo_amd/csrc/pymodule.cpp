@@ -2,7 +2,10 @@
 // (QSEPythonModule/QuantumSimEvoPy.cpp:8-41: Circuit with snake_case gate
 // methods, execute/measure/sample/reset/print_state) on the HIP backend, plus
 // state I/O, expectation values and seeded readout.
+#include <algorithm>
 #include <cstring>
+#include <thread>
+#include <vector>
 #include <pybind11/complex.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -79,6 +82,9 @@ PYBIND11_MODULE(_quantumsimevo, m) {
         .def("expectation_z", &Circuit::expectationZ, py::arg("masks"), py::arg("weights"))
         .def("measure_qubit", &Circuit::measureQubit, py::arg("q"))
         .def("reset_qubit", &Circuit::resetQubit, py::arg("q"))
+        .def("queue_measure", &Circuit::queueMeasure, py::arg("q"))
+        .def("queue_reset", &Circuit::queueReset, py::arg("q"))
+        .def_property_readonly("measurement_results", &Circuit::measurementResults)
         .def_property_readonly("num_qubits", &Circuit::numQubits)
         .def_property_readonly("num_commands", [](const Circuit& c) { return c.commands().size(); })
         .def("noise_stats", [](Circuit& c) {
@@ -143,16 +149,25 @@ PYBIND11_MODULE(_quantumsimevo, m) {
               uint64_t* z = zs.mutable_data();
               {
                   py::gil_scoped_release nogil;
-                  for (ssize_t j = 0; j < nt; ++j) {
-                      Noise nz(n, sd[j], multiplier);
-                      if (rate >= 0.0)
-                          for (GateType g : {GateType::X, GateType::Y, GateType::Z, GateType::H, GateType::CNOT,
-                                             GateType::PHASE, GateType::RX, GateType::RY, GateType::RZ})
-                              nz.setRate(g, rate);
-                      for (ssize_t i = 0; i < ng; ++i) nz.accumulateError((GateType)k[i], (size_t)t[i]);
-                      x[j] = nz.bitFlipMask();
-                      z[j] = nz.phaseFlipMask();
-                  }
+                  // trajectories are independent draw processes: split them over host threads
+                  auto work = [&](ssize_t j0, ssize_t j1) {
+                      for (ssize_t j = j0; j < j1; ++j) {
+                          Noise nz(n, sd[j], multiplier);
+                          if (rate >= 0.0)
+                              for (GateType g : {GateType::X, GateType::Y, GateType::Z, GateType::H, GateType::CNOT,
+                                                 GateType::PHASE, GateType::RX, GateType::RY, GateType::RZ})
+                                  nz.setRate(g, rate);
+                          for (ssize_t i = 0; i < ng; ++i) nz.accumulateError((GateType)k[i], (size_t)t[i]);
+                          x[j] = nz.bitFlipMask();
+                          z[j] = nz.phaseFlipMask();
+                      }
+                  };
+                  const ssize_t nthr = std::min<ssize_t>(
+                      {nt / 16 + 1, (ssize_t)std::max(1u, std::thread::hardware_concurrency()), (ssize_t)16});
+                  std::vector<std::thread> pool;
+                  for (ssize_t w = 1; w < nthr; ++w) pool.emplace_back(work, nt * w / nthr, nt * (w + 1) / nthr);
+                  work(0, nt / nthr);
+                  for (auto& th : pool) th.join();
               }
               return py::make_tuple(xs, zs);
           },

@@ -11,11 +11,14 @@
 //      the walk's values at j-1 and j sit outside the band around the target,
 //      taking into account that both the device and the sequential sums carry
 //      at most (index + depth) unit roundoffs of relative error;
-//   2. shots inside the band are re-resolved by one workgroup that reproduces
-//      the sequential sum exactly (same order, same rounding) over the whole
-//      state: first the sequential total, then one walk that hands out the
-//      flagged (sorted) targets as the running sum crosses them.
-// Outcomes therefore equal the sequential reference on identical amplitudes.
+//   2. shots inside the band are re-resolved exactly: one workgroup per frame
+//      reproduces the sequential sum (same order, same rounding) once over the
+//      whole state and keeps it every 64 amplitudes; a shot then finds its
+//      group among these checkpoints and replays at most 64 additions.
+// When the band would catch about one shot per trajectory or more (large
+// states, many shots), stage 1 is skipped and every shot is resolved from the
+// checkpoints. Outcomes therefore equal the sequential reference on identical
+// amplitudes.
 // Pauli frames (noise trajectories) are sampled the same way over the framed
 // order psi[i ^ x] without materialising the framed state.
 #include "qse_device.hpp"
@@ -147,80 +150,196 @@ __global__ __launch_bounds__(RB_THREADS) void k_shots(const cx<R>* __restrict__ 
     }
 }
 
-// Exact sequential resolution for flagged shots. One workgroup per listed
-// trajectory: pass 1 reproduces the sequential total, pass 2 walks the
-// sequential running sum and hands out the (ascending) targets.
-//   traj[w]: trajectory of work item w; [first[w], first[w+1]) its flagged
-//   shots in `ulist` (ascending u) with output slots `slot`.
+// ---- exact sequential cumulative sums, from checkpoints ----
+// One workgroup per frame f reproduces the reference's cumulative array over
+// the framed order i -> psi[i ^ x_f] (StateVector.cpp:84-90: cum[0] = p_0,
+// cum[i] = cum[i-1] + p_i, FP64, rounding for rounding) and stores it at the
+// end of every G-amplitude group: ck[f][g] = cum[g*G + G - 1], so the last
+// entry is the total. The sum is a serial chain: wave 0 adds, every lane the
+// same values (lane 0 stores), while waves 1-3 compute the probabilities of
+// the next chunk into the other LDS buffer.
+static constexpr uint64_t CK_G = 64;    // amplitudes per checkpoint
+static constexpr int CH_CHUNK = 2048;   // probabilities per LDS buffer
+
 template <class R>
-__global__ __launch_bounds__(RB_THREADS) void k_exact_resolve(const cx<R>* __restrict__ s, uint64_t N,
-                                                              const uint64_t* __restrict__ xmasks,
-                                                              const int* __restrict__ traj,
-                                                              const int64_t* __restrict__ first,
-                                                              const double* __restrict__ ulist,
-                                                              const int64_t* __restrict__ slot,
-                                                              uint64_t* __restrict__ out) {
-    constexpr int CH = 4096;
-    __shared__ double p[CH];
-    __shared__ double total_sh;
-    __shared__ int64_t ksh;
-    const int w = blockIdx.x;
-    const uint64_t x = xmasks ? xmasks[traj[w]] : 0;
-    const int64_t f0 = first[w], f1 = first[w + 1];
-    // pass 1: sequential total
-    double run = 0.0;
-    for (uint64_t base = 0; base < N; base += CH) {
-        const uint64_t len = std::min<uint64_t>(CH, N - base);
-        __syncthreads();
-        for (uint64_t i = threadIdx.x; i < len; i += RB_THREADS) p[i] = prob(s[(base + i) ^ x]);
-        __syncthreads();
-        if (threadIdx.x == 0)
-            for (uint64_t i = 0; i < len; ++i) run += p[i];
+__global__ __launch_bounds__(256) void k_chain(const cx<R>* __restrict__ s, uint64_t N, uint64_t G,
+                                               const uint64_t* __restrict__ xs, double* __restrict__ ck) {
+    __shared__ double buf[2][CH_CHUNK];
+    const uint64_t x = xs[blockIdx.x];
+    const uint64_t ng = N / G;
+    double* out = ck + (uint64_t)blockIdx.x * ng;
+    const uint64_t nchunks = (N + CH_CHUNK - 1) / CH_CHUNK;
+    const int wave = threadIdx.x >> 6;
+    {
+        const uint64_t len = std::min<uint64_t>(CH_CHUNK, N);
+        for (uint64_t i = threadIdx.x; i < len; i += 256) buf[0][i] = prob(s[i ^ x]);
     }
-    if (threadIdx.x == 0) total_sh = run;
     __syncthreads();
-    const double T = total_sh;
-    // pass 2: walk and hand out targets in ascending order
-    int64_t k = f0;
-    double c = 0.0;
-    for (uint64_t base = 0; base < N && k < f1; base += CH) {
-        const uint64_t len = std::min<uint64_t>(CH, N - base);
+    double run = 0.0;
+    for (uint64_t c = 0; c < nchunks; ++c) {
+        const int b = (int)(c & 1);
+        const uint64_t base = c * CH_CHUNK;
+        const uint64_t len = std::min<uint64_t>(CH_CHUNK, N - base);
+        if (wave == 0) {
+            if (G == CK_G && len == (uint64_t)CH_CHUNK) {
+                for (int g0 = 0; g0 < CH_CHUNK; g0 += (int)CK_G) {
+                    double v[CK_G];
+#pragma unroll
+                    for (int i = 0; i < (int)CK_G; ++i) v[i] = buf[b][g0 + i];
+#pragma unroll
+                    for (int i = 0; i < (int)CK_G; ++i) run += v[i];
+                    if (threadIdx.x == 0) out[(base + g0) / CK_G] = run;
+                }
+            } else {
+                for (uint64_t i = 0; i < len; ++i) {
+                    run += buf[b][i];
+                    if ((base + i + 1) % G == 0 && threadIdx.x == 0) out[(base + i) / G] = run;
+                }
+            }
+        } else if (c + 1 < nchunks) {
+            const uint64_t nbase = base + CH_CHUNK;
+            const uint64_t nlen = std::min<uint64_t>(CH_CHUNK, N - nbase);
+            for (uint64_t i = threadIdx.x - 64; i < nlen; i += 192) buf[b ^ 1][i] = prob(s[(nbase + i) ^ x]);
+        }
         __syncthreads();
-        for (uint64_t i = threadIdx.x; i < len; i += RB_THREADS) p[i] = prob(s[(base + i) ^ x]);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (uint64_t i = 0; i < len && k < f1; ++i) {
-                c += p[i];
-                while (k < f1 && c >= ulist[k] * T) {
-                    out[slot[k]] = base + i;
-                    ++k;
+    }
+}
+
+// Exact outcome of a shot from its frame's checkpoints: target = u * total,
+// the first group whose checkpoint reaches the target (binary search), then
+// the sequential sum replayed from the previous checkpoint through that group
+// (the same additions in the same order as k_chain, so the same values): the
+// first index with cum >= target, i.e. std::lower_bound over the sequential
+// cumulative array (StateVector.cpp:94-107). Shots are all of them
+// (list == nullptr, out[g]) or the listed ones (out[k] for list[k]).
+template <class R>
+__global__ __launch_bounds__(RB_THREADS) void k_exact_shots(const cx<R>* __restrict__ s, uint64_t N, uint64_t G,
+                                                            const uint64_t* __restrict__ xs,
+                                                            const int* __restrict__ fidx,
+                                                            const double* __restrict__ ck,
+                                                            const double* __restrict__ u, int64_t nshots,
+                                                            uint64_t nlist, const int64_t* __restrict__ list,
+                                                            uint64_t* __restrict__ out) {
+    const uint64_t ng = N / G;
+    for (uint64_t k = gtid(); k < nlist; k += gstride()) {
+        const uint64_t g = list ? (uint64_t)list[k] : k;
+        const int f = fidx[g / (uint64_t)nshots];
+        const uint64_t x = xs[f];
+        const double* C = ck + (uint64_t)f * ng;
+        const double target = u[g] * C[ng - 1];
+        uint64_t lo = 0, hi = ng - 1;  // C[ng-1] = total >= u * total
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (C[mid] >= target) hi = mid;
+            else lo = mid + 1;
+        }
+        double run = lo ? C[lo - 1] : 0.0;
+        const uint64_t b0 = lo * G;
+        uint64_t j = b0 + G - 1;
+        bool found = false;
+        for (uint64_t i0 = 0; i0 < G && !found; i0 += 16) {
+            double pr[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) pr[c] = (i0 + c < G) ? prob(s[(b0 + i0 + c) ^ x]) : 0.0;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                if (!found && i0 + c < G) {
+                    run += pr[c];
+                    if (run >= target) {
+                        j = b0 + i0 + c;
+                        found = true;
+                    }
                 }
             }
         }
-        // broadcast k so every thread leaves the loop together
-        if (threadIdx.x == 0) ksh = k;
-        __syncthreads();
-        k = ksh;
+        out[list ? k : g] = j;
     }
-    if (threadIdx.x == 0)
-        for (; k < f1; ++k) out[slot[k]] = N - 1;  // unreachable: u*T <= T
 }
 
+// Checkpoint chains for frames xs_d[0..nf) into ck, then the exact outcomes
+// of the listed shots (or of all `count` shots when list is null).
+template <class R>
+static int exact_from_chains(qse_ctx* ctx, const qse_state* st, const uint64_t* xs_d, int nf, const int* fidx_d,
+                             double* ck_d, const double* u_d, int64_t nshots, uint64_t count, const int64_t* list_d,
+                             uint64_t* out_d) {
+    const uint64_t N = 1ULL << st->n;
+    const uint64_t G = std::min<uint64_t>(CK_G, N);
+    const cx<R>* s = (const cx<R>*)st->d;
+    TimedLaunch tl;
+    launch_begin(ctx, FAM_READOUT, &tl);
+    k_chain<R><<<nf, 256, 0, ctx->stream>>>(s, N, G, xs_d, ck_d);
+    QSE_HIP(hipGetLastError());
+    launch_end(ctx, &tl, (double)amp_bytes(st->prec) * N * nf);
+    TimedLaunch t2;
+    launch_begin(ctx, FAM_READOUT, &t2);
+    k_exact_shots<R><<<grid_for(ctx, count, RB_THREADS, 8), RB_THREADS, 0, ctx->stream>>>(
+        s, N, G, xs_d, fidx_d, ck_d, u_d, nshots, count, list_d, out_d);
+    QSE_HIP(hipGetLastError());
+    launch_end(ctx, &t2, (double)amp_bytes(st->prec) * G * count);
+    return QSE_OK;
+}
+
+// Sampling driver. Frames are grouped (trajectories with equal X masks share
+// one framed order). When the blocked CDF would leave about one shot per
+// trajectory or more inside its rounding band (shots * 2 N (N + depth) * u'
+// >= 1, u' = 2^-52), every shot is resolved exactly from checkpoint chains of
+// all frames at once; otherwise the blocked CDF resolves the shots and chains
+// are built only for the frames of the flagged ones.
 template <class R>
 static int sample_frames_t(qse_ctx* ctx, qse_state* st, const uint64_t* xmask_h, int ntraj, const double* u_h,
                            int64_t nshots, uint64_t* out_h) {
     const uint64_t N = 1ULL << st->n;
     const uint64_t B = std::min<uint64_t>(1024, N);
     const uint64_t nb = N / B;
+    const uint64_t G = std::min<uint64_t>(CK_G, N);
+    const uint64_t ng = N / G;
     const uint64_t total = (uint64_t)nshots * ntraj;
-    // scratch layout (slot 0): xmasks | bsum | pre | u | out | flags
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    // distinct frames
+    std::vector<uint64_t> xs;
+    std::vector<int> fidx(ntraj, 0);
+    if (xmask_h) {
+        xs.assign(xmask_h, xmask_h + ntraj);
+        std::sort(xs.begin(), xs.end());
+        xs.erase(std::unique(xs.begin(), xs.end()), xs.end());
+        for (int t = 0; t < ntraj; ++t)
+            fidx[t] = (int)(std::lower_bound(xs.begin(), xs.end(), xmask_h[t]) - xs.begin());
+    } else {
+        xs.assign(1, 0);
+    }
+    const int F = (int)xs.size();
+    const double depth = (double)(B + nb + 64);
+    const double flag_rate = 2.0 * ((double)N + depth) * UR * (double)N;
+    const bool direct = (double)nshots * flag_rate >= 1.0 && (double)F * ng * 8.0 <= (double)(1ULL << 30);
+    const cx<R>* s = (const cx<R>*)st->d;
+    const size_t sz_u = sizeof(double) * (size_t)total;
+    const size_t sz_o = sizeof(uint64_t) * (size_t)total;
+    if (direct) {
+        // scratch layout (slot 0): xs | fidx | checkpoints | u | out
+        const size_t sz_x = sizeof(uint64_t) * F, sz_fi = sizeof(int) * ntraj, sz_ck = sizeof(double) * F * ng;
+        void* base;
+        int rc = scratch(ctx, 0, al(sz_x) + al(sz_fi) + al(sz_ck) + al(sz_u) + al(sz_o), &base);
+        if (rc) return rc;
+        char* p = (char*)base;
+        uint64_t* d_x = (uint64_t*)p; p += al(sz_x);
+        int* d_fi = (int*)p; p += al(sz_fi);
+        double* d_ck = (double*)p; p += al(sz_ck);
+        double* d_u = (double*)p; p += al(sz_u);
+        uint64_t* d_o = (uint64_t*)p;
+        QSE_HIP(hipMemcpyAsync(d_x, xs.data(), sz_x, hipMemcpyHostToDevice, ctx->stream));
+        QSE_HIP(hipMemcpyAsync(d_fi, fidx.data(), sz_fi, hipMemcpyHostToDevice, ctx->stream));
+        QSE_HIP(hipMemcpyAsync(d_u, u_h, sz_u, hipMemcpyHostToDevice, ctx->stream));
+        rc = exact_from_chains<R>(ctx, st, d_x, F, d_fi, d_ck, d_u, nshots, total, nullptr, d_o);
+        if (rc) return rc;
+        QSE_HIP(hipMemcpyAsync(out_h, d_o, sz_o, hipMemcpyDeviceToHost, ctx->stream));
+        QSE_HIP(hipStreamSynchronize(ctx->stream));
+        return QSE_OK;
+    }
+    // blocked CDF; scratch layout (slot 0): xmasks | bsum | pre | u | out | flags
     const size_t sz_x = sizeof(uint64_t) * (size_t)ntraj;
     const size_t sz_b = sizeof(double) * (size_t)(nb * ntraj);
     const size_t sz_p = sizeof(double) * (size_t)((nb + 1) * ntraj);
-    const size_t sz_u = sizeof(double) * (size_t)total;
-    const size_t sz_o = sizeof(uint64_t) * (size_t)total;
     const size_t sz_f = (size_t)total;
-    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
     const size_t need = al(sz_x) + al(sz_b) + al(sz_p) + al(sz_u) + al(sz_o) + al(sz_f);
     void* base;
     int rc = scratch(ctx, 0, need, &base);
@@ -235,7 +354,6 @@ static int sample_frames_t(qse_ctx* ctx, qse_state* st, const uint64_t* xmask_h,
     const bool framed = (xmask_h != nullptr);
     if (framed) QSE_HIP(hipMemcpyAsync(d_x, xmask_h, sz_x, hipMemcpyHostToDevice, ctx->stream));
     QSE_HIP(hipMemcpyAsync(d_u, u_h, sz_u, hipMemcpyHostToDevice, ctx->stream));
-    const cx<R>* s = (const cx<R>*)st->d;
     TimedLaunch tl;
     launch_begin(ctx, FAM_READOUT, &tl);
     k_block_sums<R><<<dim3((unsigned)nb, (unsigned)ntraj), RB_THREADS, 0, ctx->stream>>>(s, B, nb, framed ? d_x : nullptr,
@@ -255,57 +373,43 @@ static int sample_frames_t(qse_ctx* ctx, qse_state* st, const uint64_t* xmask_h,
         QSE_HIP(hipMemcpyAsync(flags.data(), d_f, sz_f, hipMemcpyDeviceToHost, ctx->stream));
     }
     QSE_HIP(hipStreamSynchronize(ctx->stream));
-    // Group flagged shots by trajectory, ascending u.
-    std::vector<std::vector<int64_t>> per(ntraj);
-    int64_t nflag = 0;
+    // flagged shots, and the frames they need (compact frame numbering)
+    std::vector<int64_t> list;
+    std::vector<int> need_f(F, -1), fidx2(ntraj, 0);
+    std::vector<uint64_t> xs2;
     for (uint64_t g = 0; g < total; ++g)
         if (flags[g]) {
-            per[g / (uint64_t)nshots].push_back((int64_t)g);
-            ++nflag;
+            list.push_back((int64_t)g);
+            const int f = fidx[g / (uint64_t)nshots];
+            if (need_f[f] < 0) {
+                need_f[f] = (int)xs2.size();
+                xs2.push_back(xs[f]);
+            }
         }
-    if (nflag == 0) return QSE_OK;
-    std::vector<int> traj;
-    std::vector<int64_t> first{0}, slot;
-    std::vector<double> ulist;
-    for (int t = 0; t < ntraj; ++t) {
-        if (per[t].empty()) continue;
-        std::stable_sort(per[t].begin(), per[t].end(), [&](int64_t a, int64_t b) { return u_h[a] < u_h[b]; });
-        traj.push_back(t);
-        for (int64_t g : per[t]) {
-            slot.push_back(g);
-            ulist.push_back(u_h[g]);
-        }
-        first.push_back((int64_t)slot.size());
-    }
-    const int nw = (int)traj.size();
-    const size_t need2 = al(sizeof(int) * nw) + al(sizeof(int64_t) * (nw + 1)) + al(sizeof(double) * nflag) +
-                         al(sizeof(int64_t) * nflag) + al(sizeof(uint64_t) * nflag);
+    if (list.empty()) return QSE_OK;
+    for (int t = 0; t < ntraj; ++t) fidx2[t] = std::max(need_f[fidx[t]], 0);
+    const int F2 = (int)xs2.size();
+    const uint64_t nl = list.size();
+    const size_t s_x = sizeof(uint64_t) * F2, s_fi = sizeof(int) * ntraj, s_ck = sizeof(double) * F2 * ng,
+                 s_l = sizeof(int64_t) * nl, s_o = sizeof(uint64_t) * nl;
     void* base2;
-    rc = scratch(ctx, 1, need2, &base2);
+    rc = scratch(ctx, 1, al(s_x) + al(s_fi) + al(s_ck) + al(s_l) + al(s_o), &base2);
     if (rc) return rc;
     char* q = (char*)base2;
-    int* d_traj = (int*)q; q += al(sizeof(int) * nw);
-    int64_t* d_first = (int64_t*)q; q += al(sizeof(int64_t) * (nw + 1));
-    double* d_ul = (double*)q; q += al(sizeof(double) * nflag);
-    int64_t* d_slot = (int64_t*)q; q += al(sizeof(int64_t) * nflag);
-    uint64_t* d_out2 = (uint64_t*)q;
-    // local slots: resolved values are written to d_out2[local index]
-    std::vector<int64_t> local(nflag);
-    for (int64_t i = 0; i < nflag; ++i) local[i] = i;
-    QSE_HIP(hipMemcpyAsync(d_traj, traj.data(), sizeof(int) * nw, hipMemcpyHostToDevice, ctx->stream));
-    QSE_HIP(hipMemcpyAsync(d_first, first.data(), sizeof(int64_t) * (nw + 1), hipMemcpyHostToDevice, ctx->stream));
-    QSE_HIP(hipMemcpyAsync(d_ul, ulist.data(), sizeof(double) * nflag, hipMemcpyHostToDevice, ctx->stream));
-    QSE_HIP(hipMemcpyAsync(d_slot, local.data(), sizeof(int64_t) * nflag, hipMemcpyHostToDevice, ctx->stream));
-    TimedLaunch t2;
-    launch_begin(ctx, FAM_READOUT, &t2);
-    k_exact_resolve<R><<<nw, RB_THREADS, 0, ctx->stream>>>(s, N, framed ? d_x : nullptr, d_traj, d_first, d_ul, d_slot,
-                                                           d_out2);
-    QSE_HIP(hipGetLastError());
-    launch_end(ctx, &t2, 2.0 * amp_bytes(st->prec) * N * nw);
-    std::vector<uint64_t> res(nflag);
-    QSE_HIP(hipMemcpyAsync(res.data(), d_out2, sizeof(uint64_t) * nflag, hipMemcpyDeviceToHost, ctx->stream));
+    uint64_t* d_x2 = (uint64_t*)q; q += al(s_x);
+    int* d_fi2 = (int*)q; q += al(s_fi);
+    double* d_ck = (double*)q; q += al(s_ck);
+    int64_t* d_l = (int64_t*)q; q += al(s_l);
+    uint64_t* d_o2 = (uint64_t*)q;
+    QSE_HIP(hipMemcpyAsync(d_x2, xs2.data(), s_x, hipMemcpyHostToDevice, ctx->stream));
+    QSE_HIP(hipMemcpyAsync(d_fi2, fidx2.data(), s_fi, hipMemcpyHostToDevice, ctx->stream));
+    QSE_HIP(hipMemcpyAsync(d_l, list.data(), s_l, hipMemcpyHostToDevice, ctx->stream));
+    rc = exact_from_chains<R>(ctx, st, d_x2, F2, d_fi2, d_ck, d_u, nshots, nl, d_l, d_o2);
+    if (rc) return rc;
+    std::vector<uint64_t> res(nl);
+    QSE_HIP(hipMemcpyAsync(res.data(), d_o2, s_o, hipMemcpyDeviceToHost, ctx->stream));
     QSE_HIP(hipStreamSynchronize(ctx->stream));
-    for (int64_t i = 0; i < nflag; ++i) out_h[slot[i]] = res[i];
+    for (uint64_t i = 0; i < nl; ++i) out_h[list[i]] = res[i];
     return QSE_OK;
 }
 

@@ -118,13 +118,15 @@ def test_sampling_exact_small_and_edge(ctx, n):
     assert np.array_equal(st.sample(u), O.sample(psi2, n, u))
 
 
-def test_sampling_flagged_draws_resolve_exactly(ctx):
-    """Targets placed on the sequential cumulative boundaries force the exact path."""
-    n = 16
+@pytest.mark.parametrize("n", [3, 9, 16])
+def test_sampling_flagged_draws_resolve_exactly(ctx, n):
+    """Targets placed on the sequential cumulative boundaries force the exact
+    path (checkpoint chains of the sequential sum): states below one LDS chunk
+    and below one checkpoint group of amplitudes, and a larger one."""
     psi = C.random_state(n, 77)
     p = psi.real * psi.real + psi.imag * psi.imag
     cum = np.cumsum(p)
-    idx = np.random.default_rng(3).integers(0, (1 << n) - 1, 500)
+    idx = np.random.default_rng(3).integers(0, (1 << n) - 1, 500) if n > 8 else np.arange((1 << n) - 1)
     u = cum[idx] / cum[-1]
     u = np.concatenate([u, np.nextafter(u, 0), np.nextafter(u, 2)])
     u = np.clip(u, 0, np.nextafter(1.0, 0))
