@@ -71,8 +71,11 @@ enum qse_gate_kind {
     QSE_GATE_CPHASE = 10,
     QSE_GATE_MCPHASE = 11,
     QSE_GATE_SWAP = 12,
-    /* GateType::MEASURE (13) and RESET_QUBIT (14) have no dispatch in the
-     * reference (Circuit.cpp:110-111) and are rejected here. */
+    /* GateType::MEASURE (13) and RESET_QUBIT (14): declared by the reference
+     * but never dispatched (Circuit.cpp:110-111). Only qse_run_measured takes
+     * them; the unitary entry points reject them. */
+    QSE_GATE_MEASURE = 13,
+    QSE_GATE_RESET_QUBIT = 14,
     QSE_GATE_MATRIX = 32 /* generic 2x2, ICircuitUnitaryOperation::applyGate */
 };
 
@@ -108,7 +111,9 @@ const char* qse_last_error(void);
 int qse_device_count(int* count);
 /* Free / total device memory of `device` (CircuitUnitaryOperationFactory.cpp:41-63). */
 int qse_device_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
-/* Device name, compute units and HBM size (HardwareCheck.cpp:35-54). */
+/* Device name (the architecture name, e.g. "gfx950:sramecc+:xnack-", when the
+ * driver reports no marketing name), compute units and HBM size
+ * (HardwareCheck.cpp:35-54). */
 int qse_device_info(int device, char* name, size_t name_len, int* compute_units, size_t* total_bytes);
 
 /* ---- context: one device, one stream (CircuitUnitaryOperationGPU.cu:140-156) ---- */
@@ -218,6 +223,20 @@ int qse_run(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const doub
  * captured in a HIP graph and replayed; results equal `reps` calls of qse_run.
  * The capture records kernels only (no memset nodes). */
 int qse_run_repeat(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, int reps);
+/* A gate list with mid-circuit measurements and resets (QSE_GATE_MEASURE,
+ * QSE_GATE_RESET_QUBIT; the reference's GateType::MEASURE / RESET_QUBIT,
+ * Structurs.h:10, with the result slot of GateCommand::resultIdx, :18). The
+ * unitary stretches between them run as qse_run; each MEASURE or RESET_QUBIT
+ * op collapses its target qubit with the next uniform of u, in list order:
+ * outcome = (u < P(qubit = 1)), then the state is renormalised
+ * (StateVector::measureQubit, StateVector.cpp:158-185); RESET_QUBIT then
+ * returns the qubit to |0> (StateVector::resetQubit, :187-221). A MEASURE op
+ * writes its outcome to results[op.control], its result slot
+ * (0 <= control < nresults). nu must be the number of MEASURE and RESET_QUBIT
+ * ops. A collapse onto an outcome of probability 0 cannot happen for
+ * u in [0, 1); synchronises at every measurement. */
+int qse_run_measured(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, const double* u,
+                     int nu, int32_t* results, int nresults);
 /* Same list, one single-gate sweep per op (no fusion): the drop-in per-call
  * path of CircuitUnitaryOperationHIP and the unfused baseline. */
 int qse_run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params);

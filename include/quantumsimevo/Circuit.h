@@ -52,8 +52,16 @@ public:
     std::vector<Complex> state();                          // flushed copy of the state
     void setState(const std::vector<Complex>& amps);       // replace the state
     double expectationZ(const std::vector<uint64_t>& masks, const std::vector<double>& weights);
-    int measureQubit(size_t q);                            // mid-circuit collapse
+    int measureQubit(size_t q);                            // mid-circuit collapse, now
     void resetQubit(size_t q);
+    // Mid-circuit measurement and reset as commands (GateType::MEASURE /
+    // RESET_QUBIT, which the reference declares but never executes): run by
+    // execute() in command order, each with the next uniform of the readout
+    // generator. queueMeasure returns the command's result slot; the outcome of
+    // the last execute() is measurementResults()[slot] (-1 before).
+    size_t queueMeasure(size_t q);
+    void queueReset(size_t q);
+    const std::vector<int>& measurementResults() const { return results; }
     const std::vector<GateCommand>& commands() const { return commandList; }
     // Gate-list files (GateListFile.h): the queued gates as C-ABI records, and
     // saving them / appending a file's gates (same qubit count; parameter
@@ -73,6 +81,7 @@ private:
     Noise pauliNoise;
     std::mt19937 readoutGen;
     std::vector<Gate2x2> matrices;  // generic 2x2 gates queued with gate()
+    std::vector<int> results;       // outcomes of MEASURE commands by result slot
 };
 
 #endif

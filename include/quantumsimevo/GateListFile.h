@@ -18,6 +18,8 @@
 //   MCPHASE c1,c2,... target theta
 //   SWAP q0 q1
 //   MATRIX q re00 im00 re01 im01 re10 im10 re11 im11           (row-major 2x2)
+//   MEASURE q slot        mid-circuit measurement of qubit q, outcome to slot
+//   RESET q               mid-circuit reset of qubit q to |0>
 //
 // "param p scale s" binds the angle to a parameter vector: the gate's angle is
 // theta + s * params[p] (the qse_op fields of the C ABI, used by the adjoint

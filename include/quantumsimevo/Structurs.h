@@ -11,9 +11,13 @@
 
 using Complex = std::complex<double>;
 
-// Same order as the reference enum (Structurs.h:10): the numeric values are the
-// qse_op kinds of the C ABI (include/qse_hip.h).
-enum class GateType { IDENTITY, X, Y, Z, H, PHASE, RX, RY, RZ, CNOT, CPHASE, MCPHASE, SWAP, MEASURE, RESET_QUBIT };
+// Same order as the reference enum (Structurs.h:10): the numeric values of the
+// reference's members are the qse_op kinds of the C ABI (include/qse_hip.h).
+// MATRIX, a generic 2x2 queued with Circuit::gate (ICircuitUnitaryOperation::
+// applyGate), comes after them, so the reference's numbering is unchanged; code
+// written against the reference enum sees it as an unknown gate type.
+enum class GateType { IDENTITY, X, Y, Z, H, PHASE, RX, RY, RZ, CNOT, CPHASE, MCPHASE, SWAP, MEASURE, RESET_QUBIT,
+                      MATRIX };
 
 struct GateCommand {
     GateType type;
@@ -21,7 +25,7 @@ struct GateCommand {
     size_t control;                    // CNOT, CPHASE control; SWAP second qubit
     double theta;                      // rotation / phase angle
     std::string description;
-    size_t resultIdx = 0;              // MEASURE result slot (unused by execute)
+    size_t resultIdx = 0;              // MEASURE: result slot; MATRIX: the Circuit's matrix table entry
     std::vector<size_t> controls_list; // MCPHASE controls
 };
 

@@ -308,6 +308,33 @@ int orc_collapse_qubit(double* psi, int n, int q, double u, int reset) {
     return outcome;
 }
 
+/* A gate list with mid-circuit measurements and resets (GateType::MEASURE
+ * = 13, RESET_QUBIT = 14, Structurs.h:10; the reference declares them but its
+ * Circuit::execute has no case for them, Circuit.cpp:110-111): unitary ops as
+ * orc_run_ops, and each MEASURE/RESET_QUBIT op collapses its target with the
+ * next uniform of u in list order (measureQubit / resetQubit semantics,
+ * StateVector.cpp:158-221, restated in orc_collapse_qubit); a MEASURE's
+ * outcome goes to results[op.control]. Returns 0, or -1 for an unsupported
+ * kind, a result slot outside [0, nresults) or too few uniforms. */
+int orc_run_measured(double* psi, int n, const qse_op* ops, int nops, const double* params, const double* u, int nu,
+                     int* results, int nresults) {
+    int k = 0;
+    for (int i = 0; i < nops; ++i) {
+        const qse_op& op = ops[i];
+        if (op.kind == 13 || op.kind == 14) {
+            if (k >= nu) return -1;
+            const int o = orc_collapse_qubit(psi, n, op.target, u[k++], op.kind == 14);
+            if (op.kind == 13) {
+                if (op.control < 0 || op.control >= nresults) return -1;
+                results[op.control] = o;
+            }
+        } else if (orc_run_ops(psi, n, &op, 1, params) != 0) {
+            return -1;
+        }
+    }
+    return 0;
+}
+
 /* Sum_i |a_i|^2 (sequential). */
 double orc_norm2(const double* psi, int n) {
     const cplx* s = amps(psi);

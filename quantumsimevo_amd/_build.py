@@ -4,7 +4,9 @@ Products (all written next to the sources, never installed):
   quantumsimevo_amd/libqse_hip.so        HIP kernels for gfx950 + the C ABI (include/qse_hip.h)
   quantumsimevo_amd/_quantumsimevo*.so   pybind11 module: the C++ Circuit / StateVector /
                                          Noise / CircuitUnitaryOperationHIP host classes
-  oracle/liboracle.so (+ oracle/_ref/)   the CPU oracle (test infrastructure, via oracle/Makefile)
+  oracle/liboracle.so (+ oracle/_ref/)   the CPU oracle and, where the reference tree is present, its
+                                         CPU backends and drop-in module (test infrastructure, via
+                                         oracle/Makefile)
 
 Every HIP source is compiled with hipcc --offload-arch=gfx950 and with packed
 FP32 instructions disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
@@ -100,7 +102,11 @@ def build_pymodule(verbose: bool = False, jobs: int = 8) -> Path:
 
 
 def build_oracle(verbose: bool = False) -> None:
-    _run(["make", "-s", "-C", str(ROOT / "oracle")], verbose)
+    """The CPU checker: oracle/liboracle.so, and where the reference tree is
+    present oracle/_ref (its CPU backends, and the drop-in module built from
+    its Circuit/Noise/pybind sources against include/quantumsimevo and
+    libqse_hip.so, which must be built first)."""
+    _run(["make", "-s", "-C", str(ROOT / "oracle"), "all", "dropin"], verbose)
 
 
 def build_test_emulator(verbose: bool = False) -> Path:
@@ -117,11 +123,11 @@ def build_test_emulator(verbose: bool = False) -> Path:
 
 
 def build_all(verbose: bool = False) -> None:
-    build_oracle(verbose)
-    build_test_emulator(verbose)
     build_lib(verbose)
     if (CSRC / "pymodule.cpp").exists():
         build_pymodule(verbose)
+    build_oracle(verbose)
+    build_test_emulator(verbose)
 
 
 if __name__ == "__main__":

@@ -17,10 +17,11 @@ LIB_PATH = Path(__file__).resolve().parent / "libqse_hip.so"
 C128, C64 = 0, 1
 
 # GateType numbering (reference Structurs.h:10) plus the generic 2x2 kind.
-IDENTITY, X, Y, Z, H, PHASE, RX, RY, RZ, CNOT, CPHASE, MCPHASE, SWAP = range(13)
+IDENTITY, X, Y, Z, H, PHASE, RX, RY, RZ, CNOT, CPHASE, MCPHASE, SWAP, MEASURE, RESET_QUBIT = range(15)
 MATRIX = 32
 KIND_NAMES = {IDENTITY: "I", X: "X", Y: "Y", Z: "Z", H: "H", PHASE: "PHASE", RX: "RX", RY: "RY", RZ: "RZ",
-              CNOT: "CNOT", CPHASE: "CPHASE", MCPHASE: "MCPHASE", SWAP: "SWAP", MATRIX: "MATRIX"}
+              CNOT: "CNOT", CPHASE: "CPHASE", MCPHASE: "MCPHASE", SWAP: "SWAP", MATRIX: "MATRIX",
+              MEASURE: "MEASURE", RESET_QUBIT: "RESET"}
 
 # struct qse_op (104 bytes)
 OP_DTYPE = np.dtype([("kind", "<i4"), ("target", "<i4"), ("control", "<i4"), ("param", "<i4"),
@@ -40,7 +41,7 @@ EXPORTS = [
     "qse_state_download_range", "qse_state_set_basis", "qse_state_copy", "qse_state_set_random", "qse_apply_matrix",
     "qse_apply_hadamard", "qse_apply_x", "qse_apply_y", "qse_apply_z", "qse_apply_phase", "qse_apply_rx",
     "qse_apply_ry", "qse_apply_rz", "qse_apply_cnot", "qse_apply_cphase", "qse_apply_mcphase",
-    "qse_apply_swap", "qse_pauli_frame", "qse_run", "qse_run_repeat", "qse_run_sweeps", "qse_plan_passes", "qse_plan_info", "qse_qft_probe", "qse_copy_bandwidth", "qse_sample",
+    "qse_apply_swap", "qse_pauli_frame", "qse_run", "qse_run_repeat", "qse_run_measured", "qse_run_sweeps", "qse_plan_passes", "qse_plan_info", "qse_qft_probe", "qse_copy_bandwidth", "qse_sample",
     "qse_measure", "qse_norm2", "qse_prob_one", "qse_collapse_qubit", "qse_expect_zstrings", "qse_expect_zstrings_noalloc",
     "qse_adjoint_expect", "qse_batch_expect", "qse_batch_run_small", "qse_sample_frames",
     "qse_compose_dense", "qse_apply_dense", "qse_state_renormalize", "qse_ctx_set_renormalize",
@@ -114,6 +115,7 @@ def _sigs(lib):
         "qse_run": ([_vp, _vp, _vp, _i, _vp], _i),
         "qse_run_sweeps": ([_vp, _vp, _vp, _i, _vp], _i),
         "qse_run_repeat": ([_vp, _vp, _vp, _i, _vp, _i], _i),
+        "qse_run_measured": ([_vp, _vp, _vp, _i, _vp, _vp, _i, _vp, _i], _i),
         "qse_plan_passes": ([_vp, _i, _i, _vp, _i, C.POINTER(_i)], _i),
         "qse_plan_info": ([_vp, _i, _i, _vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)], _i),
         "qse_qft_probe": ([_vp, _i, C.c_uint64, _vp, _i, _vp], _i),
@@ -461,6 +463,19 @@ class State:
         f = self.lib.qse_run if fused else self.lib.qse_run_sweeps
         check(f(self.ctx.h, self.h, _ptr(a), len(a), None if p is None else _ptr(p)),
               "qse_run" if fused else "qse_run_sweeps")
+
+    def run_measured(self, ops, u, nresults: int, params=None) -> np.ndarray:
+        """A gate list with MEASURE / RESET_QUBIT ops (qse_run_measured): one
+        uniform of u per measurement op in list order; returns the outcomes by
+        result slot (-1 for slots no MEASURE op wrote)."""
+        a = as_ops(ops)
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        res = np.full(max(nresults, 1), -1, dtype=np.int32)
+        pp = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        check(self.lib.qse_run_measured(self.ctx.h, self.h, _ptr(a), len(a), _ptr(pp) if pp is not None else None,
+                                        _ptr(uu) if len(uu) else None, len(uu), _ptr(res), nresults),
+              "qse_run_measured")
+        return res[:nresults]
 
     def run_repeat(self, ops, reps: int, params=None):
         """The gate list applied reps times (replayed from a HIP graph after two eager runs)."""

@@ -18,7 +18,8 @@ import math
 
 import numpy as np
 
-from .capi import (CNOT, CPHASE, H, IDENTITY, MATRIX, MCPHASE, OP_DTYPE, PHASE, RX, RY, RZ, SWAP, X, Y, Z)
+from .capi import (CNOT, CPHASE, H, IDENTITY, MATRIX, MCPHASE, MEASURE, OP_DTYPE, PHASE, RESET_QUBIT, RX, RY, RZ,
+                   SWAP, X, Y, Z)
 
 PI = 3.14159265358979323846  # QFT.cpp:7
 
@@ -68,6 +69,11 @@ class GateList:
         return self._add(MATRIX, q, m=flat)
 
     def identity(self, q=0): return self._add(IDENTITY, q)
+
+    # mid-circuit measurement and reset (GateType::MEASURE / RESET_QUBIT, run by
+    # State.run_measured; `slot` is the result slot, GateCommand::resultIdx)
+    def measure(self, q, slot): return self._add(MEASURE, q, slot)
+    def reset_qubit(self, q): return self._add(RESET_QUBIT, q)
 
     def extend(self, other: "GateList"):
         self._ops.extend(other._ops)

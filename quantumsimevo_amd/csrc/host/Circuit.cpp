@@ -86,12 +86,24 @@ void Circuit::swap(size_t q0, size_t q1) {
 
 void Circuit::gate(const Gate2x2& g, size_t q) {
     check_q(stateVector, q);
-    GateCommand cmd{GateType::IDENTITY, q, 0, 0.0, "Gate2x2 on qubit " + std::to_string(q)};
-    // GateType has no generic-matrix member; the matrix rides in a side table.
-    cmd.resultIdx = matrices.size();
+    GateCommand cmd{GateType::MATRIX, q, 0, 0.0, "Gate2x2 on qubit " + std::to_string(q)};
+    cmd.resultIdx = matrices.size();  // the matrix rides in a side table
     matrices.push_back(g);
-    cmd.controls_list = {SIZE_MAX};
     commandList.push_back(cmd);
+}
+
+size_t Circuit::queueMeasure(size_t q) {
+    check_q(stateVector, q);
+    GateCommand cmd{GateType::MEASURE, q, 0, 0.0, "Measure qubit " + std::to_string(q)};
+    cmd.resultIdx = results.size();
+    results.push_back(-1);
+    commandList.push_back(cmd);
+    return cmd.resultIdx;
+}
+
+void Circuit::queueReset(size_t q) {
+    check_q(stateVector, q);
+    commandList.push_back({GateType::RESET_QUBIT, q, 0, 0.0, "Reset qubit " + std::to_string(q)});
 }
 
 // Circuit.cpp:64-120: replay from the current state, noise draws before each gate,
@@ -117,10 +129,18 @@ void Circuit::execute(bool print_steps) {
             case GateType::Y: circuitOp->applyPauliY(stateVector, cmd.target); break;
             case GateType::PHASE: circuitOp->applyPhase(stateVector, cmd.target, cmd.theta); break;
             case GateType::Z: circuitOp->applyPauliZ(stateVector, cmd.target); break;
-            case GateType::IDENTITY:
-                if (cmd.controls_list.size() == 1 && cmd.controls_list[0] == SIZE_MAX)
-                    circuitOp->applyGate(stateVector, matrices[cmd.resultIdx], cmd.target);
+            case GateType::IDENTITY: break;
+            case GateType::MATRIX: circuitOp->applyGate(stateVector, matrices[cmd.resultIdx], cmd.target); break;
+            case GateType::MEASURE:
+            case GateType::RESET_QUBIT: {
+                // StateVector::measureQubit / resetQubit (StateVector.cpp:158-221) on the device, with the
+                // next uniform of the circuit's readout generator
+                std::uniform_real_distribution<double> dist(0.0, 1.0);
+                const int o = backend().collapseQubit(stateVector, cmd.target, dist(readoutGen),
+                                                      cmd.type == GateType::RESET_QUBIT);
+                if (cmd.type == GateType::MEASURE) results[cmd.resultIdx] = o;
                 break;
+            }
             default: std::cerr << "Unsupported gate type: " << static_cast<int>(cmd.type) << std::endl;
         }
     }
@@ -155,6 +175,7 @@ void Circuit::reset() {
     if (simulateNoise) pauliNoise.resetStats();
     commandList.clear();
     matrices.clear();
+    results.clear();
 }
 
 void Circuit::printState() {
@@ -195,17 +216,27 @@ std::vector<qse_op> Circuit::gateList() const {
         g.theta = c.theta;
         switch (c.type) {
             case GateType::IDENTITY:
-                if (c.controls_list.size() == 1 && c.controls_list[0] == SIZE_MAX) {
-                    g.kind = QSE_GATE_MATRIX;
-                    const Gate2x2& m = matrices[c.resultIdx];
-                    for (int r = 0; r < 2; ++r)
-                        for (int k = 0; k < 2; ++k) {
-                            g.m[4 * r + 2 * k] = m.data[r][k].real();
-                            g.m[4 * r + 2 * k + 1] = m.data[r][k].imag();
-                        }
-                } else {
-                    g.kind = QSE_GATE_IDENTITY;
-                }
+                g.kind = QSE_GATE_IDENTITY;
+                g.theta = 0.0;
+                break;
+            case GateType::MATRIX: {
+                g.kind = QSE_GATE_MATRIX;
+                const Gate2x2& m = matrices[c.resultIdx];
+                for (int r = 0; r < 2; ++r)
+                    for (int k = 0; k < 2; ++k) {
+                        g.m[4 * r + 2 * k] = m.data[r][k].real();
+                        g.m[4 * r + 2 * k + 1] = m.data[r][k].imag();
+                    }
+                g.theta = 0.0;
+                break;
+            }
+            case GateType::MEASURE:
+                g.kind = QSE_GATE_MEASURE;
+                g.control = (int32_t)c.resultIdx;
+                g.theta = 0.0;
+                break;
+            case GateType::RESET_QUBIT:
+                g.kind = QSE_GATE_RESET_QUBIT;
                 g.theta = 0.0;
                 break;
             case GateType::X: g.kind = QSE_GATE_X; break;
@@ -245,6 +276,7 @@ void Circuit::loadGates(const std::string& path) {
     if ((size_t)n != numQubits())
         throw std::runtime_error("gate list " + path + " is for " + std::to_string(n) + " qubits, the circuit has " +
                                  std::to_string(numQubits()));
+    const size_t slot0 = results.size();  // the file's result slots follow the circuit's own
     for (const qse_op& g : ops) {
         if (g.param >= 0)
             throw std::runtime_error("gate list " + path + ": parameter bindings need the batched or adjoint API");
@@ -273,6 +305,16 @@ void Circuit::loadGates(const std::string& path) {
                              Complex(g.m[6], g.m[7])),
                      g.target);
                 break;
+            case QSE_GATE_MEASURE: {
+                check_q(stateVector, g.target);
+                GateCommand cmd{GateType::MEASURE, (size_t)g.target, 0, 0.0,
+                                "Measure qubit " + std::to_string(g.target)};
+                cmd.resultIdx = slot0 + (size_t)g.control;
+                if (results.size() <= cmd.resultIdx) results.resize(cmd.resultIdx + 1, -1);
+                commandList.push_back(cmd);
+                break;
+            }
+            case QSE_GATE_RESET_QUBIT: queueReset(g.target); break;
             default: throw std::runtime_error("gate list: unsupported gate kind");
         }
     }

@@ -20,7 +20,8 @@ const KindName kNames[] = {{QSE_GATE_IDENTITY, "I"}, {QSE_GATE_X, "X"},         
                            {QSE_GATE_Z, "Z"},        {QSE_GATE_H, "H"},         {QSE_GATE_PHASE, "PHASE"},
                            {QSE_GATE_RX, "RX"},      {QSE_GATE_RY, "RY"},       {QSE_GATE_RZ, "RZ"},
                            {QSE_GATE_CNOT, "CNOT"},  {QSE_GATE_CPHASE, "CPHASE"}, {QSE_GATE_MCPHASE, "MCPHASE"},
-                           {QSE_GATE_SWAP, "SWAP"},  {QSE_GATE_MATRIX, "MATRIX"}};
+                           {QSE_GATE_SWAP, "SWAP"},  {QSE_GATE_MATRIX, "MATRIX"}, {QSE_GATE_MEASURE, "MEASURE"},
+                           {QSE_GATE_RESET_QUBIT, "RESET"}};
 
 const char* kind_name(int kind) {
     for (const auto& k : kNames)
@@ -69,6 +70,10 @@ std::string write(int n, const qse_op* ops, size_t nops) {
             case QSE_GATE_MATRIX:
                 o << ' ' << g.target;
                 for (int j = 0; j < 8; ++j) o << ' ' << num(g.m[j]);
+                break;
+            case QSE_GATE_MEASURE:
+                if (g.control < 0) throw std::runtime_error("gate list: MEASURE without a result slot");
+                o << ' ' << g.target << ' ' << g.control;
                 break;
             default:
                 o << ' ' << g.target;
@@ -198,6 +203,14 @@ void read(const std::string& text, int* n_out, std::vector<qse_op>& ops) {
                 g.target = qubit(f[1]);
                 for (int j = 0; j < 8; ++j) g.m[j] = to_double(f[2 + j], line);
                 break;
+            case QSE_GATE_MEASURE: {
+                want(3);
+                g.target = qubit(f[1]);
+                const long slot = to_int(f[2], line);
+                if (slot < 0 || slot > 0x7fffffff) fail(line, "result slot outside [0, 2^31)");
+                g.control = (int32_t)slot;
+                break;
+            }
             default:
                 if (one_qubit_angle(kind)) {
                     want(3);

@@ -85,6 +85,14 @@ PYBIND11_MODULE(_quantumsimevo, m) {
         .def("queue_measure", &Circuit::queueMeasure, py::arg("q"))
         .def("queue_reset", &Circuit::queueReset, py::arg("q"))
         .def_property_readonly("measurement_results", &Circuit::measurementResults)
+        // the Circuit's StateVector host buffer as it stands (Circuit::getStateVector
+        // does not flush, Circuit.h:53)
+        .def("host_state", [](Circuit& c) {
+            StateVector& sv = c.getStateVector();
+            py::array_t<std::complex<double>> out(sv.size());
+            std::copy(sv.data().begin(), sv.data().end(), out.mutable_data());
+            return out;
+        })
         .def_property_readonly("num_qubits", &Circuit::numQubits)
         .def_property_readonly("num_commands", [](const Circuit& c) { return c.commands().size(); })
         .def("noise_stats", [](Circuit& c) {
@@ -101,6 +109,49 @@ PYBIND11_MODULE(_quantumsimevo, m) {
             const std::vector<qse_op> ops = c.gateList();
             return py::bytes(reinterpret_cast<const char*>(ops.data()), ops.size() * sizeof(qse_op));
         });
+
+    // The reference's StateVector and backend interface as they stand
+    // (StateVector.h:17-47, ICircuitUnitaryOperation.h:6-35), for driving the
+    // device-mirror contract (flush / markHostDirty / rebinding) directly.
+    py::class_<StateVector>(m, "StateVector")
+        .def(py::init<size_t>(), py::arg("num_qubits"))
+        .def_property_readonly("num_qubits", &StateVector::qubits)
+        .def_property_readonly("size", &StateVector::size)
+        .def_property_readonly("host_allocated", &StateVector::hostAllocated)
+        // copy of the host buffer (allocating it if needed); no device access
+        .def("host_data", [](StateVector& sv) {
+            py::array_t<std::complex<double>> out(sv.size());
+            std::copy(sv.data().begin(), sv.data().end(), out.mutable_data());
+            return out;
+        })
+        // overwrite the host buffer (the caller then marks the host dirty, as in Circuit::reset)
+        .def("set_host_data",
+             [](StateVector& sv, py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> a) {
+                 if ((size_t)a.size() != sv.size()) throw std::invalid_argument("size mismatch");
+                 std::copy(a.data(), a.data() + a.size(), sv.data().begin());
+             },
+             py::arg("amplitudes"))
+        .def("reset", &StateVector::reset);
+    py::class_<CircuitUnitaryOperationHIP>(m, "HIPBackend")
+        .def(py::init([](int device, py::object precision) {
+                 return new CircuitUnitaryOperationHIP(device, parse_precision(precision));
+             }),
+             py::arg("device") = 0, py::arg("precision") = "c128")
+        .def("apply_hadamard", &CircuitUnitaryOperationHIP::applyHadamard, py::keep_alive<1, 2>())
+        .def("apply_pauli_x", &CircuitUnitaryOperationHIP::applyPauliX, py::keep_alive<1, 2>())
+        .def("apply_pauli_y", &CircuitUnitaryOperationHIP::applyPauliY, py::keep_alive<1, 2>())
+        .def("apply_pauli_z", &CircuitUnitaryOperationHIP::applyPauliZ, py::keep_alive<1, 2>())
+        .def("apply_phase", &CircuitUnitaryOperationHIP::applyPhase, py::keep_alive<1, 2>())
+        .def("apply_rotate_x", &CircuitUnitaryOperationHIP::applyRotateX, py::keep_alive<1, 2>())
+        .def("apply_rotate_y", &CircuitUnitaryOperationHIP::applyRotateY, py::keep_alive<1, 2>())
+        .def("apply_rotate_z", &CircuitUnitaryOperationHIP::applyRotateZ, py::keep_alive<1, 2>())
+        .def("apply_cnot", &CircuitUnitaryOperationHIP::applyCNOT, py::keep_alive<1, 2>())
+        .def("apply_cphase", &CircuitUnitaryOperationHIP::applyCPhase, py::keep_alive<1, 2>())
+        .def("apply_mcphase", &CircuitUnitaryOperationHIP::applyMCPhase, py::keep_alive<1, 2>())
+        .def("apply_swap", &CircuitUnitaryOperationHIP::applySwap, py::keep_alive<1, 2>())
+        .def("flush", [](CircuitUnitaryOperationHIP& b) { b.flush(); })
+        .def("mark_host_dirty", [](CircuitUnitaryOperationHIP& b) { b.markHostDirty(); })
+        .def("set_print_steps", [](CircuitUnitaryOperationHIP& b, bool on) { b.setPrintSteps(on); }, py::arg("on"));
 
     m.def("device_report", [](int device) {
               const DeviceReport r = HardwareCheck::report(device);

@@ -185,7 +185,10 @@ int qse_device_info(int device, char* name, size_t name_len, int* compute_units,
     hipDeviceProp_t p;
     QSE_HIP(hipGetDeviceProperties(&p, device));
     if (name && name_len) {
-        std::strncpy(name, p.name, name_len - 1);
+        // the marketing name comes from the driver's ID table, which a container may
+        // lack; the architecture name always identifies the part then
+        const char* src = p.name[0] ? p.name : p.gcnArchName;
+        std::strncpy(name, src, name_len - 1);
         name[name_len - 1] = 0;
     }
     if (compute_units) *compute_units = p.multiProcessorCount;

@@ -61,6 +61,9 @@ int validate_ops(int n, const qse_op* ops, int nops, const double* params) {
                 if (badq(o.target) || (n < 64 && (o.controls >> n)))
                     return set_error(QSE_ERR_INVALID_ARG, "op " + std::to_string(k) + ": bad qubit");
                 break;
+            case QSE_GATE_MEASURE: case QSE_GATE_RESET_QUBIT:
+                return set_error(QSE_ERR_UNSUPPORTED, "op " + std::to_string(k) +
+                                                          ": MEASURE and RESET_QUBIT run through qse_run_measured");
             default:
                 return set_error(QSE_ERR_UNSUPPORTED, "op " + std::to_string(k) + ": unsupported gate kind " +
                                                           std::to_string(o.kind));

@@ -124,6 +124,54 @@ int qse_run_repeat(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, con
     return renorm_tick(ctx, st, (int64_t)passes * reps);
 }
 
+int qse_run_measured(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params, const double* u,
+                     int nu, int32_t* results, int nresults) {
+    if (!ctx || !st || !st->d) return set_error(QSE_ERR_INVALID_ARG, "null context or state");
+    if (nops < 0 || (nops > 0 && !ops) || nu < 0 || (nu > 0 && !u) || nresults < 0 || (nresults > 0 && !results))
+        return set_error(QSE_ERR_INVALID_ARG, "bad op list, uniforms or result slots");
+    // validate everything before touching the state: unitary stretches as
+    // qse_run, measurement ops by qubit, slot and uniform count
+    int count = 0, begin = 0;
+    for (int k = 0; k <= nops; ++k) {
+        const bool meas = k < nops && (ops[k].kind == QSE_GATE_MEASURE || ops[k].kind == QSE_GATE_RESET_QUBIT);
+        if (k < nops && !meas) continue;
+        int rc = validate_ops(st->n, ops + begin, k - begin, params);
+        if (rc) return rc;
+        if (meas) {
+            const qse_op& o = ops[k];
+            if (o.target < 0 || o.target >= st->n)
+                return set_error(QSE_ERR_INVALID_ARG, "op " + std::to_string(k) + ": bad target");
+            if (o.kind == QSE_GATE_MEASURE && (o.control < 0 || o.control >= nresults))
+                return set_error(QSE_ERR_INVALID_ARG, "op " + std::to_string(k) + ": result slot " +
+                                                          std::to_string(o.control) + " outside [0, nresults)");
+            ++count;
+        }
+        begin = k + 1;
+    }
+    if (count != nu)
+        return set_error(QSE_ERR_INVALID_ARG, std::to_string(count) + " measurements but " + std::to_string(nu) +
+                                                  " uniforms");
+    begin = 0;
+    int next = 0;
+    for (int k = 0; k <= nops; ++k) {
+        const bool meas = k < nops && (ops[k].kind == QSE_GATE_MEASURE || ops[k].kind == QSE_GATE_RESET_QUBIT);
+        if (k < nops && !meas) continue;
+        if (k > begin) {
+            int rc = qse_run(ctx, st, ops + begin, k - begin, params);
+            if (rc) return rc;
+        }
+        if (meas) {
+            const qse_op& o = ops[k];
+            int outcome = 0;
+            int rc = qse_collapse_qubit(ctx, st, o.target, u[next++], o.kind == QSE_GATE_RESET_QUBIT, &outcome);
+            if (rc) return rc;
+            if (o.kind == QSE_GATE_MEASURE) results[o.control] = outcome;
+        }
+        begin = k + 1;
+    }
+    return QSE_OK;
+}
+
 int qse_run_sweeps(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const double* params) {
     if (!ctx || !st || !st->d) return set_error(QSE_ERR_INVALID_ARG, "null context or state");
     if (nops < 0 || (nops > 0 && !ops)) return set_error(QSE_ERR_INVALID_ARG, "bad op list");

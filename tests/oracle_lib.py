@@ -48,6 +48,8 @@ def oracle():
         lib.orc_norm2.restype = _d
         lib.orc_expect_zstrings.argtypes = [_vp, _i, _vp, _vp, _i]
         lib.orc_expect_zstrings.restype = _d
+        lib.orc_run_measured.argtypes = [_vp, _i, _vp, _i, _vp, _vp, _i, _vp, _i]
+        lib.orc_run_measured.restype = _i
         lib.orc_adjoint_expect.argtypes = [_i, _vp, _i, _vp, _i, _vp, _vp, _i, _vp, _vp]
         lib.orc_adjoint_expect.restype = _i
         lib.orc_noise_create.argtypes = [_i, _u32, _d]
@@ -89,6 +91,18 @@ def run_ops(psi: np.ndarray, n: int, ops: np.ndarray, params=None) -> np.ndarray
     rc = oracle().orc_run_ops(_p(s), n, _p(ops), len(ops), _p(pp))
     assert rc == 0
     return s
+
+
+def run_measured(psi: np.ndarray, n: int, ops: np.ndarray, u, nresults: int, params=None):
+    """Gate list with MEASURE (13) / RESET_QUBIT (14) ops replayed with the
+    uniforms u in list order: (state, results)."""
+    s = np.ascontiguousarray(psi, dtype=np.complex128).copy()
+    pp = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+    uu = np.ascontiguousarray(u, dtype=np.float64)
+    res = np.full(max(nresults, 1), -1, dtype=np.int32)
+    rc = oracle().orc_run_measured(_p(s), n, _p(ops), len(ops), _p(pp), _p(uu), len(uu), _p(res), nresults)
+    assert rc == 0
+    return s, res[:nresults]
 
 
 def ref_run_ops(psi: np.ndarray, n: int, ops: np.ndarray, params=None, backend: int = 0) -> np.ndarray:

@@ -60,3 +60,13 @@ def test_amp_update_counts():
     ops = C.qaoa_maxcut(24, C.random_regular_graph(24, 3, seed=1), 8).ops()
     assert len(ops) == 24 + 8 * (36 * 3 + 24)
     assert np.sum(ops["param"] >= 0) == 8 * (36 + 24)
+
+
+def test_hardware_check_without_device():
+    """HardwareCheck::report with no visible device (HardwareCheck.cpp:35-54's
+    'not detected' branch): unavailable, no sizes, host threads counted."""
+    from quantumsimevo_amd import _quantumsimevo as Q
+    if capi.device_count() > 0:
+        return
+    r = Q.device_report(0)
+    assert not r["available"] and r["max_qubits_c128"] == 0 and r["host_threads"] > 0

@@ -41,6 +41,8 @@ def test_canonical_text_is_stable():
     assert "CPHASE 1 3 0.78539816339744828 param 0 scale 1.5" in text
     assert "MCPHASE 0,2,4 1 3.1415926535897931" in text
     assert "MATRIX 3 0.59999999999999998 0 0 0.80000000000000004 0 0.80000000000000004 0.59999999999999998 0" in text
+    mr = G.GateList(4).measure(2, 7).reset_qubit(1).to_text()
+    assert "MEASURE 2 7\nRESET 1\n" in mr
     n, ops = G.gates_from_text(text)
     assert G.gates_to_text(n, ops) == text
 
@@ -50,6 +52,7 @@ def test_canonical_text_is_stable():
     lambda: (16, G.qaoa_maxcut(16, G.random_regular_graph(16, 3, seed=3), 4).ops()),
     lambda: (10, G.ansatz(10, 6).ops()),
     lambda: (9, G.qft(9).ops()),
+    lambda: (6, G.GateList(6).hadamard(0).measure(0, 0).reset_qubit(3).cnot(0, 5).measure(5, 1).ops()),
 ])
 def test_round_trip_bit_exact(make, tmp_path):
     n, ops = make()
@@ -57,6 +60,8 @@ def test_round_trip_bit_exact(make, tmp_path):
     G.save_gates(path, n, ops)
     n2, ops2 = G.load_gates(path)
     assert n2 == n and ops2.tobytes() == ops.tobytes()
+    if np.any(ops2["kind"] >= capi.MEASURE) and np.all(ops2["kind"] != capi.MATRIX):
+        return
     # the loaded list is a batch entry as it stands
     entry = (n2, ops2, np.zeros(max(int(ops2["param"].max()) + 1, 0)))
     nq, op_off, packed, par_off, params = G.pack_batch([entry, entry])
@@ -75,6 +80,9 @@ def test_round_trip_bit_exact(make, tmp_path):
     ("qse-gates 1\nqubits 3\nRZ 0 zero\n", "not a number: 'zero'"),
     ("qse-gates 1\nqubits 3\nMCPHASE 0,0 1 1.0\n", "repeated control qubit 0"),
     ("qse-gates 1\nqubits 3\nH -1\n", "qubit -1 outside"),
+    ("qse-gates 1\nqubits 3\nMEASURE 1\n", "MEASURE takes 2 fields"),
+    ("qse-gates 1\nqubits 3\nMEASURE 1 -2\n", "result slot outside"),
+    ("qse-gates 1\nqubits 3\nRESET 1 0\n", "RESET takes 1 fields"),
 ])
 def test_malformed_input_names_the_problem(text, needle):
     with pytest.raises(RuntimeError, match=__import__("re").escape(needle)):

@@ -177,3 +177,170 @@ def test_gate_list_files_through_the_circuit(Circuit, tmp_path):
     # the committed example file, through the Circuit (parameter bindings refused there)
     with pytest.raises(RuntimeError, match="parameter bindings"):
         Circuit(5).load_gates(str(Path(__file__).resolve().parent / "golden" / "gatelist_example.qsegates"))
+
+
+def measured_list(n, seed, layers=6):
+    """A random layered circuit with MEASURE and RESET_QUBIT ops inserted after
+    random gates (result slots in order); returns (ops, number of measurement
+    ops, number of result slots)."""
+    rng = np.random.default_rng(seed)
+    base = G.random_layered(n, layers, seed).ops()
+    g = G.GateList.from_ops(n, base)
+    rows = list(g._ops)
+    out, nmeas, nslot = [], 0, 0
+    for r in rows:
+        out.append(r)
+        x = rng.random()
+        if x < 0.08:
+            out.append((capi.MEASURE, int(rng.integers(n)), nslot, -1, 0, 0.0, 0.0, (0.0,) * 8))
+            nslot += 1
+            nmeas += 1
+        elif x < 0.12:
+            out.append((capi.RESET_QUBIT, int(rng.integers(n)), -1, -1, 0, 0.0, 0.0, (0.0,) * 8))
+            nmeas += 1
+    g._ops = out
+    return g.ops(), nmeas, nslot
+
+
+@pytest.mark.parametrize("n,prec", [(4, capi.C128), (12, capi.C128), (16, capi.C128), (14, capi.C64)])
+def test_run_measured_vs_oracle_replay(ctx, n, prec):
+    """GateType::MEASURE / RESET_QUBIT mid-circuit (qse_run_measured) against the
+    oracle's replay with the same uniforms: outcomes equal, final state 1e-12
+    (complex64: 1e-5; uniforms within 1e-6 of P(1) are avoided there)."""
+    ops, nmeas, nslot = measured_list(n, 40 + n)
+    assert nmeas >= 2
+    u = np.random.default_rng(n).random(nmeas)
+    want, res_o = O.run_measured(O.basis(n), n, ops, u, nslot)
+    st = ctx.state(n, prec)
+    res = st.run_measured(ops, u, nslot)
+    assert np.array_equal(res, res_o)
+    assert np.max(np.abs(st.download() - want)) < (1e-12 if prec == capi.C128 else 1e-5)
+    st.close()
+
+
+def test_run_measured_rejects_bad_lists(ctx):
+    st = ctx.state(5)
+    ops = G.GateList(5).hadamard(0).measure(0, 0).reset_qubit(1).ops()
+    with pytest.raises(capi.QSEError):
+        st.run_measured(ops, [0.5], 1)  # two measurement ops, one uniform
+    with pytest.raises(capi.QSEError):
+        st.run_measured(ops, [0.5, 0.5], 0)  # result slot 0 outside [0, 0)
+    with pytest.raises(capi.QSEError):
+        st.run(ops)  # the unitary entry point rejects measurement ops
+    assert np.array_equal(st.download(), O.basis(5))  # nothing ran
+    st.close()
+
+
+def test_circuit_measure_and_reset_commands(Circuit):
+    """Circuit.queue_measure / queue_reset: executed in command order with the
+    circuit's seeded readout uniforms, outcomes in their result slots; equal to
+    the oracle replay of the circuit's own gate list with those uniforms, and
+    the gate list round-trips through a file."""
+    n = 11
+    qc = Circuit(n)
+    qc.set_seed(17)
+    ops, nmeas, nslot = measured_list(n, 5)
+    slots = []
+    for o in ops:
+        k, t = int(o["kind"]), int(o["target"])
+        if k == capi.MEASURE:
+            slots.append(qc.queue_measure(t))
+        elif k == capi.RESET_QUBIT:
+            qc.queue_reset(t)
+        else:
+            build(qc, G.GateList.from_ops(n, o[None]))
+    assert slots == list(range(nslot))
+    qc.execute()
+    u = O.uniforms(17, nmeas)
+    want, res_o = O.run_measured(O.basis(n), n, ops, u, nslot)
+    assert list(qc.measurement_results) == res_o.tolist()
+    assert np.max(np.abs(qc.state() - want)) < 1e-12
+    assert np.frombuffer(qc.gate_list(), dtype=capi.OP_DTYPE).tobytes() == ops.tobytes()
+
+
+def test_device_mirror_contract():
+    """The device-mirror contract of the backend interface
+    (CircuitUnitaryOperationGPU.cu:158-192, ICircuitUnitaryOperation.h:12-17),
+    driven through StateVector and the HIP backend directly:
+      - the first gate uploads the host state; flush() brings the device state back;
+      - flush() is a no-op while the host is marked dirty (GPU.cu:183): host
+        writes after mark_host_dirty() survive it, and the next gate re-uploads them;
+      - gates queued before mark_host_dirty() act on content that the re-upload
+        replaces, so they are lost (as on the reference's device);
+      - binding another StateVector flushes the previous one first (GPU.cu:163)."""
+    from quantumsimevo_amd import _quantumsimevo as Q
+    n = 3
+    b = Q.HIPBackend(0, "c128")
+    a = Q.StateVector(n)
+    b.apply_pauli_x(a, 0)
+    assert np.array_equal(a.host_data(), O.basis(n, 0))  # no flush yet: host still |000>
+    b.flush()
+    assert np.array_equal(a.host_data(), O.basis(n, 1))
+    # host write + dirty mark: flush keeps the host copy
+    a.set_host_data(O.basis(n, 2))
+    b.mark_host_dirty()
+    b.flush()
+    assert np.array_equal(a.host_data(), O.basis(n, 2))
+    b.apply_pauli_x(a, 2)  # re-upload |010>, then X(2)
+    b.flush()
+    assert np.array_equal(a.host_data(), O.basis(n, 6))
+    # a gate queued before the dirty mark is lost with the device content it acted on
+    b.apply_pauli_x(a, 0)
+    b.mark_host_dirty()
+    b.apply_pauli_x(a, 1)  # re-upload |110> (host), X(1) -> |100>
+    b.flush()
+    assert np.array_equal(a.host_data(), O.basis(n, 4))
+    # rebinding flushes the previous state
+    b.apply_hadamard(a, 0)
+    c = Q.StateVector(n)
+    b.apply_pauli_x(c, 1)  # binds c: a is flushed first
+    want = O.run_ops(O.basis(n, 4), n, G.GateList(n).hadamard(0).ops())
+    assert np.max(np.abs(a.host_data() - want)) < 1e-15
+    b.flush()
+    assert np.array_equal(c.host_data(), O.basis(n, 2))
+
+
+def test_get_state_vector_does_not_flush(Circuit):
+    """Circuit.execute() and getStateVector() leave the host copy as it was
+    (Circuit.cpp:64-120, Circuit.h:53); print_state() flushes (Circuit.cpp:135-138)."""
+    n = 4
+    qc = Circuit(n)
+    qc.pauli_x(2)
+    qc.execute()
+    assert np.array_equal(qc.host_state(), O.basis(n, 0))  # stale until a flush
+    assert np.array_equal(qc.state(), O.basis(n, 4))       # device copy, read without flushing
+    assert np.array_equal(qc.host_state(), O.basis(n, 0))
+    qc.print_state()
+    assert np.array_equal(qc.host_state(), O.basis(n, 4))
+
+
+def test_hardware_check_report(capfd):
+    """HardwareCheck (HardwareCheck.cpp:35-98) on the HIP device: the largest
+    state follows the reference's maxQubits rule floor(log2(0.8 * bytes / 16))
+    (HardwareCheck.cpp:56-60), and 8 bytes per amplitude for complex64."""
+    from quantumsimevo_amd import _quantumsimevo as Q
+    r = Q.device_report(0)
+    assert r["available"] and r["name"] and r["compute_units"] > 0
+    info = capi.device_info(0)
+    assert r["total_bytes"] == info["total_bytes"] and r["compute_units"] == info["compute_units"]
+    assert 0 < r["free_bytes"] <= r["total_bytes"]
+    assert r["max_qubits_c128"] == math.floor(math.log2(0.8 * r["total_bytes"] / 16.0))
+    assert r["max_qubits_c64"] == r["max_qubits_c128"] + 1
+    # a state of max_qubits_c128 qubits fits the device at 16 bytes per amplitude (MI355X: 33)
+    assert 16 * 2 ** r["max_qubits_c128"] <= r["total_bytes"]
+    Q.print_report(0)
+    out = capfd.readouterr().out
+    assert f"Max qubits: {r['max_qubits_c128']} (complex128), {r['max_qubits_c64']} (complex64)" in out
+    assert r["name"] in out
+
+
+def test_reference_import_name():
+    """`import QuantumSimEvo` (the reference's module name,
+    QuantumSimEvoPy.cpp:8) gives the Circuit API on the HIP backend."""
+    import QuantumSimEvo
+    c = QuantumSimEvo.Circuit(20)
+    c.hadamard(0)
+    for q in range(19):
+        c.cnot(q, q + 1)
+    c.execute()
+    assert set(int(x) for x in c.sample(500)) == {0, (1 << 20) - 1}

@@ -394,3 +394,30 @@ def test_oracle_adjoint_qaoa_closed_form_p1():
         assert abs(E - closed(gam, bet)) < 1e-12
         assert abs(g[0] - (closed(gam + h, bet) - closed(gam - h, bet)) / (2 * h)) < 1e-7
         assert abs(g[1] - (closed(gam, bet + h) - closed(gam, bet - h)) / (2 * h)) < 1e-7
+
+
+def test_oracle_measured_replay():
+    """Mid-circuit MEASURE / RESET_QUBIT replay (orc_run_measured) against closed
+    forms: a Bell pair's two measurements agree, the outcome rule is u < P(1),
+    reset leaves the qubit in |0> with the other qubits' amplitudes renormalised,
+    and unitary stretches equal the plain replay."""
+    n = 3
+    g = C.GateList(n).hadamard(0).cnot(0, 1).measure(0, 0).measure(1, 1)
+    for u0, want in ((0.2, 1), (0.7, 0), (0.4999, 1), (0.5001, 0)):  # P(q0 = 1) = 1/2 (to rounding)
+        psi, res = O.run_measured(O.basis(n), n, g.ops(), [u0, 0.9], 2)
+        assert res.tolist() == [want, want]
+        assert abs(psi[want * 3] - 1) < 1e-15 and abs(np.sum(np.abs(psi) ** 2) - 1) < 1e-15
+    # RY(t) on qubit 2, then reset: P(1) = sin^2(t/2); either branch ends in |0> on qubit 2,
+    # and an H after it gives |+> there
+    t = 1.0
+    p1 = math.sin(t / 2) ** 2
+    ops = C.GateList(n).pauli_x(0).rotate_y(t, 2).reset_qubit(2).hadamard(2).measure(2, 0).ops()
+    for u in (p1 - 1e-9, p1 + 1e-9):
+        psi, res = O.run_measured(O.basis(n), n, ops, [u, 0.25], 1)
+        want = np.zeros(8, dtype=complex)
+        want[0b001 | (4 if res[0] else 0)] = 1.0  # H then MEASURE(2) with u = 0.25 < 1/2 -> outcome 1
+        assert res[0] == 1 and np.max(np.abs(psi - want)) < 1e-15
+    # unitary stretches between measurements equal the plain replay
+    lay = C.random_layered(6, 4, seed=3)
+    psi, res = O.run_measured(O.basis(6), 6, lay.ops(), [], 0)
+    assert np.max(np.abs(psi - O.run_ops(O.basis(6), 6, lay.ops()))) == 0.0
