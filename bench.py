@@ -13,9 +13,10 @@ The JSON line also carries:
                 library's own stream during the timed steps), against 8 TB/s
   cpu_baseline  the reference simulator's AVX2+OpenMP backend
                 (oracle/_ref/libqse_ref.so, built from the reference sources)
-                timed on this host on a bounded prefix of the same QFT-30
-                gate list at 30 qubits; the restated scalar oracle when the
-                reference build is absent
+                timed on this host on the same QFT gate list, one thread per
+                physical core, median of 3 runs after a warm-up; the restated
+                scalar oracle on a 32-gate prefix when the reference build is
+                absent
   qaoa          QAOA-24q gradients/s, adjoint vs parameter shift (when built)
 """
 from __future__ import annotations
@@ -56,34 +57,100 @@ def profiled_traffic(kernel: str, n: int, precision: str):
     return None, None
 
 
-def cpu_baseline(n: int, ops: np.ndarray, prefix: int) -> dict:
-    """Reference AVX2/OpenMP backend (kind "reference") or the scalar oracle
-    (kind "port") on the first `prefix` gates of the workload's gate list."""
-    sub = np.ascontiguousarray(ops[:prefix])
-    upd = C.amp_updates(sub, n)
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+def host_cores() -> dict:
+    """Physical cores available to this process: the (core, socket) pairs that
+    `lscpu -p` lists for the CPUs of the process's affinity mask (sysfs topology
+    when lscpu is missing), capped by a cgroup CPU quota when one is set."""
+    import math
+    import subprocess
+    cpus = sorted(os.sched_getaffinity(0))
+    pairs = set()
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line and not line.startswith("#"):
+                c, core, sock = line.split(",")[:3]
+                if int(c) in cpus:
+                    pairs.add((sock, core))
+        how = "lscpu -p"
+    except (OSError, ValueError, subprocess.SubprocessError):
+        for c in cpus:
+            try:
+                top = Path(f"/sys/devices/system/cpu/cpu{c}/topology")
+                pairs.add(((top / "physical_package_id").read_text().strip(), (top / "core_id").read_text().strip()))
+            except OSError:
+                pairs.add(("?", str(c)))
+        how = "sysfs topology"
+    phys = max(len(pairs), 1)
+    quota = None
+    try:
+        q, per = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    use = min(phys, max(1, math.floor(quota))) if quota else phys
+    return {"physical": phys, "logical": len(cpus), "cgroup_cpus": quota, "threads": use, "source": how}
+
+
+def cpu_baseline(n: int, ops: np.ndarray, reps: int = 3) -> dict:
+    """The reference's AVX2+OpenMP backend (kind "reference", built from its
+    sources into oracle/_ref) on the whole gate list: one thread per physical
+    core (OMP_PROC_BIND=close, OMP_PLACES=cores), one untimed warm-up over the
+    first 32 gates (page faults, thread start), then the median of `reps` timed
+    replays, in a child process so the OpenMP settings apply from its start.
+    Without the reference build: the restated scalar oracle (kind "port", one
+    thread) on the first 32 gates."""
+    import subprocess
+    cores = host_cores()
     ref_path = ROOT / "oracle" / "_ref" / "libqse_ref.so"
     if ref_path.exists():
-        lib = ctypes.CDLL(str(ref_path))
-        lib.ref_time_ops.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
-                                     ctypes.c_void_p]
-        lib.ref_time_ops.restype = ctypes.c_double
-        secs = lib.ref_time_ops(1, n, None, sub.ctypes.data, len(sub), None)
-        kind, cores = "reference", threads
-        what = "CircuitUnitaryOperationAVX (AVX2+OpenMP) from the reference sources"
-    else:
-        lib = ctypes.CDLL(str(ROOT / "oracle" / "liboracle.so"))
-        lib.orc_run_ops.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
-        psi = np.zeros(1 << n, dtype=np.complex128)
-        psi[0] = 1
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores["threads"]), OMP_PROC_BIND="close", OMP_PLACES="cores")
         t0 = time.perf_counter()
-        lib.orc_run_ops(psi.ctypes.data, n, sub.ctypes.data, len(sub), None)
-        secs = time.perf_counter() - t0
-        kind, cores = "port", 1
-        what = "restated scalar FP64 oracle"
-    return {"value": upd / secs, "unit": "amp-updates/s", "cores": cores, "kind": kind,
-            "sample": f"first {prefix} gates of the {n}-qubit QFT gate list ({upd:.3e} amp-updates, "
-                      f"{secs:.2f} s) on {what}"}
+        r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-child", str(n), "--cpu-reps", str(reps)],
+                           env=env, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"cpu baseline child failed: {r.stderr[-2000:]}")
+        # the reference's StateVector prints its allocation time on stdout too
+        times = next(json.loads(x) for x in r.stdout.splitlines() if x.startswith('{"times"'))["times"]
+        secs = float(np.median(times))
+        upd = C.amp_updates(ops, n)
+        return {"value": upd / secs, "unit": "amp-updates/s", "cores": cores["threads"], "kind": "reference",
+                "sample": f"the whole {len(ops)}-gate QFT-{n} gate list ({upd:.4e} amp-updates) on "
+                          f"CircuitUnitaryOperationAVX (AVX2+OpenMP, built from the reference sources), "
+                          f"{cores['threads']} threads = physical cores ({cores['source']}: {cores['physical']} "
+                          f"physical / {cores['logical']} logical CPUs in the affinity mask, cgroup quota "
+                          f"{cores['cgroup_cpus']}), OMP_PROC_BIND=close OMP_PLACES=cores, one warm-up over the "
+                          f"first 32 gates, median of {reps} runs: {', '.join(f'{t:.2f}' for t in times)} s "
+                          f"(child wall {time.perf_counter() - t0:.1f} s)",
+                "seconds_median": secs, "seconds": times}
+    sub = np.ascontiguousarray(ops[:32])
+    upd = C.amp_updates(sub, n)
+    lib = ctypes.CDLL(str(ROOT / "oracle" / "liboracle.so"))
+    lib.orc_run_ops.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    psi = np.zeros(1 << n, dtype=np.complex128)
+    psi[0] = 1
+    t0 = time.perf_counter()
+    lib.orc_run_ops(psi.ctypes.data, n, sub.ctypes.data, len(sub), None)
+    secs = time.perf_counter() - t0
+    return {"value": upd / secs, "unit": "amp-updates/s", "cores": 1, "kind": "port",
+            "sample": f"first 32 gates of the {n}-qubit QFT gate list ({upd:.3e} amp-updates, {secs:.2f} s) on the "
+                      f"restated scalar FP64 oracle (the reference build is absent)"}
+
+
+def cpu_child(n: int, reps: int) -> int:
+    """Child process of cpu_baseline: times the reference AVX2 backend."""
+    ops = np.ascontiguousarray(C.qft(n).ops())
+    lib = ctypes.CDLL(str(ROOT / "oracle" / "_ref" / "libqse_ref.so"))
+    lib.ref_time_ops_repeat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    lib.ref_time_ops_repeat.restype = ctypes.c_int
+    times = np.zeros(reps)
+    rc = lib.ref_time_ops_repeat(1, n, ops.ctypes.data, len(ops), 32, 1, reps, times.ctypes.data)
+    if rc != 0:
+        return 1
+    print(json.dumps({"times": times.tolist()}), flush=True)
+    return 0
 
 
 def qaoa_leg(ctx, batch: int, shift_sets: int) -> dict:
@@ -151,11 +218,22 @@ def noise_leg(ctx) -> dict:
     st.run(ops)
     out = st.sample_frames(x, u)
     t2 = time.perf_counter()
+    # complex64 error against FP64: the same circuit on a complex128 state (which the tests hold to 1e-12 of
+    # the oracle), after the timed region
+    psi = st.download()
     st.close()
+    s128 = ctx.state(n, capi.C128)
+    s128.run(ops)
+    ref = s128.download()
+    s128.close()
+    fid = float(abs(np.vdot(ref, psi)) ** 2 / (np.vdot(psi, psi).real * np.vdot(ref, ref).real))
     return {"workload": f"{ntraj} noise trajectories x {shots} shots, {n}-qubit random layered circuit depth {depth}, "
                         "complex64 state, FP64 CDF", "seconds": t2 - t0, "frames_seconds": t1 - t0,
             "trajectories_per_s": ntraj / (t2 - t0), "shots_per_s": ntraj * shots / (t2 - t0),
-            "distinct_frames": int(len(np.unique(x))), "outcomes_checksum": int(np.bitwise_xor.reduce(out.ravel()))}
+            "distinct_frames": int(len(np.unique(x))), "outcomes_checksum": int(np.bitwise_xor.reduce(out.ravel())),
+            "c64_max_amp_err_vs_fp64": float(np.max(np.abs(psi - ref))), "c64_infidelity_vs_fp64": 1.0 - fid,
+            "sampling": "outcomes equal the sequential-CDF lower_bound on the complex64 amplitudes (bit-exact, "
+                        "tests/test_gpu_configs.py)"}
 
 
 def uneven_batch_leg(ctx) -> dict:
@@ -184,6 +262,19 @@ def uneven_batch_leg(ctx) -> dict:
     E, Gr = ctx.batch_run_small(capi.C128, circs, masks, w)
     t2 = time.perf_counter()
     nparams = sum(len(c[2]) for c in circs)
+    # independent FP64 values for a sample (the 8 largest circuits and 8 seeded others): the host-driven
+    # complex128 adjoint, a different code path from the engine (tests compare all 1024 and the oracle)
+    order = sorted(range(len(circs)), key=lambda i: (-circs[i][0], -len(circs[i][1])))
+    pick = order[:8] + [int(i) for i in np.random.default_rng(3).choice(order[8:], 8, replace=False)]
+    ctx.set_jit(capi.JIT_OFF)
+    eh = gh = e64h = g64h = 0.0
+    for i in pick:
+        nq, ops_i, par_i = circs[i]
+        zm, zw = C.zsum_observable(nq)
+        Eh, Gh = ctx.adjoint_expect(nq, capi.C128, ops_i, par_i, zm, zw)
+        eh, gh = max(eh, abs(E[i] - Eh[0])), max(gh, float(np.max(np.abs(Gr[i] - Gh[0]))))
+        e64h, g64h = max(e64h, abs(E64[i] - Eh[0])), max(g64h, float(np.max(np.abs(G64[i] - Gh[0]))))
+    ctx.set_jit(capi.JIT_AUTO)
     return {"workload": "1024 ansatz circuits, 10..22 qubits, depth 4..12, sum of Z, energies + adjoint gradients, "
                         "complex64 / FP64 accumulation, one persistent launch",
             "seconds": t1 - t0, "kernel_seconds": kern["ms"] / 1e3, "gradient_vectors_per_s": len(circs) / (t1 - t0),
@@ -191,8 +282,12 @@ def uneven_batch_leg(ctx) -> dict:
             "autotuned_shape": {"tile_qubits": tune["tile_qubits"], "low_qubits": tune["low_qubits"],
                                 "team_workgroups": tune["team_workgroups"],
                                 "sample": "the whole batch, 16 timed runs", "tuning_seconds": tune_s},
-            "max_abs_energy_err_vs_fp64": float(np.max(np.abs(E64 - E))),
-            "max_abs_grad_err_vs_fp64": float(max(np.max(np.abs(a - b)) for a, b in zip(G64, Gr)))}
+            "c64_max_abs_energy_err_vs_c128_engine": float(np.max(np.abs(E64 - E))),
+            "c64_max_abs_grad_err_vs_c128_engine": float(max(np.max(np.abs(a - b)) for a, b in zip(G64, Gr))),
+            "host_adjoint_check": {"circuits": len(pick), "c128_energy_err": eh, "c128_grad_err": gh,
+                                   "c64_energy_err": e64h, "c64_grad_err": g64h,
+                                   "reference": "qse_adjoint_expect complex128 (host-driven fused sweeps), one "
+                                                "call per circuit"}}
 
 
 def layered_leg(ctx, n: int = 30, layers: int = 20) -> dict:
@@ -403,7 +498,8 @@ def main() -> int:
     ap.add_argument("--jit", choices=["auto", "off", "always"], default="auto",
                     help="run-time specialised passes: compiled on the program's second run (auto), never, or at once")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-prefix", type=int, default=32)
+    ap.add_argument("--cpu-reps", type=int, default=3, help="timed runs of the CPU baseline (median reported)")
+    ap.add_argument("--cpu-child", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--no-qaoa", action="store_true")
     ap.add_argument("--qaoa-batch", type=int, default=64)
     ap.add_argument("--qaoa-shift-sets", type=int, default=1)
@@ -418,6 +514,8 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run X on qubit 0 (one read + one write of every amplitude) for counter calibration")
     args = ap.parse_args()
+    if args.cpu_child:
+        return cpu_child(args.cpu_child, args.cpu_reps)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: the library drives one GPU per process and this benchmark measures one (asked for --gpus {args.gpus})",
               file=sys.stderr)
@@ -463,7 +561,7 @@ def main() -> int:
         log(f"uneven batch: {result['uneven_batch']['seconds']:.3f} s")
     if "cpu" in legs:
         log("cpu baseline ...")
-        result["cpu_baseline"] = cpu_baseline(n, ops, args.cpu_prefix)
+        result["cpu_baseline"] = cpu_baseline(n, ops, args.cpu_reps)
         log(f"cpu baseline: {result['cpu_baseline']['value']:.3e} amp-updates/s")
     print(json.dumps(result), flush=True)
     ctx.close()

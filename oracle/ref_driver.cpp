@@ -13,6 +13,7 @@
  *
  * Nothing here is part of the product path.
  */
+#include <algorithm>
 #include <chrono>
 #include <cstdint>
 #include <cstring>
@@ -121,6 +122,25 @@ double ref_time_ops(int backend, int n, const double* psi_init, const qse_op* op
     replay(*b, sv, ops, nops, params);
     const auto t1 = std::chrono::steady_clock::now();
     return std::chrono::duration<double>(t1 - t0).count();
+}
+
+/* Timing protocol of the CPU baseline: one StateVector (allocated and zero-filled
+ * once by its constructor, StateVector.cpp:14-32), `warmups` untimed replays of
+ * ops[0..warm_ops) (page faults, thread pools), then `reps` timed replays of the
+ * whole list, each time written to times[]. The state keeps evolving between
+ * replays; the work per replay is the same. */
+int ref_time_ops_repeat(int backend, int n, const qse_op* ops, int nops, int warm_ops, int warmups, int reps,
+                        double* times) {
+    StateVector sv((size_t)n);
+    auto b = make_backend(backend);
+    for (int w = 0; w < warmups; ++w) replay(*b, sv, ops, std::min(warm_ops, nops), nullptr);
+    for (int r = 0; r < reps; ++r) {
+        const auto t0 = std::chrono::steady_clock::now();
+        replay(*b, sv, ops, nops, nullptr);
+        const auto t1 = std::chrono::steady_clock::now();
+        times[r] = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return 0;
 }
 
 /* Drive the reference Noise class over a gate list exactly as Circuit::execute

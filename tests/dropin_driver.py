@@ -6,10 +6,10 @@ name (QuantumSimEvo) does not meet the package's. Prints one JSON object.
   python tests/dropin_driver.py <module dir> describe
   python tests/dropin_driver.py <module dir> run
 """
-import contextlib
-import io
 import json
+import os
 import sys
+import tempfile
 
 sys.path.insert(0, sys.argv[1])
 import QuantumSimEvo as Q  # noqa: E402
@@ -43,11 +43,19 @@ def main():
     c.phase(0.4, 2); c.pauli_y(3); c.cphase(0.9, 1, 2); c.mcphase(1.3, [0, 1], 3); c.swap(0, 3)
     c.pauli_x(1); c.pauli_z(2); c.hadamard(3)
     c.execute()
-    buf = io.StringIO()
-    with contextlib.redirect_stdout(buf):
-        c.print_state()
+    # printState writes through std::cout: capture file descriptor 1
     sys.stdout.flush()
-    out["state4"] = {str(k): [v.real, v.imag] for k, v in parse_state(buf.getvalue()).items()}
+    saved = os.dup(1)
+    with tempfile.TemporaryFile(mode="w+") as tmp:
+        os.dup2(tmp.fileno(), 1)
+        try:
+            c.print_state()
+        finally:
+            os.dup2(saved, 1)
+            os.close(saved)
+        tmp.seek(0)
+        text = tmp.read()
+    out["state4"] = {str(k): [v.real, v.imag] for k, v in parse_state(text).items()}
     # 2. GHZ on 20 qubits: samples and one measurement
     g = Q.Circuit(20)
     g.hadamard(0)
