@@ -342,6 +342,7 @@ def dense_block_leg(ctx, restore_jit: int, n: int = 30) -> dict:
         for layers in (1, 16, 64):
             ops = C.random_layered(m, layers, seed=layers).ops()
             u = capi.compose_dense(m, ops)
+            ctx.set_dense_blocks(0)  # the fused gate path, for the comparison
             for _ in range(2):
                 st.run(ops)
                 st.apply_dense(m, u)
@@ -355,15 +356,25 @@ def dense_block_leg(ctx, restore_jit: int, n: int = 30) -> dict:
             ctx.sync()
             ctx.enable_timing(False)
             f, d = ctx.timing(capi.FAM_FUSED), ctx.timing(capi.FAM_DENSE)
+            # qse_run with its default routing: which path it takes for this block
+            ctx.set_dense_blocks(20)
+            ctx.reset_timing()
+            ctx.enable_timing(True)
+            st.run(ops)
+            ctx.sync()
+            ctx.enable_timing(False)
+            routed = "mfma" if ctx.timing(capi.FAM_DENSE)["launches"] else "fused"
+            ctx.set_dense_blocks(0)
             a.set_random(3)
             b.set_random(3)
             a.run(ops)
             b.apply_dense(m, u)
             diff = float(np.max(np.abs(a.download() - b.download())))
+            ctx.set_dense_blocks(20)
             fms, dms = f["ms"] / max(f["launches"], 1), d["ms"] / max(d["launches"], 1)
             rows.append({"m": m, "layers": layers, "gates": int(len(ops)), "fused_ms": fms, "dense_mfma_ms": dms,
                          "dense_gbs": 2.0 * 16 * 2.0 ** n / (dms * 1e-3) / 1e9, "faster": "mfma" if dms < fms else "fused",
-                         "max_abs_diff_20q": diff})
+                         "qse_run_takes": routed, "max_abs_diff_20q": diff})
             log(f"dense block m={m} layers={layers}: fused {fms:.3f} ms, MFMA {dms:.3f} ms")
     ctx.set_jit(restore_jit)
     for s_ in (st, a, b):

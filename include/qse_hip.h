@@ -273,6 +273,15 @@ int qse_copy_bandwidth(qse_ctx* ctx, int log2_elems, int run_log2, double* gbs);
  * other m. Stream-ordered. */
 int qse_compose_dense(int m, const qse_op* ops, int nops, const double* params, double* u);
 int qse_apply_dense(qse_ctx* ctx, qse_state* st, int m, const double* u);
+/* Routing of qse_run to the matrix cores (no counterpart in the reference): a
+ * maximal run of consecutive ops confined to qubits 0..m-1 (m = 4, 5 or 6,
+ * the highest touched qubit + 1, at least 4) with at least ops_per_qubit * m
+ * ops, on a complex128 state of m + 4 qubits or more, runs as one
+ * qse_apply_dense pass of its composed unitary instead of fused gate passes.
+ * Default 20 (about 16 layers of one-qubit gates plus a CNOT brickwork, the
+ * measured crossover on MI355X); 0 turns the routing off. qse_run_repeat and
+ * qse_run_sweeps never route. */
+int qse_ctx_set_dense_blocks(qse_ctx* ctx, int ops_per_qubit);
 
 /* ---- readout (StateVector.cpp:77-156), driven by host-supplied uniforms ----
  * u are canonical uniforms in [0,1) (std::generate_canonical<double,53>); the

@@ -282,3 +282,172 @@ def optimize_maxcut(ctx, n: int, edges, p: int, params0, steps: int = 40, lr: fl
     E, _ = ctx.adjoint_expect(n, prec, ops, x, masks, w)
     hist.append(float(E[0]))
     return x[0], hist
+
+
+# ---- TSP as a constraint-preserving QAOA (QSEplayground/TVS.h:1-71, TVS.cpp:10-381) ----
+#
+# Binary variable x_{v,t} = 1: city v is visited at step t, on qubit v*n + t (n^2
+# qubits). QUBO cost of a closed tour:
+#   A sum_v (1 - sum_t x_{v,t})^2 + A sum_t (1 - sum_v x_{v,t})^2
+#   + B sum_t sum_{u != v} w_uv x_{u,t} x_{v,t+1 mod n}.
+# With x = (1 - Z)/2 a term c x_i becomes -c/2 Z_i (+ const) and q x_i x_j becomes
+# q/4 (Z_i Z_j - Z_i - Z_j) (+ const). The circuit starts in the identity tour
+# (city v at step v), applies p layers of the phase separator e^{-i gamma H_C}
+# (route cost and the one-city-per-step penalty: the row constraint is kept by
+# the mixer) and an XY ring mixer e^{-i beta (X_a X_b + Y_a Y_b)} on neighbouring
+# steps of each city's row, which moves the row's single excitation and so never
+# leaves the one-step-per-city subspace.
+
+def tsp_index(city: int, step: int, n: int) -> int:
+    """Qubit of x_{city,step} (TVS.cpp:9-12)."""
+    return city * n + step
+
+
+def tsp_ising(n: int, penalty: float, weight: float, dist, rows: bool = True, cols: bool = True):
+    """Ising form h_i Z_i + J_ij Z_i Z_j (+ a constant) of the TSP QUBO above:
+    returns (h, pairs, coeffs) with pairs (i < j) and J in matching order
+    (TVS::buildIsing, TVS.cpp:30-101; rows / cols select the two penalties)."""
+    q = n * n
+    h = np.zeros(q)
+    J: dict[tuple[int, int], float] = {}
+
+    def linear(i, c):
+        h[i] -= c / 2.0
+
+    def quad(i, j, c):
+        h[i] -= c / 4.0
+        h[j] -= c / 4.0
+        key = (min(i, j), max(i, j))
+        J[key] = J.get(key, 0.0) + c / 4.0
+
+    def one_hot(group):
+        # A (1 - S)^2 = A (1 - sum x_i + 2 sum_{i<j} x_i x_j) for 0/1 variables
+        for a in range(len(group)):
+            linear(group[a], -penalty)
+            for b in range(a + 1, len(group)):
+                quad(group[a], group[b], 2.0 * penalty)
+
+    if rows:
+        for v in range(n):
+            one_hot([tsp_index(v, t, n) for t in range(n)])
+    if cols:
+        for t in range(n):
+            one_hot([tsp_index(v, t, n) for v in range(n)])
+    for u in range(n):
+        for v in range(n):
+            if u != v:
+                for t in range(n):
+                    quad(tsp_index(u, t, n), tsp_index(v, (t + 1) % n, n), weight * dist[u][v])
+    pairs = [k for k, c in J.items() if abs(c) >= 1e-15]
+    return h, pairs, [J[k] for k in pairs]
+
+
+def tsp_qubo_cost(bits: int, n: int, dist, penalty: float, weight: float) -> float:
+    """QUBO cost of an assignment (bit v*n + t = x_{v,t}) (TVS.cpp:232-262)."""
+    x = [(bits >> i) & 1 for i in range(n * n)]
+    cost = 0.0
+    for v in range(n):
+        s = sum(x[tsp_index(v, t, n)] for t in range(n))
+        cost += penalty * (1 - s) ** 2
+    for t in range(n):
+        s = sum(x[tsp_index(v, t, n)] for v in range(n))
+        cost += penalty * (1 - s) ** 2
+    for t in range(n):
+        for u in range(n):
+            for v in range(n):
+                if u != v and x[tsp_index(u, t, n)] and x[tsp_index(v, (t + 1) % n, n)]:
+                    cost += weight * dist[u][v]
+    return cost
+
+
+def tsp_circuit(n: int, penalty: float, weight: float, gammas, betas, dist) -> GateList:
+    """The p-layer constraint-preserving QAOA of TVS::createCircuit
+    (TVS.cpp:121-186): identity tour; per layer the ZZ phases (CNOT, RZ(2 gamma
+    J), CNOT), the Z phases RZ(2 gamma h), then for every row v and step t the
+    XY interaction on (v,t), (v,t+1 mod n) as Rxx(2 beta) (H-conjugated Rzz)
+    followed by Ryy(2 beta) (RX(pi/2)-conjugated Rzz)."""
+    h, pairs, coeffs = tsp_ising(n, penalty, weight, dist, rows=False, cols=True)
+    g = GateList(n * n)
+    for v in range(n):
+        g.pauli_x(tsp_index(v, v, n))
+
+    def rzz(a, b, theta):
+        g.cnot(a, b).rotate_z(theta, b).cnot(a, b)
+
+    for gamma, beta in zip(gammas, betas):
+        for (a, b), c in zip(pairs, coeffs):
+            rzz(a, b, 2.0 * gamma * c)
+        for q in range(n * n):
+            g.rotate_z(2.0 * gamma * h[q], q)
+        for v in range(n):
+            for t in range(n):
+                a, b = tsp_index(v, t, n), tsp_index(v, (t + 1) % n, n)
+                g.hadamard(a).hadamard(b)
+                rzz(a, b, 2.0 * beta)
+                g.hadamard(a).hadamard(b)
+                g.rotate_x(PI / 2, a).rotate_x(PI / 2, b)
+                rzz(a, b, 2.0 * beta)
+                g.rotate_x(-PI / 2, a).rotate_x(-PI / 2, b)
+    return g
+
+
+def tsp_decode(sample: int, n: int) -> list[int] | None:
+    """Visiting order [city at step 0, ...], or None when some step has not
+    exactly one city or a city repeats (TVS::decodeSample, TVS.cpp:191-210)."""
+    order, used = [], set()
+    for t in range(n):
+        cities = [v for v in range(n) if (sample >> tsp_index(v, t, n)) & 1]
+        if len(cities) != 1 or cities[0] in used:
+            return None
+        used.add(cities[0])
+        order.append(cities[0])
+    return order
+
+
+def tour_length(dist, order) -> float:
+    """Closed tour length (TVS.cpp:267-274)."""
+    return sum(dist[order[i]][order[(i + 1) % len(order)]] for i in range(len(order)))
+
+
+def tsp_brute_force(dist) -> tuple[list[int], float]:
+    """Shortest closed tour with city 0 first (TVS.cpp:276-294)."""
+    import itertools
+    n = len(dist)
+    best, best_len = None, math.inf
+    for perm in itertools.permutations(range(1, n)):
+        order = [0, *perm]
+        length = tour_length(dist, order)
+        if length < best_len:
+            best, best_len = order, length
+    return best, best_len
+
+
+def tsp_schedule(layers: int, gamma: float = 0.8, beta: float = 0.4):
+    """Per-layer angles of TVS::runTVS (TVS.cpp:320-327): gamma ramps up,
+    beta ramps down and keeps a quarter on the last layer."""
+    gammas, betas = [], []
+    for k in range(layers):
+        frac = 1.0 if layers == 1 else (k + 1) / layers
+        gammas.append(gamma * frac)
+        betas.append(beta * (1.0 - frac) + beta * 0.25)
+    return gammas, betas
+
+
+def tsp_penalty(dist, weight: float = 1.0) -> float:
+    """Automatic penalty A = B * max w * n + 1 (TVS.cpp:311-317)."""
+    n = len(dist)
+    return weight * max(max(r) for r in dist) * n + 1.0
+
+
+def best_sampled_tour(samples, n: int, dist):
+    """Shortest feasible tour among the samples and how often it was drawn
+    (TVS.cpp:335-357): (order, length, count) or (None, inf, 0)."""
+    vals, counts = np.unique(np.asarray(samples, dtype=np.uint64), return_counts=True)
+    best, best_len, best_count = None, math.inf, 0
+    for s, c in zip(vals, counts):
+        order = tsp_decode(int(s), n)
+        if order is not None:
+            length = tour_length(dist, order)
+            if length < best_len:
+                best, best_len, best_count = order, length, int(c)
+    return best, best_len, best_count

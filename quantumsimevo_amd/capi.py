@@ -45,7 +45,7 @@ EXPORTS = [
     "qse_measure", "qse_norm2", "qse_prob_one", "qse_collapse_qubit", "qse_expect_zstrings", "qse_expect_zstrings_noalloc",
     "qse_adjoint_expect", "qse_batch_expect", "qse_batch_run_small", "qse_sample_frames",
     "qse_compose_dense", "qse_apply_dense", "qse_state_renormalize", "qse_ctx_set_renormalize",
-    "qse_ctx_set_adjoint_terms", "qse_batch_autotune",
+    "qse_ctx_set_adjoint_terms", "qse_batch_autotune", "qse_ctx_set_dense_blocks",
 ]
 
 
@@ -127,6 +127,7 @@ def _sigs(lib):
         "qse_batch_autotune": ([_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, C.POINTER(_i), C.POINTER(_i),
                                 C.POINTER(_i), _pd], _i),
         "qse_apply_dense": ([_vp, _vp, _i, _vp], _i),
+        "qse_ctx_set_dense_blocks": ([_vp, _i], _i),
         "qse_sample": ([_vp, _vp, _vp, _i64, _vp], _i),
         "qse_measure": ([_vp, _vp, _d, _pu64], _i),
         "qse_norm2": ([_vp, _vp, _pd], _i),
@@ -236,6 +237,11 @@ class Context:
         """ADJ_TERMS_AUTO (default), ADJ_TERMS_ROTATE or ADJ_TERMS_PAIRS: how the
         adjoint evaluates RX/RY generator terms on 13+ qubit states."""
         check(self.lib.qse_ctx_set_adjoint_terms(self.h, int(mode)), "qse_ctx_set_adjoint_terms")
+
+    def set_dense_blocks(self, ops_per_qubit: int):
+        """qse_run's routing of deep blocks on qubits 0..5 to the matrix cores
+        (threshold in ops per qubit of the block; 0 = never)."""
+        check(self.lib.qse_ctx_set_dense_blocks(self.h, ops_per_qubit), "qse_ctx_set_dense_blocks")
 
     def set_renormalize(self, every_passes: int):
         """Renormalise states (FP64, on the device) once qse_run calls have made

@@ -78,18 +78,6 @@ __device__ void batch_pass(cx<R>* s, int n, const FPass& Pg, const BArgs& A, cx<
     constexpr int TB = K - 4;
     const int tid = threadIdx.x;
     const bool act = tid < NT;
-    uint64_t gt = 0, ot = 0;
-    uint32_t st_t = 0;
-    if (act) {
-#pragma unroll
-        for (int b = 0; b < TB; ++b)
-            if ((tid >> b) & 1) {
-                gt |= 1ULL << P.tq[b];
-                ot |= 1ULL << P.so_q[b];
-                st_t |= 1u << P.so_l[b];
-            }
-    }
-    const uint32_t swz_tid = swz((uint32_t)tid), swz_st = swz(st_t);
     const uint64_t ntiles = 1ULL << (n - K);
     // Tile index -> source / destination base addresses: the external qubits in
     // groups of 6 tile-index bits, each group's 64 deposits tabulated once per
@@ -111,6 +99,26 @@ __device__ void batch_pass(cx<R>* s, int n, const FPass& Pg, const BArgs& A, cx<
     }
     __syncthreads();
     for (uint64_t ti = (uint64_t)r; ti < ntiles; ti += (uint64_t)T) {
+        // The pass's fields and this thread's tile offsets are read and computed
+        // again for every tile: the pass pointer and the thread id go through
+        // empty asm barriers, so the compiler cannot hoist them out of the tile
+        // loop, where they would hold scalar and vector registers for its whole
+        // length (and spill).
+        const FPass* Pp = &Pg;
+        int tidv = tid;
+        __asm__ volatile("" : "+s"(Pp));
+        __asm__ volatile("" : "+v"(tidv));
+        const auto& P = *kc(Pp);
+        uint64_t gt = 0, ot = 0;
+        uint32_t st_t = 0;
+#pragma unroll
+        for (int b = 0; b < TB; ++b)
+            if ((tidv >> b) & 1) {
+                gt |= 1ULL << P.tq[b];
+                ot |= 1ULL << P.so_q[b];
+                st_t |= 1u << P.so_l[b];
+            }
+        const uint32_t swz_tid = swz((uint32_t)tidv), swz_st = swz(st_t);
         uint64_t base = 0, obase = 0;
         for (int lvl = 0; lvl < nlev; ++lvl) {
             const int j = (int)((ti >> (lvl * BATCH_DEP_BITS)) & ((1 << BATCH_DEP_BITS) - 1)) + (lvl << BATCH_DEP_BITS);

@@ -14,7 +14,7 @@
 
 namespace qse {
 
-constexpr int BATCH_DIAG = 16;       // Z-string masks per reduction step
+constexpr int BATCH_DIAG = 32;       // Z-string masks per reduction step
 constexpr int BATCH_MAX_TERMS = 64;  // observable terms (one bit each in BCirc::obs_keep)
 
 enum BStepKind : int32_t {

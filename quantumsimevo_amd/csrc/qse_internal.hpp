@@ -157,6 +157,9 @@ struct qse_ctx {
     int batch_team[2] = {0, 0};  // workgroups per team for circuits of 16+ qubits (0: 16)
     // Renormalise states after this many passes of qse_run calls (0: never).
     int64_t renorm_every = 0;
+    // qse_run sends runs of at least this many ops per qubit on qubits 0..m-1
+    // (m = 4..6) to the matrix cores as one dense block (0: never).
+    int dense_per_qubit = 20;
     // Fusion shape per precision (tile qubits k, contiguous low qubits L).
     int tile_bits[2] = {0, 0};
     int low_bits[2] = {0, 0};

@@ -112,3 +112,70 @@ def test_max_cut_helpers():
     edges = [(0, 1), (1, 2), (2, 3), (3, 0)]
     assert A.brute_force_max_cut(4, edges)[0] == 4
     assert A.best_sampled_cut([0, 0b0101, 0b0011], edges) == (4, 0b0101)
+
+
+def _tsp_instance(n, seed):
+    rng = np.random.default_rng(seed)
+    pts = rng.random((n, 2)) * 10
+    return [[float(np.hypot(*(pts[a] - pts[b]))) for b in range(n)] for a in range(n)]
+
+
+@pytest.mark.parametrize("n,rows,cols", [(3, True, True), (4, True, True), (4, False, True), (3, True, False)])
+def test_tsp_ising_equals_qubo_up_to_a_constant(n, rows, cols):
+    """The Ising coefficients (tsp_ising) against a brute-force evaluation of the
+    QUBO (TVS::quboCost, TVS.cpp:232-262) over every assignment of the n^2
+    variables: with z = 1 - 2x, QUBO(x) - (sum h z + sum J z z) is one constant.
+    A penalty left out of the Ising form is subtracted from the QUBO."""
+    dist = _tsp_instance(n, 3)
+    pen, wgt = 17.5, 1.3
+    h, pairs, coeffs = A.tsp_ising(n, pen, wgt, dist, rows=rows, cols=cols)
+    q = n * n
+    xs = np.arange(1 << q, dtype=np.int64)
+    z = 1.0 - 2.0 * ((xs[:, None] >> np.arange(q)) & 1)
+    ising = z @ h
+    for (a, b), c in zip(pairs, coeffs):
+        ising += c * z[:, a] * z[:, b]
+    qubo = np.array([A.tsp_qubo_cost(int(x), n, dist, pen, wgt) for x in xs])
+    bits = (xs[:, None] >> np.arange(q)) & 1
+    if not rows:
+        qubo -= pen * sum((1 - bits[:, [A.tsp_index(v, t, n) for t in range(n)]].sum(1)) ** 2 for v in range(n))
+    if not cols:
+        qubo -= pen * sum((1 - bits[:, [A.tsp_index(v, t, n) for v in range(n)]].sum(1)) ** 2 for t in range(n))
+    d = qubo - ising
+    assert np.max(np.abs(d - d[0])) < 1e-9
+
+
+def test_tsp_classical_helpers():
+    dist = _tsp_instance(5, 8)
+    best, best_len = A.tsp_brute_force(dist)
+    import itertools
+    every = min(A.tour_length(dist, list(p)) for p in itertools.permutations(range(5)))
+    assert abs(best_len - every) < 1e-12 and best[0] == 0
+    n = 4
+    ident = sum(1 << A.tsp_index(v, v, n) for v in range(n))
+    assert A.tsp_decode(ident, n) == [0, 1, 2, 3]
+    assert A.tsp_decode(ident | 1 << A.tsp_index(0, 1, n), n) is None  # two cities at step 1
+    assert A.tsp_decode(ident & ~(1 << A.tsp_index(2, 2, n)), n) is None  # empty step
+    # a feasible assignment's QUBO cost is its route length times the weight
+    assert abs(A.tsp_qubo_cost(ident, n, dist, 99.0, 2.0) - 2.0 * A.tour_length(dist, [0, 1, 2, 3])) < 1e-12
+
+
+def test_tsp_qaoa_on_the_oracle_finds_the_optimum():
+    """4 cities (16 qubits), the reference's defaults (4 layers, gamma 0.8,
+    beta 0.4, automatic penalty): the XY mixer keeps one excitation per city
+    row (no amplitude elsewhere) and the best of 20 000 sampled feasible tours
+    is the brute-force optimum."""
+    n = 4
+    dist = _tsp_instance(n, 7)
+    gam, bet = A.tsp_schedule(4)
+    g = A.tsp_circuit(n, A.tsp_penalty(dist), 1.0, gam, bet, dist)
+    psi = O.run_ops(O.basis(n * n), n * n, g.ops())
+    idx = np.arange(1 << (n * n))
+    rows_ok = np.ones(len(idx), dtype=bool)
+    for v in range(n):
+        cnt = sum((idx >> A.tsp_index(v, t, n)) & 1 for t in range(n))
+        rows_ok &= cnt == 1
+    assert np.max(np.abs(psi[~rows_ok])) < 1e-12
+    s = O.sample(psi, n * n, O.uniforms(5, 20000))
+    order, length, count = A.best_sampled_tour(s, n, dist)
+    assert abs(length - A.tsp_brute_force(dist)[1]) < 1e-12 and count > 0

@@ -89,3 +89,42 @@ def test_rejections(ctx):
     with pytest.raises(ValueError):
         st.apply_dense(4, np.eye(8))        # wrong matrix shape
     st.close()
+
+
+def test_qse_run_routes_deep_low_qubit_blocks(ctx):
+    """qse_run sends a run of at least 20 m ops on qubits 0..m-1 (m = 4..6) to
+    the matrix cores (one qse_apply_dense pass) and fuses the rest; results
+    equal the oracle (1e-12) with the routing on and off, and the timing
+    families show which path ran."""
+    n = 20
+    head = G.random_layered(n, 2, seed=1)
+    block = G.random_layered(5, 64, seed=2)   # 64 layers on qubits 0..4: 448 ops >= 20 * 5
+    short = G.random_layered(4, 3, seed=3)    # 18 ops < 20 * 4: stays fused
+    tail = G.random_layered(n, 2, seed=4)
+    ops = np.concatenate([head.ops(), block.ops(), tail.ops(), short.ops()])
+    psi = G.random_state(n, 9)
+    want = O.run_ops(psi, n, ops)
+    for per, dense_launches in ((20, 1), (0, 0)):
+        ctx.set_dense_blocks(per)
+        st = ctx.state(n)
+        st.upload(psi)
+        ctx.reset_timing()
+        ctx.enable_timing(True)
+        st.run(ops)
+        ctx.sync()
+        ctx.enable_timing(False)
+        assert ctx.timing(capi.FAM_DENSE)["launches"] == dense_launches
+        assert np.max(np.abs(st.download() - want)) < 1e-12
+        st.close()
+    ctx.set_dense_blocks(20)
+    # complex64 states never route
+    st = ctx.state(n, capi.C64)
+    st.upload(psi)
+    ctx.reset_timing()
+    ctx.enable_timing(True)
+    st.run(ops)
+    ctx.sync()
+    ctx.enable_timing(False)
+    assert ctx.timing(capi.FAM_DENSE)["launches"] == 0
+    assert np.max(np.abs(st.download() - want)) < 2e-5
+    st.close()
