@@ -98,15 +98,43 @@ __device__ void batch_pass(cx<R>* s, int n, const FPass& Pg, const BArgs& A, cx<
         dep[BATCH_DEP_HALF + idx] = o;
     }
     __syncthreads();
+    // Complex64: the next tile's amplitudes are loaded into registers while
+    // this tile's stages run (its 16 amplitudes fit the register budget of two
+    // waves per SIMD; complex128's would not). A workgroup's tiles are its own
+    // in the pass, so the prefetch never reads what another one writes.
+    constexpr bool PF = sizeof(R) == 4;
+    auto load_tile = [&](const FPass& Q, uint64_t t, int tv, cx<R>* out) {
+        uint64_t b = 0, g0 = 0;
+        for (int lvl = 0; lvl < nlev; ++lvl)
+            b |= dep[(int)((t >> (lvl * BATCH_DEP_BITS)) & ((1 << BATCH_DEP_BITS) - 1)) + (lvl << BATCH_DEP_BITS)];
+#pragma unroll
+        for (int q = 0; q < TB; ++q)
+            if ((tv >> q) & 1) g0 |= 1ULL << Q.tq[q];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            uint64_t g = b | g0;
+            if (i & 1) g |= 1ULL << Q.tq[TB + 0];
+            if (i & 2) g |= 1ULL << Q.tq[TB + 1];
+            if (i & 4) g |= 1ULL << Q.tq[TB + 2];
+            if (i & 8) g |= 1ULL << Q.tq[TB + 3];
+            out[i] = s[g];
+        }
+    };
+    cx<R> ld[16];
+    if (PF && act && (uint64_t)r < ntiles) load_tile(P, (uint64_t)r, tid, ld);
     for (uint64_t ti = (uint64_t)r; ti < ntiles; ti += (uint64_t)T) {
         // The pass's fields and this thread's tile offsets are read and computed
         // again for every tile: the pass pointer and the thread id go through
         // empty asm barriers, so the compiler cannot hoist them out of the tile
         // loop, where they would hold scalar and vector registers for its whole
         // length (and spill).
-        const FPass* Pp = &Pg;
+        // (The pointer is uniform: readfirstlane places it in scalar registers
+        // for the barrier whatever register class its producer used.)
+        const uint64_t pw = (uint64_t)&Pg;
+        uint32_t plo = __builtin_amdgcn_readfirstlane((uint32_t)pw), phi = __builtin_amdgcn_readfirstlane((uint32_t)(pw >> 32));
+        __asm__ volatile("" : "+s"(plo), "+s"(phi));
+        const FPass* Pp = (const FPass*)(((uint64_t)phi << 32) | plo);
         int tidv = tid;
-        __asm__ volatile("" : "+s"(Pp));
         __asm__ volatile("" : "+v"(tidv));
         const auto& P = *kc(Pp);
         uint64_t gt = 0, ot = 0;
@@ -125,8 +153,7 @@ __device__ void batch_pass(cx<R>* s, int n, const FPass& Pg, const BArgs& A, cx<
             base |= dep[j];
             obase |= dep[BATCH_DEP_HALF + j];
         }
-        cx<R> ld[16];
-        if (act) {
+        if (!PF && act) {
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 uint64_t g = base | gt;
@@ -152,6 +179,7 @@ __device__ void batch_pass(cx<R>* s, int n, const FPass& Pg, const BArgs& A, cx<
             for (int i = 0; i < 16; ++i) tile[swz_tid ^ swz((uint32_t)(i * NT))] = ld[i];
         }
         __syncthreads();
+        if (PF && act && ti + (uint64_t)T < ntiles) load_tile(P, ti + (uint64_t)T, tidv, ld);
         for (int sg = 0; sg < P.nstages; ++sg) {
             const auto& S = *kc(A.stages + P.stage_begin + sg);
             if (act) {
@@ -279,12 +307,54 @@ __device__ double block_sum(double v, double* red) {
     return t;
 }
 
-__device__ double obs_factor(uint64_t i, uint64_t keep, const uint64_t* __restrict__ m, const double* __restrict__ w,
-                             int nterms) {
+// Reductions and observables run over blocks of WB = 8 amplitudes
+// i = i0 + stride j (j < 8), whose sums against Z-strings come from one
+// in-register Walsh-Hadamard transform of the block.
+constexpr int WB = 8, WB_LOG = 3;
+
+// Observable terms of the running circuit, in LDS: masks and weights of the
+// kept terms (their qubits lie on the circuit), grouped by the 3-bit field
+// u = (mask >> log2(stride)) & 7 that selects a Walsh-Hadamard coefficient
+// (group u: entries [ubeg[u], ubeg[u + 1])); built once per circuit.
+struct ObsLds {
+    uint64_t m[BATCH_MAX_TERMS];
+    double w[BATCH_MAX_TERMS];
+    int ubeg[WB + 1];
+    int n;
+    // Z-string step: bit d of kflip[b] = bit (b + log2(WB stride)) of mask d
+    uint32_t kflip[64];
+};
+
+__device__ __forceinline__ double obs_factor(uint64_t i, const ObsLds& O) {
     double f = 0.0;
-    for (int t = 0; t < nterms; ++t)
-        if ((keep >> t) & 1) f += (__popcll(i & m[t]) & 1) ? -w[t] : w[t];
+    for (int t = 0; t < O.n; ++t) f += (__popcll(i & O.m[t]) & 1) ? -O.w[t] : O.w[t];
     return f;
+}
+
+// In-register Walsh-Hadamard transform of WB values: w[u] <- sum_j w[j] (-1)^popcount(j & u).
+__device__ __forceinline__ void wht(double* w) {
+#pragma unroll
+    for (int h = 1; h < WB; h <<= 1)
+#pragma unroll
+        for (int j = 0; j < WB; ++j)
+            if (!(j & h)) {
+                const double a = w[j], b = w[j + h];
+                w[j] = a + b;
+                w[j + h] = a - b;
+            }
+}
+
+// G[u] = sum over the terms of group u of w_t (-1)^parity(i0 & m_t): with
+// i0 = first + WB stride kbi and i = i0 + stride j, f(i) = sum_u G[u]
+// (-1)^popcount(j & u) (first < stride, so the bits of first, of WB stride kbi
+// and of stride j are disjoint).
+__device__ __forceinline__ void obs_groups(const ObsLds& O, uint64_t i0, double* G) {
+#pragma unroll
+    for (int u = 0; u < WB; ++u) {
+        double g = 0.0;
+        for (int t = O.ubeg[u]; t < O.ubeg[u + 1]; ++t) g += (__popcll(i0 & O.m[t]) & 1) ? -O.w[t] : O.w[t];
+        G[u] = g;
+    }
 }
 
 // Bound on the iterations of one barrier wait (each sleeps ~128 cycles): about
@@ -324,7 +394,7 @@ __device__ bool team_sync(Team& tm, int* s_flag) {
         *s_flag = ok;
     }
     __syncthreads();
-    return *s_flag != 0;
+    return __builtin_amdgcn_readfirstlane(*s_flag) != 0;  // one LDS word for all lanes
 }
 
 struct BKernelArgs {
@@ -344,7 +414,7 @@ struct BKernelArgs {
 // barrier failed.
 template <class R>
 __device__ __attribute__((always_inline)) bool run_circuit(const BCirc& C, Team& tm, const BKernelArgs& K, cx<R>* tile, SlotVal* slotv,
-                            uint64_t* dep, double* red, int* s_flag) {
+                            uint64_t* dep, double* red, ObsLds& O, int* s_flag) {
     const int tid = threadIdx.x, nth = blockDim.x;
     cx<R>* arena = (cx<R>*)K.arena_d;
     cx<R>* psi = arena + C.psi_off;
@@ -352,6 +422,28 @@ __device__ __attribute__((always_inline)) bool run_circuit(const BCirc& C, Team&
     const uint64_t N = 1ULL << C.n;
     const uint64_t first = (uint64_t)tm.r * nth + tid, stride = (uint64_t)tm.T * nth;
     double* part = K.tpart + (size_t)tm.id * tm.T * BATCH_DIAG;
+    // Reductions and the observable run over blocks of WB strides when the
+    // state holds at least WB strides; below, element by element.
+    const int log2S = __builtin_ctzll(stride);
+    const bool blocked = (stride & (stride - 1)) == 0 && N >= WB * stride;
+    const uint64_t nblk = blocked ? N / (WB * stride) : 0;
+    // the circuit's observable terms in LDS, grouped by their Walsh-Hadamard field
+    if (tid == 0) {
+        int cnt[WB] = {};
+        for (int t = 0; t < K.nterms; ++t)
+            if ((C.obs_keep >> t) & 1) ++cnt[(int)((K.obs_m[t] >> log2S) & (WB - 1))];
+        O.ubeg[0] = 0;
+        for (int u = 0; u < WB; ++u) O.ubeg[u + 1] = O.ubeg[u] + cnt[u];
+        for (int u = 0; u < WB; ++u) cnt[u] = O.ubeg[u];
+        for (int t = 0; t < K.nterms; ++t)
+            if ((C.obs_keep >> t) & 1) {
+                const int u = (int)((K.obs_m[t] >> log2S) & (WB - 1));
+                O.m[cnt[u]] = K.obs_m[t];
+                O.w[cnt[u]++] = K.obs_w[t];
+            }
+        O.n = O.ubeg[WB];
+    }
+    __syncthreads();
     for (int si = C.step_begin; si < C.step_end; ++si) {
         const BStep& S = K.steps[si];
         const int kind = S.kind;
@@ -370,19 +462,90 @@ __device__ __attribute__((always_inline)) bool run_circuit(const BCirc& C, Team&
             }
             continue;
         } else if (kind == BS_OBS) {
-            for (uint64_t i = first; i < N; i += stride) {
-                const double f = obs_factor(i, C.obs_keep, K.obs_m, K.obs_w, K.nterms);
-                const cx<R> a = psi[i];
-                lam[i] = {(R)(f * (double)a.re), (R)(f * (double)a.im)};
+            if (blocked) {
+                for (uint64_t kbi = 0; kbi < nblk; ++kbi) {
+                    const uint64_t i0 = first + WB * stride * kbi;
+                    cx<R> a[WB];
+#pragma unroll
+                    for (int j = 0; j < WB; ++j) a[j] = psi[i0 + stride * j];
+                    double f[WB];
+                    obs_groups(O, i0, f);
+                    wht(f);  // f[j] = f(i0 + stride j)
+#pragma unroll
+                    for (int j = 0; j < WB; ++j)
+                        lam[i0 + stride * j] = {(R)(f[j] * (double)a[j].re), (R)(f[j] * (double)a[j].im)};
+                }
+            } else {
+                for (uint64_t i = first; i < N; i += stride) {
+                    const double f = obs_factor(i, O);
+                    const cx<R> a = psi[i];
+                    lam[i] = {(R)(f * (double)a.re), (R)(f * (double)a.im)};
+                }
             }
         } else {  // BS_ENERGY, BS_GDIAG: partial sums of this workgroup, then the team's in rank order
             const int nv = (kind == BS_ENERGY) ? 1 : S.nmask;
             double acc[BATCH_DIAG];
 #pragma unroll
             for (int d = 0; d < BATCH_DIAG; ++d) acc[d] = 0.0;
-            if (kind == BS_ENERGY) {
-                for (uint64_t i = first; i < N; i += stride)
-                    acc[0] += prob(psi[i]) * obs_factor(i, C.obs_keep, K.obs_m, K.obs_w, K.nterms);
+            if (kind == BS_ENERGY && blocked) {
+                // sum_j p_j f_j = sum_u G[u] W[u], W = transform of the block's probabilities
+                for (uint64_t kbi = 0; kbi < nblk; ++kbi) {
+                    const uint64_t i0 = first + WB * stride * kbi;
+                    double w[WB], G[WB];
+#pragma unroll
+                    for (int j = 0; j < WB; ++j) w[j] = prob(psi[i0 + stride * j]);
+                    wht(w);
+                    obs_groups(O, i0, G);
+#pragma unroll
+                    for (int u = 0; u < WB; ++u) acc[0] += G[u] * w[u];
+                }
+            } else if (kind == BS_ENERGY) {
+                for (uint64_t i = first; i < N; i += stride) acc[0] += prob(psi[i]) * obs_factor(i, O);
+            } else if (blocked) {
+                // Z-string d over the block = (-1)^parity(i0 & m_d) W[(m_d >> log2 stride) & 7], W the
+                // block's transform, looked up through this thread's column of the (idle) tile buffer.
+                // parity(i0 & m_d) = parity(first & m_d) (bit d of cbits) ^ parity(WB stride kbi & m_d).
+                // Blocks go in Gray-code order g = kbi ^ (kbi >> 1), which flips bit ctz(kbi) of g per
+                // block, so the block part of the signs follows from one kflip word each.
+                double* hs = (double*)tile;
+                {
+                    const int at = (tid & 63) + log2S + WB_LOG;
+                    uint32_t b = 0;
+#pragma unroll
+                    for (int d = 0; d < BATCH_DIAG; ++d)
+                        b |= (uint32_t)(at < 64 ? (S.mask[d] >> (at & 63)) & 1 : 0) << d;
+                    if (tid < 64) O.kflip[tid] = b;
+                }
+                uint32_t flip = 0;  // bit d: parity(i0 & mask d)
+                for (int d = 0; d < S.nmask; ++d) flip |= (uint32_t)(__popcll(first & S.mask[d]) & 1) << d;
+                uint32_t upk[BATCH_DIAG / 8];  // field u of mask d in bits 4 (d & 7) of word d >> 3
+#pragma unroll
+                for (int q = 0; q < BATCH_DIAG / 8; ++q) {
+                    upk[q] = 0;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e)
+                        upk[q] |= (uint32_t)((S.mask[8 * q + e] >> log2S) & (WB - 1)) << (4 * e);
+                }
+                __syncthreads();
+                for (uint64_t kbi = 0; kbi < nblk; ++kbi) {
+                    if (kbi) flip ^= O.kflip[__builtin_ctzll(kbi)];
+                    const uint64_t i0 = first + WB * stride * (kbi ^ (kbi >> 1));
+                    double w[WB];
+#pragma unroll
+                    for (int j = 0; j < WB; ++j) {
+                        const cx<R> p = psi[i0 + stride * j], l = lam[i0 + stride * j];
+                        w[j] = (double)l.re * (double)p.im - (double)l.im * (double)p.re;
+                    }
+                    wht(w);
+#pragma unroll
+                    for (int u = 0; u < WB; ++u) hs[u * nth + tid] = w[u];
+#pragma unroll
+                    for (int d = 0; d < BATCH_DIAG; ++d) {
+                        const int u = (int)((upk[d >> 3] >> (4 * (d & 7))) & (WB - 1));
+                        const double v = hs[u * nth + tid];
+                        acc[d] += ((flip >> d) & 1) ? -v : v;
+                    }
+                }
             } else {
                 for (uint64_t i = first; i < N; i += stride) {
                     const cx<R> p = psi[i], l = lam[i];
@@ -431,6 +594,7 @@ __global__ __launch_bounds__(BATCH_THREADS) __attribute__((amdgpu_waves_per_eu(2
     __shared__ SlotVal slotv[FUSE_MAX_SLOTS];
     __shared__ uint64_t dep[2 * BATCH_DEP_HALF];
     __shared__ double red[BATCH_THREADS / 64 * BATCH_DIAG];
+    __shared__ ObsLds obs;
     __shared__ int s_next, s_flag;
     const int tid = threadIdx.x;
     // Team circuits first (queue 0), then this workgroup alone (queue 1); one
@@ -446,7 +610,7 @@ __global__ __launch_bounds__(BATCH_THREADS) __attribute__((amdgpu_waves_per_eu(2
             if (!team_sync(tm, &s_flag)) return;
             if (tid == 0) s_next = __hip_atomic_load(&team_cur[tm.id], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __syncthreads();
-            qi = s_next;
+            qi = __builtin_amdgcn_readfirstlane(s_next);  // the same LDS word in every lane
             if (qi >= nbig) {
                 in_team = false;
                 tm = Team{0, 0, 1, nullptr, 0, err};
@@ -456,10 +620,10 @@ __global__ __launch_bounds__(BATCH_THREADS) __attribute__((amdgpu_waves_per_eu(2
             __syncthreads();
             if (tid == 0) s_next = first + atomicAdd(&qctr[1], 1);
             __syncthreads();
-            qi = s_next;
+            qi = __builtin_amdgcn_readfirstlane(s_next);
             if (qi >= ncirc) break;
         }
-        if (!run_circuit<R>(K.circs[K.order[qi]], tm, K, tile, slotv, dep, red, &s_flag)) return;
+        if (!run_circuit<R>(K.circs[K.order[qi]], tm, K, tile, slotv, dep, red, obs, &s_flag)) return;
     }
 }
 
