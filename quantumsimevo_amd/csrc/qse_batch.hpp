@@ -14,15 +14,18 @@
 
 namespace qse {
 
-constexpr int BATCH_DIAG = 32;       // Z-string masks per reduction step
 constexpr int BATCH_MAX_TERMS = 64;  // observable terms (one bit each in BCirc::obs_keep)
+constexpr int BATCH_KMAX = 12;       // largest tile of a pass
+// Largest tile of a joint (psi, lambda) pass: both tiles share the LDS of one
+// complex64 tile pair of BATCH_KMAX qubits (64 KiB).
+constexpr int batch_joint_kmax(int prec) { return prec == QSE_C128 ? BATCH_KMAX - 1 : BATCH_KMAX; }
 
 enum BStepKind : int32_t {
     BS_BASIS = 0,   // psi = |0...0>
     BS_PROG = 1,    // run passes [pass_begin, pass_end) on psi (target bit 0) and/or lambda (bit 1)
     BS_OBS = 2,     // lambda_i = f(i) psi_i, f = the circuit's observable
     BS_ENERGY = 3,  // vals[out] = sum_i |psi_i|^2 f(i)
-    BS_GDIAG = 4    // vals[out + d] = sum_i (-1)^popc(i & mask[d]) Im(conj(lambda_i) psi_i), d < nmask
+    BS_JOINT = 4    // joint passes [pass_begin, pass_end) on (psi, lambda) with probes: vals[out + v], v < nprobes
 };
 
 // All fields 32/64-bit: the kernel reads steps with scalar loads.
@@ -30,9 +33,8 @@ struct BStep {
     int32_t kind;
     int32_t target;
     int32_t pass_begin, pass_end;
-    int32_t nmask;
+    int32_t nprobes;
     int32_t out;
-    uint64_t mask[BATCH_DIAG];
 };
 
 struct BCirc {
@@ -58,7 +60,9 @@ struct BatchPlan {
     int nvals = 0;
     uint64_t arena_amps = 0;           // amplitudes of all psi and lambda vectors
     int kmax = 0;                      // largest tile of any pass
+    int max_probes = 0;                // most probe values of one joint step
     double est_bytes = 0.0;            // algorithmic HBM bytes of the whole batch (per amplitude byte = 1)
+    double est_bytes_rev = 0.0;        // the part of est_bytes in joint (reverse-sweep) steps
 };
 
 // Plans a batch (arguments as qse_batch_run_small). kmax: largest tile the

@@ -1,15 +1,14 @@
 // qse_batch_plan.cpp — host builder of the batched small-circuit engine's
 // work (qse_batch.hpp): per circuit, the fused program of its gate list, its
-// observable and energy, and the adjoint reverse sweep of
-// qse_adjoint_plan.cpp as Z-string reduction steps and un-application
-// programs (with the X/Y-generator rotations folded in); every
-// program's passes are appended to one shared program with rebased indices.
+// observable and energy, and the adjoint reverse sweep as joint (psi, lambda)
+// passes whose probes evaluate each parameter's generator term in the tile
+// (build_reverse_plan, qse_plan.cpp); every program's passes are appended to
+// one shared program with rebased indices.
 #include <algorithm>
 #include <numeric>
 #include <string>
 #include <thread>
 
-#include "qse_adjoint_plan.hpp"
 #include "qse_batch.hpp"
 
 namespace qse {
@@ -66,12 +65,13 @@ struct CircWork {
     std::vector<BStep> steps;
     std::vector<BatchGradRef> grads;  // slot relative to the circuit
     int nvals = 0;
+    int max_probes = 0;
     int energy_slot = 0;
-    double cost = 0.0;
+    double cost = 0.0, cost_rev = 0.0;  // estimated bytes per amplitude byte: all steps, joint steps
     int rc = QSE_OK;
 };
 
-int plan_circuit(int c, int n, const qse_op* cops, int nops, const double* cpar, int npar,
+int plan_circuit(int prec, int c, int n, const qse_op* cops, int nops, const double* cpar, int npar,
                  const uint64_t* obs_masks, int nterms, bool with_grads, int kmax, int default_L, CircWork& W) {
     for (int i = 0; i < nops; ++i)
         if (cops[i].param >= npar) return set_error(QSE_ERR_INVALID_ARG, "batch: op parameter out of range");
@@ -86,7 +86,7 @@ int plan_circuit(int c, int n, const qse_op* cops, int nops, const double* cpar,
     BStep s{};
     s.kind = BS_BASIS;
     W.steps.push_back(s);
-    int npasses = 0;
+    int npasses = 0, njoint = 0;
     auto add_prog = [&](const std::vector<qse_op>& list, const double* pr, int target) -> int {
         if (list.empty()) return QSE_OK;
         // dependency scheduling, in place (the engine has no second buffer per state)
@@ -108,50 +108,38 @@ int plan_circuit(int c, int n, const qse_op* cops, int nops, const double* cpar,
     s.out = W.nvals++;
     W.energy_slot = s.out;
     W.steps.push_back(s);
-    int nreduce = 1;
+    double fwd_units = 2.0 * npasses + 2.0;  // passes read and write; basis write, energy read
     if (with_grads && npar > 0) {
-        std::vector<AdjSegment> segs;
-        std::vector<qse_op> prefix;
-        rc = adjoint_segments(cops, nops, cpar, segs, prefix);
-        if (rc) return rc;
         s = BStep{};
         s.kind = BS_OBS;
         W.steps.push_back(s);
-        // rotate psi and lambda into the first evaluation point's frame
-        if (!segs.empty()) {
-            rc = add_prog(segs[0].pre, nullptr, 3);
-            if (rc) return rc;
-        }
-        for (size_t g = 0; g < segs.size(); ++g) {
-            const AdjSegment& seg = segs[g];
-            // distinct Z-string masks of this point, 16 per reduction step
-            std::vector<uint64_t> masks;
-            std::vector<int> slot_of;
-            for (const auto& dt : seg.diag)
-                if (std::find(masks.begin(), masks.end(), dt.first) == masks.end()) masks.push_back(dt.first);
-            for (size_t m0 = 0; m0 < masks.size(); m0 += BATCH_DIAG) {
-                BStep d{};
-                d.kind = BS_GDIAG;
-                d.nmask = (int32_t)std::min<size_t>(BATCH_DIAG, masks.size() - m0);
-                d.out = W.nvals;
-                for (int i = 0; i < d.nmask; ++i) d.mask[i] = masks[m0 + i];
-                W.nvals += d.nmask;
-                W.steps.push_back(d);
-                ++nreduce;
-                for (int i = 0; i < d.nmask; ++i) slot_of.push_back(d.out + i);
+        fwd_units += 2.0;  // psi read, lambda written
+        // reverse sweep: joint passes over (psi, lambda) with one probe per generator term
+        const int kj = std::min(k, batch_joint_kmax(prec));
+        const int Lj = std::max(0, std::min(default_L, kj - FUSE_REG));
+        std::vector<HostPlan> chunks;
+        std::vector<ProbeRef> refs;
+        rc = build_reverse_plan(n, kj, Lj, cops, nops, cpar, chunks, refs);
+        if (rc) return rc;
+        for (const HostPlan& C : chunks) {
+            BStep js{};
+            js.kind = BS_JOINT;
+            append_plan(W.prog, C, &js.pass_begin, &js.pass_end);
+            js.nprobes = (int32_t)C.probe_ids.size();
+            js.out = W.nvals;
+            W.nvals += js.nprobes;
+            for (int v = 0; v < js.nprobes; ++v) {
+                const ProbeRef& r = refs[C.probe_ids[v]];
+                W.grads.push_back({js.out + v, c, r.param, r.coeff});
             }
-            for (const auto& dt : seg.diag) {
-                const size_t mi = std::find(masks.begin(), masks.end(), dt.first) - masks.begin();
-                W.grads.push_back({slot_of[mi], c, dt.second.param, dt.second.coeff});
-            }
-            if (g + 1 < segs.size()) {
-                rc = add_prog(seg.unapply, nullptr, 3);
-                if (rc) return rc;
-            }
+            njoint += js.pass_end - js.pass_begin;
+            W.max_probes = std::max(W.max_probes, js.nprobes);
+            W.steps.push_back(js);
         }
     }
-    // estimated HBM bytes per amplitude byte: passes read+write, reductions read psi and lambda
-    W.cost = (double)(1ULL << n) * (2.0 * npasses + 2.0 * nreduce + 2.0);
+    // estimated HBM bytes per amplitude byte; a joint pass reads and writes psi and lambda
+    W.cost_rev = (double)(1ULL << n) * 4.0 * njoint;
+    W.cost = (double)(1ULL << n) * fwd_units + W.cost_rev;
     return QSE_OK;
 }
 
@@ -160,7 +148,6 @@ int plan_circuit(int c, int n, const qse_op* cops, int nops, const double* cpar,
 int build_batch_plan(int prec, int ncirc, const int* nq, const int64_t* op_off, const qse_op* ops,
                      const int64_t* par_off, const double* params, const uint64_t* obs_masks, int nterms,
                      bool with_grads, int kmax, int default_L, BatchPlan& B) {
-    (void)prec;
     B = BatchPlan();
     if (nterms > BATCH_MAX_TERMS) return set_error(QSE_ERR_LIMIT, "batch: at most 64 observable terms");
     for (int c = 0; c < ncirc; ++c) {
@@ -174,7 +161,7 @@ int build_batch_plan(int prec, int ncirc, const int* nq, const int64_t* op_off, 
     auto plan_into = [&](int c, CircWork& W) {
         const int64_t p0 = par_off ? par_off[c] : 0, p1 = par_off ? par_off[c + 1] : 0;
         const int npar = (int)(p1 - p0);
-        return plan_circuit(c, nq[c], ops + op_off[c], (int)(op_off[c + 1] - op_off[c]),
+        return plan_circuit(prec, c, nq[c], ops + op_off[c], (int)(op_off[c + 1] - op_off[c]),
                             (params && npar > 0) ? params + p0 : nullptr, npar, obs_masks, nterms, with_grads, kmax,
                             default_L, W);
     };
@@ -209,11 +196,11 @@ int build_batch_plan(int prec, int ncirc, const int* nq, const int64_t* op_off, 
         append_plan(B.prog, W.prog, &pb, &pe);
         bc.step_begin = (int32_t)B.steps.size();
         for (BStep st : W.steps) {
-            if (st.kind == BS_PROG) {
+            if (st.kind == BS_PROG || st.kind == BS_JOINT) {
                 st.pass_begin += pb;
                 st.pass_end += pb;
             }
-            if (st.kind == BS_ENERGY || st.kind == BS_GDIAG) st.out += B.nvals;
+            if (st.kind == BS_ENERGY || st.kind == BS_JOINT) st.out += B.nvals;
             B.steps.push_back(st);
         }
         bc.step_end = (int32_t)B.steps.size();
@@ -225,7 +212,9 @@ int build_batch_plan(int prec, int ncirc, const int* nq, const int64_t* op_off, 
         B.nvals += W.nvals;
         B.circs.push_back(bc);
         B.kmax = std::max(B.kmax, bc.k);
+        B.max_probes = std::max(B.max_probes, W.max_probes);
         B.est_bytes += W.cost;
+        B.est_bytes_rev += W.cost_rev;
         W.prog = HostPlan();  // merged: release as we go
     }
     B.order.resize(ncirc);

@@ -19,12 +19,15 @@
 #include <string>
 #include <vector>
 
+#include "../../include/qse_hip.h"
+
 namespace qse {
 
 constexpr int FUSE_REG = 4;        // register bits per thread (16 amplitudes)
 constexpr int FUSE_MAX_K = 13;     // largest tile (qubits)
 constexpr int FUSE_MIN_K = 8;      // smallest tile (16 threads)
 constexpr int FUSE_MAX_SLOTS = 256;  // per-tile external factor slots per pass
+constexpr int FUSE_MAX_PROBES = 128; // probe accumulators per joint pass
 
 enum FOpType : uint32_t {
     FOP_DENSE = 0,  // 2x2 matrix on register position p
@@ -33,8 +36,21 @@ enum FOpType : uint32_t {
     FOP_MONO = 3,   // multiply by the product of the op's terms
     // Uncontrolled specialisations chosen by the planner (same math as FOP_DENSE):
     FOP_REAL = 4,   // all four entries real
-    FOP_RXLIKE = 5  // real diagonal, imaginary off-diagonal
+    FOP_RXLIKE = 5, // real diagonal, imaginary off-diagonal
+    // Joint passes over (psi, lambda) of an adjoint reverse sweep only: adds
+    // Im<lambda|G|psi> over the tile to the pass's accumulator term_begin.
+    FOP_PROBE = 6
 };
+
+// FOp::mode of a FOP_PROBE: the generator G.
+enum ProbeKind : uint32_t {
+    PROBE_ZSIGN = 0,  // Z-string: (-1)^(number of set bits among its qubits)
+    PROBE_PROJ = 1,   // projector: 1 where all its qubits are set, else 0
+    PROBE_X = 2,      // X on register position p
+    PROBE_Y = 3       // Y on register position p
+};
+// A probe's qubits: register positions in rctl, other tile local bits in
+// tctl, external qubits (global mask) in ectl.
 
 // All fields are 32/64-bit so the kernel reads them with scalar loads (no
 // byte-sized scalar loads on gfx950).
@@ -96,6 +112,8 @@ struct FPass {
     uint32_t stage_begin;
     uint32_t slot_begin;
     int32_t out_of_place; // 1: write to the other buffer (qubit permutation)
+    int32_t nprobes;      // FOP_PROBE accumulators of the pass (joint passes)
+    int32_t probe_base;   // value index of the pass's first accumulator within its program
     int32_t pad;
 };
 
@@ -110,6 +128,12 @@ struct FProgram {
     const double* alphas;
 };
 
+// A probe's gradient term: dE/dparams[param] += coeff * (probed value).
+struct ProbeRef {
+    int param;
+    double coeff;
+};
+
 // Host-side plan (built by qse_plan.cpp, serialized and launched by qse_fused.hip).
 struct HostPlan {
     int k = 0;
@@ -120,6 +144,7 @@ struct HostPlan {
     std::vector<FSlot> slots;
     std::vector<FExtTerm> eterms;
     std::vector<double> alphas;
+    std::vector<int32_t> probe_ids;  // host only: caller's id of each probe value, in value order
 };
 
 // A host plan flattened into one buffer: the seven program arrays, each
@@ -130,6 +155,15 @@ struct Serialized {
     size_t off[7];
 };
 void serialize(const HostPlan& P, Serialized& S);
+
+// Adjoint reverse sweep of ops (applied to |0>) as joint (psi, lambda) plans:
+// from the last op back to the first parameterised one, each parameterised
+// op's generator probe (value: Im<lambda|G|psi> after the op) and then the
+// op's inverse; lambda must start as O psi. refs[id] is the gradient term of
+// probe id (HostPlan::probe_ids); the sweep is split into several plans when
+// a pass would hold more than FUSE_MAX_PROBES probes.
+int build_reverse_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params,
+                       std::vector<HostPlan>& chunks, std::vector<ProbeRef>& refs);
 
 // HIP source of every pass of plan P as run-time specialised kernels
 // (qse_jit_gen.cpp); complex128 passes p with (*paired)[p] process tiles in

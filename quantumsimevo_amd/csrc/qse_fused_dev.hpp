@@ -129,10 +129,11 @@ struct SlotVal {
 // bits set in both tid and op.tctl of their factor x product of the terms whose
 // thread/external conditions hold (external ones from the per-tile slots).
 // TermPtr / AlphaPtr: generic pointers, or pointers into the constant address
-// space (uniform reads then become scalar loads).
-template <int TB, class TermPtr, class AlphaPtr>
+// space (uniform reads then become scalar loads); Slot: any type with the
+// slot's factor in fields c, s.
+template <int TB, class TermPtr, class AlphaPtr, class Slot>
 __device__ __forceinline__ void mono_factor(const FOp& op, TermPtr __restrict__ terms, AlphaPtr __restrict__ alphas,
-                                            int tid, uint32_t toff, uint64_t base, const SlotVal* slotv, double& gr,
+                                            int tid, uint32_t toff, uint64_t base, const Slot* slotv, double& gr,
                                             double& gi) {
     gr = op.m[0];
     gi = op.m[1];

@@ -16,6 +16,13 @@
 // writes each tile bit to its final qubit). Inside a pass the gates are
 // ordered the same way over register sets of 4 target qubits, so that the
 // stage builder below packs them into few stages.
+// Probes (build_reverse_plan): the reverse sweep of the adjoint method runs
+// as joint passes over (psi, lambda); a parameterised gate's generator G is
+// probed, Im<lambda|G|psi> at the point after the gate, before the gate is
+// un-applied. X/Y probes are scheduled like a non-diagonal gate on their
+// qubit (a register bit when evaluated), Z-string and projector probes like
+// a diagonal gate (they commute with every diagonal gate applied to both
+// states).
 // Phase 2 (stages): inside a pass, consecutive non-diagonal gates share a
 // stage while their targets fit in 4 register bits. Diagonal gates are kept
 // pending as monomials (d applied where all bits of a mask are 1) and flushed
@@ -78,17 +85,21 @@ using cd = std::complex<double>;
 
 namespace {
 
-enum ItemType { IT_DENSE, IT_MONO, IT_SWAP };
+enum ItemType { IT_DENSE, IT_MONO, IT_SWAP, IT_PROBE };
 
 struct Item {
     ItemType t;
-    int q = -1, q2 = -1;   // dense target / swap qubits
+    int q = -1, q2 = -1;   // dense target / swap qubits; X/Y probe qubit
     uint32_t dtype = FOP_DENSE;
     uint64_t cmask = 0;    // dense: control qubits
     cd m[4];
-    uint64_t qmask = 0;    // mono: qubits that must be 1
+    uint64_t qmask = 0;    // mono: qubits that must be 1; Z-string/projector probe: its qubits
     cd d;
+    uint32_t pkind = 0;    // probe: ProbeKind
+    int pid = -1;          // probe: caller's id
 };
+
+bool probe_nd(const Item& it) { return it.t == IT_PROBE && (it.pkind == PROBE_X || it.pkind == PROBE_Y); }
 
 struct Mono {
     uint64_t lmask;  // tile local bits
@@ -105,6 +116,9 @@ struct SOp {
     cd m[4];
     // mono
     std::vector<Mono> monos;
+    // probe (dtype == FOP_PROBE): kind, id; X/Y: lt; Z-string/projector: tile local bits in lctl, external in ectl
+    uint32_t pkind = 0;
+    int pid = -1;
 };
 
 struct SStage {
@@ -223,13 +237,13 @@ namespace {
 // Qubits on which an item is not diagonal (must be tile/register qubits) and
 // qubits on which it is diagonal (controls and phase masks).
 uint64_t nd_mask(const Item& it) {
-    if (it.t == IT_DENSE) return 1ULL << it.q;
+    if (it.t == IT_DENSE || probe_nd(it)) return 1ULL << it.q;
     if (it.t == IT_SWAP) return (1ULL << it.q) | (1ULL << it.q2);
     return 0;
 }
 uint64_t dg_mask(const Item& it) {
     if (it.t == IT_DENSE) return it.cmask;
-    if (it.t == IT_MONO) return it.qmask;
+    if (it.t == IT_MONO || (it.t == IT_PROBE && !probe_nd(it))) return it.qmask;
     return 0;
 }
 
@@ -322,8 +336,8 @@ void list_schedule(const std::vector<Item>& v, int n, uint64_t start, int cap, N
 // are tile-bit relabellings in phase 2 and need no register.
 void order_for_stages(std::vector<Item>& items, int n) {
     std::vector<std::pair<uint64_t, std::vector<int>>> groups;
-    list_schedule(items, n, 0, FUSE_REG, [](const Item& it) { return it.t == IT_DENSE ? nd_mask(it) : 0ULL; },
-                  groups);
+    list_schedule(items, n, 0, FUSE_REG,
+                  [](const Item& it) { return (it.t == IT_DENSE || probe_nd(it)) ? nd_mask(it) : 0ULL; }, groups);
     std::vector<Item> out;
     out.reserve(items.size());
     for (auto& g : groups)
@@ -354,10 +368,10 @@ void fusion_shape(const qse_ctx* ctx, int n, int prec, int* k_out, int* L_out) {
 // the last pass then stores out of place with the inverse map so the state
 // ends in the canonical layout. Otherwise SWAPs are in-tile relabellings (in
 // place). PLAN_DAG: passes by list scheduling over the dependency graph.
-int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P, int flags) {
+namespace {
+
+int plan_items(int n, int k, int L, const std::vector<Item>& items, HostPlan& P, int flags) {
     const bool relabel = (flags & PLAN_RELABEL) != 0;
-    std::vector<Item> items;
-    to_items(ops, nops, params, items);
     const uint64_t lowT = (L > 0) ? ((1ULL << L) - 1) : 0;
     int phys[64];
     for (int q = 0; q < 64; ++q) phys[q] = q;
@@ -376,7 +390,7 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
                 std::swap(phys[it.q], phys[it.q2]);
                 continue;
             }
-            if (it.t == IT_DENSE) {
+            if (it.t == IT_DENSE || probe_nd(it)) {
                 it.q = phys[it.q];
                 it.cmask = map_mask(it.cmask);
             } else {
@@ -460,7 +474,20 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
             st.ops.push_back(std::move(op));
         };
         for (const Item& it : pi.items) {
-            if (it.t == IT_MONO) {
+            if (it.t == IT_PROBE && !probe_nd(it)) {
+                // diagonal probe: commutes with the pending monomials, evaluated in the current stage
+                SOp op;
+                op.mono = false;
+                op.dtype = FOP_PROBE;
+                op.pkind = it.pkind;
+                op.pid = it.pid;
+                op.lt = -1;
+                op.lctl = 0;
+                for (int q = 0; q < n; ++q)
+                    if (((it.qmask >> q) & 1) && loc[q] >= 0) op.lctl |= 1ULL << loc[q];
+                op.ectl = it.qmask & ~pi.T;
+                stages.back().ops.push_back(op);
+            } else if (it.t == IT_MONO) {
                 uint64_t lm = 0;
                 for (int q = 0; q < n; ++q)
                     if (((it.qmask >> q) & 1) && loc[q] >= 0) lm |= 1ULL << loc[q];
@@ -480,7 +507,9 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
                 flush(*st, lt);
                 SOp op;
                 op.mono = false;
-                op.dtype = it.dtype;
+                op.dtype = it.t == IT_PROBE ? (uint32_t)FOP_PROBE : it.dtype;
+                op.pkind = it.pkind;
+                op.pid = it.pid;
                 op.lt = lt;
                 op.lctl = 0;
                 op.ectl = 0;
@@ -516,7 +545,8 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
         // ---- emit stages ----
         fp.stage_begin = (uint32_t)P.stages.size();
         fp.slot_begin = (uint32_t)P.slots.size();
-        int nslots = 0;
+        fp.probe_base = (int32_t)P.probe_ids.size();
+        int nslots = 0, nprobes = 0;
         for (SStage& st : stages) {
             // complete the register set with unused local bits
             for (int l = 0; l < k && (int)st.R.size() < FUSE_REG; ++l)
@@ -545,6 +575,25 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
             for (size_t t = 0; t < tm.size(); ++t) fs.tmap[t] = (uint32_t)tm[t];
             fs.op_begin = (uint32_t)P.ops.size();
             for (SOp& so : st.ops) {
+                if (!so.mono && so.dtype == FOP_PROBE) {
+                    FOp f{};
+                    f.type = FOP_PROBE;
+                    f.mode = so.pkind;
+                    if (so.lt >= 0) {
+                        f.p = (uint32_t)pos[so.lt];
+                    } else {
+                        uint32_t rm = 0;
+                        for (int l = 0; l < k; ++l)
+                            if (((so.lctl >> l) & 1) && pos[l] >= 0) rm |= 1u << pos[l];
+                        f.rctl = rm;
+                        f.tctl = (uint32_t)(so.lctl & ~Rmask);
+                        f.ectl = so.ectl;
+                    }
+                    f.term_begin = (uint32_t)nprobes++;
+                    P.probe_ids.push_back(so.pid);
+                    P.ops.push_back(f);
+                    continue;
+                }
                 if (!so.mono) {
                     FOp f{};
                     f.type = so.dtype;
@@ -677,7 +726,107 @@ int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* p
         }
         fp.nstages = (int)stages.size();
         fp.nslots = nslots;
+        fp.nprobes = nprobes;
         P.passes.push_back(fp);
+    }
+    return QSE_OK;
+}
+
+}  // namespace
+
+int build_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params, HostPlan& P, int flags) {
+    std::vector<Item> items;
+    to_items(ops, nops, params, items);
+    return plan_items(n, k, L, items, P, flags);
+}
+
+qse_op dagger(const qse_op& o, const double* params);
+
+int build_reverse_plan(int n, int k, int L, const qse_op* ops, int nops, const double* params,
+                       std::vector<HostPlan>& chunks, std::vector<ProbeRef>& refs) {
+    chunks.clear();
+    refs.clear();
+    int kmin = -1;
+    for (int i = 0; i < nops; ++i)
+        if (ops[i].param >= 0) {
+            kmin = i;
+            break;
+        }
+    if (kmin < 0) return QSE_OK;
+    std::vector<Item> items;
+    auto probe = [&](uint32_t kind, int q, uint64_t qmask, int param, double coeff) {
+        Item it;
+        it.t = IT_PROBE;
+        it.pkind = kind;
+        it.q = q;
+        it.qmask = qmask;
+        it.pid = (int)refs.size();
+        refs.push_back({param, coeff});
+        items.push_back(it);
+    };
+    auto unapply = [&](const qse_op* seq, int cnt) {  // seq: ops in application order
+        std::vector<qse_op> d;
+        for (int i = 0; i < cnt; ++i) d.push_back(dagger(seq[i], params));
+        to_items(d.data(), cnt, nullptr, items);
+    };
+    for (int i = nops - 1; i >= kmin; --i) {
+        const qse_op& o = ops[i];
+        if (i >= 2 && is_zz_triple(ops, nops, i - 2) && ops[i].param < 0 && ops[i - 2].param < 0) {
+            // CNOT RZ CNOT = exp(-i theta Z_c Z_t / 2): probed after the triple
+            const qse_op& rz = ops[i - 1];
+            if (rz.param >= 0) probe(PROBE_ZSIGN, -1, (1ULL << o.control) | (1ULL << o.target), rz.param, rz.scale);
+            if (i - 2 > kmin) {
+                const qse_op rev[3] = {ops[i], ops[i - 1], ops[i - 2]};
+                unapply(rev, 3);
+            }
+            i -= 2;
+            continue;
+        }
+        if (o.param >= 0) {
+            const uint64_t tb = 1ULL << o.target;
+            switch (o.kind) {
+                case QSE_GATE_RX: probe(PROBE_X, o.target, 0, o.param, o.scale); break;
+                case QSE_GATE_RY: probe(PROBE_Y, o.target, 0, o.param, o.scale); break;
+                case QSE_GATE_RZ: probe(PROBE_ZSIGN, -1, tb, o.param, o.scale); break;
+                // PHASE and CPHASE = exp(i theta P): dE/dtheta = -2 Im<lambda|P|psi>, P the projector
+                case QSE_GATE_PHASE: probe(PROBE_PROJ, -1, tb, o.param, -2.0 * o.scale); break;
+                case QSE_GATE_CPHASE:
+                    probe(PROBE_PROJ, -1, tb | (1ULL << o.control), o.param, -2.0 * o.scale);
+                    break;
+                default:
+                    return set_error(QSE_ERR_UNSUPPORTED,
+                                     "adjoint: parameters are supported on RX, RY, RZ, PHASE and CPHASE only");
+            }
+        }
+        if (i > kmin) unapply(&o, 1);
+    }
+    // at most FUSE_MAX_PROBES accumulators per pass: split the sweep where needed
+    std::vector<std::vector<Item>> parts(1);
+    {
+        HostPlan W;
+        int rc = plan_items(n, k, L, items, W, PLAN_DAG);
+        if (rc) return rc;
+        bool fits = true;
+        for (const FPass& fp : W.passes) fits &= fp.nprobes <= FUSE_MAX_PROBES;
+        if (fits) {
+            chunks.push_back(std::move(W));
+            return QSE_OK;
+        }
+        int cnt = 0;
+        for (const Item& it : items) {
+            if (it.t == IT_PROBE && cnt == FUSE_MAX_PROBES) {
+                parts.emplace_back();
+                cnt = 0;
+            }
+            cnt += it.t == IT_PROBE ? 1 : 0;
+            parts.back().push_back(it);
+        }
+    }
+    for (const auto& part : parts) {
+        HostPlan C;
+        const int rc = plan_items(n, k, L, part, C, PLAN_DAG);
+        if (rc) return rc;
+        chunks.push_back(std::move(C));
     }
     return QSE_OK;
 }

@@ -146,6 +146,109 @@ static void emu_pass(const HostPlan& P, const FPass& fp, int n, const cd* s, cd*
     }
 }
 
+// This thread's part of a probe (k_batch's probe_regs): Im<lambda|G|psi> over
+// the 16 register amplitudes, psi in a and lambda in b.
+static double probe_value(const FOp& op, const cd* a, const cd* b, uint32_t toff, uint64_t base) {
+    auto im = [](cd l, cd p) { return (std::conj(l) * p).imag(); };
+    double s = 0.0;
+    if (op.mode == PROBE_X || op.mode == PROBE_Y) {
+        const int p = op.p;
+        for (int jj = 0; jj < 8; ++jj) {
+            const int j0 = ((jj >> p) << (p + 1)) | (jj & ((1 << p) - 1)), j1 = j0 | (1 << p);
+            if (op.mode == PROBE_X) s += im(b[j0], a[j1]) + im(b[j1], a[j0]);
+            else s += (std::conj(b[j1]) * a[j0]).real() - (std::conj(b[j0]) * a[j1]).real();
+        }
+        return s;
+    }
+    if (op.mode == PROBE_PROJ) {
+        if ((toff & op.tctl) != op.tctl || (base & op.ectl) != op.ectl) return 0.0;
+        for (int j = 0; j < 16; ++j)
+            if (((uint32_t)j & op.rctl) == op.rctl) s += im(b[j], a[j]);
+        return s;
+    }
+    for (int j = 0; j < 16; ++j) s += (__builtin_popcount((uint32_t)j & op.rctl) & 1) ? -im(b[j], a[j]) : im(b[j], a[j]);
+    return ((__builtin_popcount(toff & op.tctl) + __builtin_popcountll(base & op.ectl)) & 1) ? -s : s;
+}
+
+// One joint pass (k_batch's batch_pass with J): the pass's gates on psi and
+// lambda, in place, and its probes summed into vals[fp.probe_base + probe].
+static void emu_joint_pass(const HostPlan& P, const FPass& fp, int n, cd* psi, cd* lam, double* vals) {
+    const int k = fp.k;
+    const int NT = 1 << (k - 4), TB = k - 4;
+    std::vector<cd> tile(1u << k), tile2(1u << k), slotv(FUSE_MAX_SLOTS);
+    std::vector<double> acc(std::max(fp.nprobes, 1), 0.0);
+    const uint64_t ntiles = 1ULL << (n - k);
+    for (uint64_t ti = 0; ti < ntiles; ++ti) {
+        uint64_t base = 0, obase = 0, t = ti;
+        for (int e = 0; e < fp.next; ++e) {
+            base |= (t & 1ULL) << fp.epos[e];
+            obase |= (t & 1ULL) << fp.eout[e];
+            t >>= 1;
+        }
+        for (int sl = 0; sl < fp.nslots; ++sl) {
+            const FSlot& S = P.slots[fp.slot_begin + sl];
+            double a = 0.0;
+            for (uint32_t e = S.begin; e < S.end; ++e)
+                if ((base & P.eterms[e].emask) == P.eterms[e].emask) a += P.eterms[e].ang;
+            slotv[sl] = cd(a, 0.0);
+        }
+        for (int tid = 0; tid < NT; ++tid)
+            for (int i = 0; i < 16; ++i) {
+                const uint32_t li = (uint32_t)(i * NT + tid);
+                uint64_t g = base;
+                for (int b = 0; b < k; ++b)
+                    if ((li >> b) & 1) g |= 1ULL << fp.tq[b];
+                tile[swz(li)] = psi[g];
+                tile2[swz(li)] = lam[g];
+            }
+        for (int sg = 0; sg < fp.nstages; ++sg) {
+            const FStage& S = P.stages[fp.stage_begin + sg];
+            for (int tid = 0; tid < NT; ++tid) {
+                uint32_t toff = 0;
+                for (int tb = 0; tb < TB; ++tb)
+                    if ((tid >> tb) & 1) toff |= 1u << S.tmap[tb];
+                uint32_t li[16];
+                cd a[16], b[16];
+                for (int j = 0; j < 16; ++j) {
+                    uint32_t l = toff;
+                    for (int p = 0; p < 4; ++p)
+                        if ((j >> p) & 1) l |= 1u << S.rmap[p];
+                    li[j] = swz(l);
+                    a[j] = tile[li[j]];
+                    b[j] = tile2[li[j]];
+                }
+                for (uint32_t oi = S.op_begin; oi < S.op_end; ++oi) {
+                    const FOp& op = P.ops[oi];
+                    if (op.type == FOP_PROBE) {
+                        acc[op.term_begin] += probe_value(op, a, b, toff, base);
+                    } else {
+                        apply_op(P, op, a, tid, TB, toff, base, slotv.data());
+                        apply_op(P, op, b, tid, TB, toff, base, slotv.data());
+                    }
+                }
+                for (int j = 0; j < 16; ++j) {
+                    tile[li[j]] = a[j];
+                    tile2[li[j]] = b[j];
+                }
+            }
+        }
+        for (int tid = 0; tid < NT; ++tid)
+            for (int i = 0; i < 16; ++i) {
+                const uint32_t go = (uint32_t)(i * NT + tid);
+                uint64_t g = obase;
+                uint32_t ls = 0;
+                for (int b = 0; b < k; ++b)
+                    if ((go >> b) & 1) {
+                        g |= 1ULL << fp.so_q[b];
+                        ls |= 1u << fp.so_l[b];
+                    }
+                psi[g] = tile[swz(ls)];
+                lam[g] = tile2[swz(ls)];
+            }
+    }
+    for (int p = 0; p < fp.nprobes; ++p) vals[fp.probe_base + p] = acc[p];
+}
+
 extern "C" int emu_run_fused(int n, int k, int L, const qse_op* ops, int nops, const double* params, double* psi,
                              int* npasses, int relabel) {
     HostPlan P;
@@ -396,14 +499,11 @@ extern "C" int emu_batch(int ncirc, const int* nq, const int64_t* op_off, const 
                     vals[S.out] = acc;
                     break;
                 }
-                case BS_GDIAG:
-                    for (int d = 0; d < S.nmask; ++d) {
-                        double acc = 0.0;
-                        for (uint64_t i = 0; i < N; ++i) {
-                            const double w = (std::conj(lam[i]) * psi[i]).imag();
-                            acc += (__builtin_popcountll(i & S.mask[d]) & 1) ? -w : w;
-                        }
-                        vals[S.out + d] = acc;
+                case BS_JOINT:
+                    for (int p = S.pass_begin; p < S.pass_end; ++p) {
+                        const FPass& fp = B.prog.passes[p];
+                        if (fp.k > C.k || fp.k > BATCH_KMAX) return 103;
+                        emu_joint_pass(B.prog, fp, C.n, psi.data(), lam.data(), vals.data() + S.out);
                     }
                     break;
                 default: return 102;
@@ -414,8 +514,9 @@ extern "C" int emu_batch(int ncirc, const int* nq, const int64_t* op_off, const 
     return 0;
 }
 
-// Batch plan summary: out[0] passes, out[1] steps, out[2] reduction steps,
-// out[3] program steps, out[4] value slots; returns the plan's status.
+// Batch plan summary: out[0] single-state passes (a joint pass counts twice),
+// out[1] steps, out[2] energy steps, out[3] program and joint steps, out[4]
+// value slots; returns the plan's status.
 extern "C" int emu_batch_stats(int ncirc, const int* nq, const int64_t* op_off, const qse_op* ops,
                                const int64_t* par_off, const double* params, const uint64_t* obs_m, int nterms,
                                int kmax, int L, int64_t* out) {
@@ -424,10 +525,10 @@ extern "C" int emu_batch_stats(int ncirc, const int* nq, const int64_t* op_off, 
     if (rc) return rc;
     int64_t red = 0, progs = 0, passes = 0;
     for (const BStep& s : B.steps) {
-        if (s.kind == BS_GDIAG || s.kind == BS_ENERGY) ++red;
-        if (s.kind == BS_PROG) {
+        if (s.kind == BS_ENERGY) ++red;
+        if (s.kind == BS_PROG || s.kind == BS_JOINT) {
             ++progs;
-            passes += (int64_t)(s.pass_end - s.pass_begin) * ((s.target == 3) ? 2 : 1);
+            passes += (int64_t)(s.pass_end - s.pass_begin) * ((s.target == 3 || s.kind == BS_JOINT) ? 2 : 1);
         }
     }
     out[0] = passes;
