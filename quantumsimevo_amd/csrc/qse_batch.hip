@@ -19,7 +19,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cstring>
 #include <vector>
 
@@ -906,9 +905,10 @@ extern "C" int qse_batch_run_small(qse_ctx* ctx, int precision, int ncirc, const
 
 // Measured choice of the batch engine's tile shape: the given circuits (a
 // sample of a larger batch, typically) run once per candidate (tiles of 12,
-// 11 or 10 qubits; 2 to 5 contiguous low qubits), and the fastest candidate
-// is kept in the context for later qse_batch_run_small calls of this
-// precision. Candidates that fail are skipped.
+// 11 or 10 qubits; 2 to 5 contiguous low qubits; then teams of 4, 8, 16 or 32
+// workgroups), timed by the engine's kernel events, and the fastest
+// candidate is kept in the context for later qse_batch_run_small calls of
+// this precision. Candidates that fail are skipped.
 extern "C" int qse_batch_autotune(qse_ctx* ctx, int precision, int ncirc, const int* nq, const int64_t* op_off,
                                   const qse_op* ops, const int64_t* par_off, const double* params,
                                   const uint64_t* obs_masks, const double* obs_weights, int nterms, int with_grads,
@@ -920,20 +920,40 @@ extern "C" int qse_batch_autotune(qse_ctx* ctx, int precision, int ncirc, const 
     std::vector<double> gr(with_grads && par_off ? (size_t)std::max<int64_t>(par_off[ncirc], 1) : 1);
     double* gp = with_grads && par_off ? gr.data() : nullptr;
     const int keep_k = ctx->batch_k[pi], keep_L = ctx->batch_L[pi], keep_T = ctx->batch_team[pi];
-    auto restore = [&] {
-        ctx->batch_k[pi] = keep_k;
-        ctx->batch_L[pi] = keep_L;
-        ctx->batch_team[pi] = keep_T;
-    };
+    // Candidates are compared by the engine's kernel time (HIP events of the
+    // batch family; host planning and uploads are the same for every shape).
+    // The caller's timing switch and batch-family totals are restored after.
+    int rc0 = fold_timing(ctx);
+    if (rc0) return rc0;
+    const bool keep_timing = ctx->timing;
+    const int64_t keep_launches = ctx->launches[FAM_BATCH];
+    const double keep_ms = ctx->total_ms[FAM_BATCH], keep_bytes = ctx->total_bytes[FAM_BATCH];
+    ctx->timing = true;
     auto run = [&](int k, int L, int team, double* ms) -> int {
         ctx->batch_k[pi] = k;
         ctx->batch_L[pi] = L;
         ctx->batch_team[pi] = team;
-        const auto t0 = std::chrono::steady_clock::now();
-        const int rc = qse_batch_run_small(ctx, precision, ncirc, nq, op_off, ops, par_off, params, obs_masks,
-                                           obs_weights, nterms, en.data(), gp);
-        *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        int rc = fold_timing(ctx);
+        const double before = ctx->total_ms[FAM_BATCH];
+        if (!rc)
+            rc = qse_batch_run_small(ctx, precision, ncirc, nq, op_off, ops, par_off, params, obs_masks, obs_weights,
+                                     nterms, en.data(), gp);
+        if (!rc) rc = fold_timing(ctx);
+        *ms = ctx->total_ms[FAM_BATCH] - before;
         return rc;
+    };
+    auto restore_timing = [&] {
+        fold_timing(ctx);
+        ctx->timing = keep_timing;
+        ctx->launches[FAM_BATCH] = keep_launches;
+        ctx->total_ms[FAM_BATCH] = keep_ms;
+        ctx->total_bytes[FAM_BATCH] = keep_bytes;
+    };
+    auto restore = [&] {
+        ctx->batch_k[pi] = keep_k;
+        ctx->batch_L[pi] = keep_L;
+        ctx->batch_team[pi] = keep_T;
+        restore_timing();
     };
     double ms = 0.0;
     int rc = run(BATCH_KMAX, precision == QSE_C128 ? 3 : 4, 16, &ms);  // warm-up: device buffer and code objects
@@ -964,6 +984,7 @@ extern "C" int qse_batch_autotune(qse_ctx* ctx, int precision, int ncirc, const 
             best_T = team;
         }
     }
+    restore_timing();
     ctx->batch_k[pi] = best_k;
     ctx->batch_L[pi] = best_L;
     ctx->batch_team[pi] = best_T;
