@@ -368,7 +368,7 @@ int qse_batch_expect(qse_ctx* ctx, int nqubits, int precision,
                      const uint64_t* obs_masks, const double* obs_weights, int nterms,
                      double* energies);
 
-/* ---- batched circuits of uneven sizes in one persistent launch ----
+/* ---- batched circuits of uneven sizes in persistent launches ----
  * (no counterpart in the reference). ncirc circuits; circuit c has nq[c]
  * qubits (8..30; its states live in HBM) and the ops ops[op_off[c] ..
  * op_off[c+1]), whose param indices refer to params[par_off[c] ..
@@ -377,9 +377,13 @@ int qse_batch_expect(qse_ctx* ctx, int nqubits, int precision,
  * whose qubits all lie below nq[c] (at most 64 terms, the same list for every
  * circuit); with grads != NULL also grads[par_off[c] + j] = dE_c/dparams[..+j]
  * by the adjoint method (parameterised kinds as qse_adjoint_expect). One
- * workgroup runs each circuit's forward passes, observable, reductions and
- * reverse sweep; circuits are taken largest first from a device work queue.
- * Complex64 states accumulate energies and gradients in FP64. Synchronises. */
+ * workgroup, or a team of workgroups for circuits of 16 or more qubits, runs
+ * each circuit's forward passes, energy and observable (first launch) and its
+ * reverse sweep (second launch, with gradients only): joint passes over psi
+ * and lambda that un-apply the gates and sum each parameter's generator term
+ * inside the tile. Circuits are taken largest first from device work queues.
+ * Complex64 states accumulate energies and gradients in FP64; sums are formed
+ * in a fixed order (bit-reproducible). Synchronises. */
 int qse_batch_run_small(qse_ctx* ctx, int precision, int ncirc,
                         const int* nq, const int64_t* op_off, const qse_op* ops,
                         const int64_t* par_off, const double* params,
@@ -389,10 +393,12 @@ int qse_batch_run_small(qse_ctx* ctx, int precision, int ncirc,
  * reference): runs the given circuits once per candidate shape (tiles of 12,
  * 11 or 10 qubits, 2 to 5 contiguous low qubits, with teams of 16
  * workgroups), then with the best shape once per team size (4, 8, 32
- * workgroups per circuit of 16 or more qubits), and keeps the fastest in the
- * context for later batches of this precision (pass a sample of a large
- * batch). with_grads: time energies plus gradients (par_off needed) or
- * energies only. Outputs the chosen shape, team size and time (may be null). */
+ * workgroups per circuit of 16 or more qubits), compares the engine's kernel
+ * times (HIP events; the context's timing totals are left as they were), and
+ * keeps the fastest in the context for later batches of this precision (pass
+ * a sample of a large batch). with_grads: time energies plus gradients
+ * (par_off needed) or energies only. Outputs the chosen shape, team size and
+ * kernel milliseconds (may be null). */
 int qse_batch_autotune(qse_ctx* ctx, int precision, int ncirc, const int* nq, const int64_t* op_off,
                        const qse_op* ops, const int64_t* par_off, const double* params,
                        const uint64_t* obs_masks, const double* obs_weights, int nterms, int with_grads,

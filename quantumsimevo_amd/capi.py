@@ -322,8 +322,9 @@ class Context:
 
     # ---- batched entry points ----
     def batch_run_small(self, precision, circuits, obs_masks, obs_weights, grads: bool = True):
-        """Many circuits [(n, ops, params), ...] of uneven sizes in one persistent
-        launch: energies (ncirc,) and, with grads, a list of gradient vectors."""
+        """Many circuits [(n, ops, params), ...] of uneven sizes in persistent
+        launches (forward; with grads, the reverse sweeps): energies (ncirc,)
+        and, with grads, a list of gradient vectors."""
         from .circuits import pack_batch
         nq, op_off, ops, par_off, params = pack_batch(circuits)
         ops = as_ops(ops) if len(ops) else np.zeros(0, dtype=OP_DTYPE)
