@@ -227,13 +227,59 @@ def noise_leg(ctx) -> dict:
     ref = s128.download()
     s128.close()
     fid = float(abs(np.vdot(ref, psi)) ** 2 / (np.vdot(psi, psi).real * np.vdot(ref, ref).real))
+    inter = interleaved_noise(ctx, n, ops, ntraj, u)
     return {"workload": f"{ntraj} noise trajectories x {shots} shots, {n}-qubit random layered circuit depth {depth}, "
-                        "complex64 state, FP64 CDF", "seconds": t2 - t0, "frames_seconds": t1 - t0,
+                        "complex64 state, FP64 CDF",
+            "mode": "deferred: the drawn flips as one Pauli frame after the last gate (the reference's model, "
+                    "Noise.cpp:34-44)", "seconds": t2 - t0, "frames_seconds": t1 - t0,
             "trajectories_per_s": ntraj / (t2 - t0), "shots_per_s": ntraj * shots / (t2 - t0),
             "distinct_frames": int(len(np.unique(x))), "outcomes_checksum": int(np.bitwise_xor.reduce(out.ravel())),
             "c64_max_amp_err_vs_fp64": float(np.max(np.abs(psi - ref))), "c64_infidelity_vs_fp64": 1.0 - fid,
             "sampling": "outcomes equal the sequential-CDF lower_bound on the complex64 amplitudes (bit-exact, "
-                        "tests/test_gpu_configs.py)"}
+                        "tests/test_gpu_configs.py)",
+            "interleaved": inter}
+
+
+def interleaved_noise(ctx, n, ops, ntraj, u) -> dict:
+    """The same trajectories (seeds 0..ntraj-1, the same draws) in the
+    interleaved mode: each drawn flip acts right after its gate, so every
+    trajectory with a flip is a gate list of its own, run from |0...0> and
+    sampled; trajectories with identical flips share one run. The tile shape
+    is fixed (12 qubits, runs of 16 amplitudes) so that each distinct list is
+    planned once without the shape search, and each runs on the generic pass
+    kernel (a list that runs once is not worth compiling)."""
+    keep = ctx.get_fusion(capi.C64)
+    ctx.set_fusion(capi.C64, 12, 4)
+    ctx.set_jit(capi.JIT_OFF)  # each distinct list runs once: no run-time compilation
+    try:
+        st = ctx.state(n, capi.C64)
+        st.set_basis(0)
+        st.run(ops)
+        ctx.sync()
+        t0 = time.perf_counter()
+        trajs = C.interleaved_trajectories(n, ops, np.arange(ntraj, dtype=np.uint32), 1e-3, 1.0)
+        t1 = time.perf_counter()
+        groups = {}
+        for j, t in enumerate(trajs):
+            groups.setdefault(t.tobytes(), []).append(j)
+        csum = 0
+        for key, members in groups.items():
+            st.set_basis(0)
+            st.run(trajs[members[0]])
+            for j in members:
+                csum ^= int(np.bitwise_xor.reduce(st.sample(u[j])))
+        ctx.sync()
+        t2 = time.perf_counter()
+        st.close()
+    finally:
+        ctx.set_fusion(capi.C64, *keep)
+        ctx.set_jit(capi.JIT_AUTO)
+    return {"mode": "interleaved: each drawn flip right after its gate (no counterpart in the reference)",
+            "seconds": t2 - t0, "insertions_seconds": t1 - t0, "distinct_runs": len(groups),
+            "trajectories_per_s": ntraj / (t2 - t0), "shots_per_s": ntraj * u.shape[1] / (t2 - t0),
+            "outcomes_checksum": csum,
+            "check": "each trajectory's state and samples against the oracle's replay of its gate list "
+                     "(tests/test_gpu_noise_modes.py)"}
 
 
 def uneven_batch_leg(ctx) -> dict:
@@ -571,7 +617,8 @@ def main() -> int:
         result["qaoa"] = qaoa_leg(ctx, args.qaoa_batch, args.qaoa_shift_sets)
     if "noise" in legs:
         result["noise_trajectories"] = noise_leg(ctx)
-        log(f"noise trajectories: {result['noise_trajectories']['seconds']:.3f} s")
+        log(f"noise trajectories: {result['noise_trajectories']['seconds']:.3f} s deferred, "
+            f"{result['noise_trajectories']['interleaved']['seconds']:.3f} s interleaved")
     if "batch" in legs:
         result["uneven_batch"] = uneven_batch_leg(ctx)
         log(f"uneven batch: {result['uneven_batch']['seconds']:.3f} s")

@@ -71,6 +71,10 @@ public:
     void loadGates(const std::string& path);
     size_t numQubits() const { return stateVector.qubits(); }
     Noise& noise() { return pauliNoise; }
+    // Deferred (default, the reference's frame after the last gate) or
+    // Interleaved (each drawn flip right after its gate); see NoiseMode.
+    void setNoiseMode(NoiseMode mode) { noiseMode = mode; }
+    NoiseMode getNoiseMode() const { return noiseMode; }
     CircuitUnitaryOperationHIP& backend();
 
 private:
@@ -79,6 +83,7 @@ private:
     std::unique_ptr<ICircuitUnitaryOperation> circuitOp;
     bool simulateNoise;
     Noise pauliNoise;
+    NoiseMode noiseMode = NoiseMode::Deferred;
     std::mt19937 readoutGen;
     std::vector<Gate2x2> matrices;  // generic 2x2 gates queued with gate()
     std::vector<int> results;       // outcomes of MEASURE commands by result slot

@@ -13,6 +13,13 @@
 #include "ICircuitUnitaryOperation.h"
 #include "Structurs.h"
 
+// Where the drawn flips act. Deferred: the reference's model (Circuit.cpp:64-120,
+// Noise.cpp:34-44): the flips are counted during the run and their parities
+// applied as one Pauli frame after the last gate. Interleaved (no counterpart in
+// the reference): each drawn flip acts right after the gate it was drawn for,
+// X before Z. Both modes make the same draws in the same order.
+enum class NoiseMode { Deferred, Interleaved };
+
 class Noise {
 public:
     explicit Noise(size_t N, unsigned int seed = 42, double errorRateMultiplier = 1);
@@ -28,6 +35,9 @@ public:
     uint64_t bitFlipMask() const;
     uint64_t phaseFlipMask() const;
     void setRate(GateType type, double rate);  // rate < 0 removes the gate type
+    // accumulateError's two draws, also returned: bit 0 a bit flip, bit 1 a
+    // phase flip on `target` (the interleaved mode applies them at once).
+    int drawError(GateType type, size_t target);
 
 private:
     std::vector<int> bitFlipCounts;

@@ -324,6 +324,43 @@ def uneven_ansatz_batch(count: int, seed: int, n_range=(10, 22), depth_range=(4,
     return out
 
 
+def interleaved_trajectories(n: int, ops: np.ndarray, seeds, rate: float = -1.0, multiplier: float = 1.0):
+    """Pauli-noise trajectories in the interleaved mode: for each seed, the
+    gate list with each drawn flip inserted right after the gate it was drawn
+    for (X before Z, on that gate's target). The draws are the Noise class's
+    (two per eligible gate, the reference's order, Noise.cpp:21-32), the ones
+    noise_frames folds into one frame after the last gate in the reference's
+    deferred mode. Returns [ops_j, ...] (an unchanged op array when nothing
+    was drawn)."""
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    off, gate, code = _native().noise_insertions(n, ops["kind"], ops["target"],
+                                                 np.ascontiguousarray(seeds, dtype=np.uint32), rate, multiplier)
+    out = []
+    for j in range(len(off) - 1):
+        a, b = int(off[j]), int(off[j + 1])
+        if a == b:
+            out.append(ops)
+            continue
+        pieces, prev = [], 0
+        for g, c in zip(gate[a:b], code[a:b]):
+            g = int(g)
+            pieces.append(ops[prev:g + 1])
+            extra = np.zeros(2, dtype=OP_DTYPE)
+            k = 0
+            for bit, kind in ((1, X), (2, Z)):
+                if int(c) & bit:
+                    extra[k]["kind"] = kind
+                    extra[k]["target"] = ops[g]["target"]
+                    extra[k]["control"] = -1
+                    extra[k]["param"] = -1
+                    k += 1
+            pieces.append(extra[:k])
+            prev = g + 1
+        pieces.append(ops[prev:])
+        out.append(np.concatenate(pieces))
+    return out
+
+
 def amp_updates(ops: np.ndarray, n: int) -> float:
     """Algorithmic amplitude updates sum_g T(g): N for H, X, Y, RX, RY, RZ,
     generic 2x2; N/2 for Z, phase, CNOT, SWAP; N/4 for CPhase; N/2^m for

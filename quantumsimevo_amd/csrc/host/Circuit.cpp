@@ -107,13 +107,15 @@ void Circuit::queueReset(size_t q) {
 }
 
 // Circuit.cpp:64-120: replay from the current state, noise draws before each gate,
-// noise frame after the last one.
+// noise frame after the last one (Deferred); Interleaved applies each gate's drawn
+// flips right after the gate instead.
 void Circuit::execute(bool print_steps) {
     if (print_steps) std::cout << "Initial state: |0...0>\n";
     circuitOp->setPrintSteps(print_steps);
+    const bool interleaved = simulateNoise && noiseMode == NoiseMode::Interleaved;
     for (const auto& cmd : commandList) {
         if (print_steps) std::cout << "Executing: " << cmd.description << std::endl;
-        if (simulateNoise) pauliNoise.accumulateError(cmd.type, cmd.target);
+        const int drawn = simulateNoise ? pauliNoise.drawError(cmd.type, cmd.target) : 0;
         switch (cmd.type) {
             case GateType::H: circuitOp->applyHadamard(stateVector, cmd.target); break;
             case GateType::X: circuitOp->applyPauliX(stateVector, cmd.target); break;
@@ -143,8 +145,12 @@ void Circuit::execute(bool print_steps) {
             }
             default: std::cerr << "Unsupported gate type: " << static_cast<int>(cmd.type) << std::endl;
         }
+        if (interleaved) {
+            if (drawn & 1) circuitOp->applyPauliX(stateVector, cmd.target);
+            if (drawn & 2) circuitOp->applyPauliZ(stateVector, cmd.target);
+        }
     }
-    if (simulateNoise) pauliNoise.applyPauliNoise(stateVector, *circuitOp);
+    if (simulateNoise && !interleaved) pauliNoise.applyPauliNoise(stateVector, *circuitOp);
     if (print_steps)
         std::cout << "Total of occured errors: \n BitFlips:" << pauliNoise.getTotalBitFlips()
                   << " times\n PhaseFlips:" << pauliNoise.getTotalPhaseFlips() << " times.\n";

@@ -18,12 +18,22 @@ Noise::Noise(size_t N, unsigned int seed, double errorRateMultiplier)
 }
 
 // Noise.cpp:21-32: two draws per eligible gate, always both.
-void Noise::accumulateError(GateType type, size_t target) {
+void Noise::accumulateError(GateType type, size_t target) { drawError(type, target); }
+
+int Noise::drawError(GateType type, size_t target) {
     auto it = errorRates.find(type);
-    if (it == errorRates.end()) return;
+    if (it == errorRates.end()) return 0;
     const double p = it->second * errorMultiplier;
-    if (dist(gen) < p) bitFlipCounts[target]++;
-    if (dist(gen) < p) phaseFlipCounts[target]++;
+    int drawn = 0;
+    if (dist(gen) < p) {
+        bitFlipCounts[target]++;
+        drawn |= 1;
+    }
+    if (dist(gen) < p) {
+        phaseFlipCounts[target]++;
+        drawn |= 2;
+    }
+    return drawn;
 }
 
 // Noise.cpp:34-44: X_q then Z_q for every qubit with an odd count, ascending q.

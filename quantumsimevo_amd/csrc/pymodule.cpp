@@ -29,14 +29,29 @@ static int parse_precision(const py::object& p) {
     return p.cast<int>();
 }
 
+// "deferred" (the reference's frame after the last gate) or "interleaved".
+static NoiseMode parse_noise_mode(const std::string& mode) {
+    if (mode == "deferred") return NoiseMode::Deferred;
+    if (mode == "interleaved") return NoiseMode::Interleaved;
+    throw std::invalid_argument("noise_mode must be 'deferred' or 'interleaved'");
+}
+
 PYBIND11_MODULE(_quantumsimevo, m) {
     m.doc() = "QuantumSimEvo circuits on the MI355X (gfx950) HIP backend";
 
     py::class_<Circuit>(m, "Circuit")
-        .def(py::init([](size_t n, bool noise, py::object precision) {
-                 return new Circuit(n, noise, parse_precision(precision));
+        .def(py::init([](size_t n, bool noise, py::object precision, const std::string& noise_mode) {
+                 auto* c = new Circuit(n, noise, parse_precision(precision));
+                 c->setNoiseMode(parse_noise_mode(noise_mode));
+                 return c;
              }),
-             py::arg("num_qubits"), py::arg("simulate_pauli_noise") = false, py::arg("precision") = "c128")
+             py::arg("num_qubits"), py::arg("simulate_pauli_noise") = false, py::arg("precision") = "c128",
+             py::arg("noise_mode") = "deferred")
+        .def("set_noise_mode", [](Circuit& c, const std::string& mode) { c.setNoiseMode(parse_noise_mode(mode)); },
+             py::arg("mode"))
+        .def_property_readonly("noise_mode", [](const Circuit& c) {
+            return std::string(c.getNoiseMode() == NoiseMode::Interleaved ? "interleaved" : "deferred");
+        })
         // single-qubit gates (QuantumSimEvoPy.cpp:16-24)
         .def("hadamard", &Circuit::hadamard, py::arg("q"))
         .def("pauli_x", &Circuit::pauliX, py::arg("q"))
@@ -101,6 +116,9 @@ PYBIND11_MODULE(_quantumsimevo, m) {
                             py::arg("x_mask") = c.noise().bitFlipMask(),
                             py::arg("z_mask") = c.noise().phaseFlipMask());
         })
+        .def("set_noise_rate",
+             [](Circuit& c, int gate_type, double rate) { c.noise().setRate((GateType)gate_type, rate); },
+             py::arg("gate_type"), py::arg("rate"))
         .def("set_fusion", [](Circuit& c, bool on) { c.backend().setFusion(on); }, py::arg("fused"))
         // gate-list files (GateListFile.h)
         .def("save_gates", &Circuit::saveGates, py::arg("path"))
@@ -221,6 +239,64 @@ PYBIND11_MODULE(_quantumsimevo, m) {
                   for (auto& th : pool) th.join();
               }
               return py::make_tuple(xs, zs);
+          },
+          py::arg("n"), py::arg("kinds"), py::arg("targets"), py::arg("seeds"), py::arg("rate") = -1.0,
+          py::arg("multiplier") = 1.0);
+    // The same draws per trajectory as noise_frames, as insertions for the
+    // interleaved mode: trajectory j's flips are (gate[i], code[i]) for i in
+    // [offsets[j], offsets[j+1]), gates ascending, code bit 0 = X and bit 1 = Z on
+    // that gate's target, acting right after the gate.
+    m.def("noise_insertions",
+          [](size_t n, py::array_t<int, py::array::c_style | py::array::forcecast> kinds,
+             py::array_t<int, py::array::c_style | py::array::forcecast> targets,
+             py::array_t<unsigned int, py::array::c_style | py::array::forcecast> seeds, double rate,
+             double multiplier) {
+              if (kinds.size() != targets.size()) throw std::invalid_argument("kinds and targets differ in length");
+              if (n < 1 || n > 64) throw std::invalid_argument("1..64 qubits");
+              const int* k = kinds.data();
+              const int* t = targets.data();
+              const unsigned int* sd = seeds.data();
+              const ssize_t ng = kinds.size(), nt = seeds.size();
+              std::vector<std::vector<std::pair<int32_t, int8_t>>> per(nt);
+              {
+                  py::gil_scoped_release nogil;
+                  auto work = [&](ssize_t j0, ssize_t j1) {
+                      for (ssize_t j = j0; j < j1; ++j) {
+                          Noise nz(n, sd[j], multiplier);
+                          if (rate >= 0.0)
+                              for (GateType g : {GateType::X, GateType::Y, GateType::Z, GateType::H, GateType::CNOT,
+                                                 GateType::PHASE, GateType::RX, GateType::RY, GateType::RZ})
+                                  nz.setRate(g, rate);
+                          for (ssize_t i = 0; i < ng; ++i) {
+                              const int d = nz.drawError((GateType)k[i], (size_t)t[i]);
+                              if (d) per[j].push_back({(int32_t)i, (int8_t)d});
+                          }
+                      }
+                  };
+                  const ssize_t nthr = std::min<ssize_t>(
+                      {nt / 16 + 1, (ssize_t)std::max(1u, std::thread::hardware_concurrency()), (ssize_t)16});
+                  std::vector<std::thread> pool;
+                  for (ssize_t w = 1; w < nthr; ++w) pool.emplace_back(work, nt * w / nthr, nt * (w + 1) / nthr);
+                  work(0, nt / nthr);
+                  for (auto& th : pool) th.join();
+              }
+              size_t total = 0;
+              for (const auto& v : per) total += v.size();
+              py::array_t<int64_t> offsets(nt + 1);
+              py::array_t<int32_t> gate(total);
+              py::array_t<int8_t> code(total);
+              int64_t* o = offsets.mutable_data();
+              int32_t* g = gate.mutable_data();
+              int8_t* c = code.mutable_data();
+              o[0] = 0;
+              for (ssize_t j = 0; j < nt; ++j) {
+                  for (size_t i = 0; i < per[j].size(); ++i) {
+                      g[o[j] + i] = per[j][i].first;
+                      c[o[j] + i] = per[j][i].second;
+                  }
+                  o[j + 1] = o[j] + (int64_t)per[j].size();
+              }
+              return py::make_tuple(offsets, gate, code);
           },
           py::arg("n"), py::arg("kinds"), py::arg("targets"), py::arg("seeds"), py::arg("rate") = -1.0,
           py::arg("multiplier") = 1.0);

@@ -158,9 +158,16 @@ int jit_for_structure(qse_ctx* ctx, const HostPlan& P, const Serialized& S, int 
         }
     if (!hit) {
         if (ctx->struct_cache.size() >= STRUCT_CACHE_MAX) {
-            size_t oldest = 0;
-            for (size_t i = 1; i < ctx->struct_cache.size(); ++i)
-                if (ctx->struct_cache[i].stamp < ctx->struct_cache[oldest].stamp) oldest = i;
+            // least recently used entry whose compilation is not running (waiting for a
+            // running one would stall the caller for the compile time); when every
+            // entry is compiling, this structure is not cached
+            size_t oldest = ctx->struct_cache.size();
+            for (size_t i = 0; i < ctx->struct_cache.size(); ++i) {
+                const StructEntry& c = ctx->struct_cache[i];
+                if (c.job && c.job->state.load(std::memory_order_acquire) == 0) continue;
+                if (oldest == ctx->struct_cache.size() || c.stamp < ctx->struct_cache[oldest].stamp) oldest = i;
+            }
+            if (oldest == ctx->struct_cache.size()) return QSE_OK;
             const int rc = drop_entry(ctx, ctx->struct_cache[oldest]);
             if (rc) return rc;
             ctx->struct_cache.erase(ctx->struct_cache.begin() + oldest);
