@@ -341,6 +341,32 @@ def uneven_batch_leg(ctx) -> dict:
                                                 "call per circuit"}}
 
 
+def exchange_leg(ctx, n: int = 30) -> dict:
+    """Pair exchange for gates on the lowest qubits, measured both ways
+    (qse_exchange_probe): reps layers of RY on qubits 0..5 in one pass over a
+    30-qubit complex128 state, amplitude pairs brought together through LDS
+    between register layouts (the fused pass's way) or across wavefront lanes
+    by DPP / ds_swizzle / ds_bpermute. Each figure is the second of two
+    launches (HIP events); the rates count one read and one write of the state."""
+    st = ctx.state(n, capi.C128)
+    st.set_random(3)
+    rows = []
+    nbytes = 2.0 * 16.0 * 2 ** n
+    for reps in (1, 2, 4, 8, 16):
+        row = {"reps": reps, "gates": 6 * reps}
+        for method, name in ((0, "lds"), (1, "dpp")):
+            st.exchange_probe(method, reps)
+            ms = st.exchange_probe(method, reps)
+            row[f"{name}_ms"] = ms
+            row[f"{name}_gbs"] = nbytes / (ms * 1e-3) / 1e9
+        row["faster"] = "lds" if row["lds_ms"] <= row["dpp_ms"] else "dpp"
+        rows.append(row)
+    st.close()
+    return {"workload": f"reps x RY on qubits 0..5 in one pass, {n}-qubit complex128 state, tiles of 4096 contiguous "
+                        "amplitudes, 256 threads x 16", "rows": rows,
+            "profile": "profiles/k_exchange_lds_dpp_30q_c128_kernel_stats.md"}
+
+
 def layered_leg(ctx, n: int = 30, layers: int = 20) -> dict:
     """The plain-method random circuit at 30 qubits complex128: per layer one
     of H/RX/RY/RZ/phase per qubit (angles uniform in [0, 2pi)), then a CNOT
@@ -570,7 +596,7 @@ def main() -> int:
     ap.add_argument("--verify", type=int, default=16, metavar="K",
                     help="after timing, rerun one QFT from the seed and compare K output amplitudes with the FP64 "
                          "DFT closed form evaluated on the GPU (qse_qft_probe)")
-    ap.add_argument("--legs", default="qft,layered,dense,qaoa,noise,batch,cpu",
+    ap.add_argument("--legs", default="qft,layered,dense,exchange,qaoa,noise,batch,cpu",
                     help="comma list of parts to run (qft = the headline QFT line; the others are the extra legs); "
                          "a leg alone, e.g. --legs noise, is what a profiler run of that leg uses")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -602,7 +628,7 @@ def main() -> int:
     info = capi.device_info(0)
     log(f"device: {info['name']} CUs={info['compute_units']} HBM={info['total_bytes'] / 2**30:.1f} GiB")
     legs = set(args.legs.split(","))
-    for flag, names in ((args.no_extra, ("layered", "noise", "batch")), (args.no_dense, ("dense",)),
+    for flag, names in ((args.no_extra, ("layered", "noise", "batch", "exchange")), (args.no_dense, ("dense",)),
                         (args.no_qaoa, ("qaoa",)), (args.no_cpu, ("cpu",))):
         if flag:
             legs -= set(names)
@@ -613,6 +639,8 @@ def main() -> int:
     if "dense" in legs:
         result["dense_block"] = dense_block_leg(
             ctx, {"auto": capi.JIT_AUTO, "off": capi.JIT_OFF, "always": capi.JIT_ALWAYS}[args.jit])
+    if "exchange" in legs and n == 30 and prec == capi.C128:
+        result["pair_exchange"] = exchange_leg(ctx)
     if "qaoa" in legs:
         result["qaoa"] = qaoa_leg(ctx, args.qaoa_batch, args.qaoa_shift_sets)
     if "noise" in legs:

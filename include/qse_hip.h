@@ -262,6 +262,14 @@ int qse_qft_probe(qse_ctx* ctx, int nqubits, uint64_t seed, const uint64_t* indi
  * pattern of a fused pass with run_log2 contiguous low qubits (complex128).
  * Allocates two buffers of 16 * 2^log2_elems bytes for the call; synchronises. */
 int qse_copy_bandwidth(qse_ctx* ctx, int log2_elems, int run_log2, double* gbs);
+/* Measurement of a kernel choice (no counterpart in the reference): `reps`
+ * layers of RY(0.3 + 0.1 q) on qubits q = 0..5 (in that order) applied in
+ * place to a complex128 state of 12 or more qubits in one pass, with the
+ * amplitude pairs of those qubits brought together through LDS between
+ * register layouts (method 0, the fused pass's exchange) or across the lanes
+ * of a wavefront by DPP / ds_swizzle / ds_bpermute (method 1). Both give the
+ * same state to rounding; ms = the launch's HIP-event time. Synchronises. */
+int qse_exchange_probe(qse_ctx* ctx, qse_state* st, int method, int reps, double* ms);
 
 /* ---- dense blocks on the matrix cores (no counterpart in the reference) ----
  * qse_compose_dense: the 2^m x 2^m unitary of a gate list on qubits 0..m-1

@@ -41,7 +41,7 @@ EXPORTS = [
     "qse_state_download_range", "qse_state_set_basis", "qse_state_copy", "qse_state_set_random", "qse_apply_matrix",
     "qse_apply_hadamard", "qse_apply_x", "qse_apply_y", "qse_apply_z", "qse_apply_phase", "qse_apply_rx",
     "qse_apply_ry", "qse_apply_rz", "qse_apply_cnot", "qse_apply_cphase", "qse_apply_mcphase",
-    "qse_apply_swap", "qse_pauli_frame", "qse_run", "qse_run_repeat", "qse_run_measured", "qse_run_sweeps", "qse_plan_passes", "qse_plan_info", "qse_qft_probe", "qse_copy_bandwidth", "qse_sample",
+    "qse_apply_swap", "qse_pauli_frame", "qse_run", "qse_run_repeat", "qse_run_measured", "qse_run_sweeps", "qse_plan_passes", "qse_plan_info", "qse_qft_probe", "qse_copy_bandwidth", "qse_exchange_probe", "qse_sample",
     "qse_measure", "qse_norm2", "qse_prob_one", "qse_collapse_qubit", "qse_expect_zstrings", "qse_expect_zstrings_noalloc",
     "qse_adjoint_expect", "qse_batch_expect", "qse_batch_run_small", "qse_sample_frames",
     "qse_compose_dense", "qse_apply_dense", "qse_state_renormalize", "qse_ctx_set_renormalize",
@@ -120,6 +120,7 @@ def _sigs(lib):
         "qse_plan_info": ([_vp, _i, _i, _vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)], _i),
         "qse_qft_probe": ([_vp, _i, C.c_uint64, _vp, _i, _vp], _i),
         "qse_copy_bandwidth": ([_vp, _i, _i, _pd], _i),
+        "qse_exchange_probe": ([_vp, _vp, _i, _i, _pd], _i),
         "qse_compose_dense": ([_i, _vp, _i, _vp, _vp], _i),
         "qse_state_renormalize": ([_vp, _vp, _vp], _i),
         "qse_ctx_set_renormalize": ([_vp, _i], _i),
@@ -506,6 +507,14 @@ class State:
         check(self.lib.qse_apply_dense(self.ctx.h, self.h, int(m), _ptr(uu)), "qse_apply_dense")
 
     # readout
+    def exchange_probe(self, method: int, reps: int) -> float:
+        """reps layers of RY(0.3 + 0.1 q) on qubits 0..5 applied in place in one pass,
+        pairs exchanged through LDS (method 0) or across wavefront lanes by
+        DPP/ds_swizzle (method 1); returns the launch's milliseconds."""
+        ms = C.c_double(0)
+        check(self.lib.qse_exchange_probe(self.ctx.h, self.h, method, reps, C.byref(ms)), "qse_exchange_probe")
+        return ms.value
+
     def sample(self, u) -> np.ndarray:
         uu = np.ascontiguousarray(u, dtype=np.float64)
         out = np.empty(len(uu), dtype=np.uint64)

@@ -108,6 +108,137 @@ __global__ __launch_bounds__(256) void k_copy_probe(const double2* __restrict__ 
     }
 }
 
+// ---- pair exchange for gates on low qubits, two ways (a measured kernel choice) ----
+// Both kernels apply `reps` layers of RY(theta_q), q = 0..5 in that order
+// (theta_q = 0.3 + 0.1 q), in place to a complex128 state, tile by tile (4096
+// contiguous amplitudes = qubits 0..11 per tile, 256 threads x 16 amplitudes),
+// with the same arithmetic per amplitude pair.
+constexpr int XP_REG = 16;
+typedef double v2d __attribute__((ext_vector_type(2)));  // one complex128 amplitude
+
+__device__ __forceinline__ void ry_pair(v2d& a0, v2d& a1, double c, double s) {
+    const v2d v0 = a0, v1 = a1;
+    a0 = v2d{c * v0.x - s * v1.x, c * v0.y - s * v1.y};
+    a1 = v2d{s * v0.x + c * v1.x, s * v0.y + c * v1.y};
+}
+
+// LDS exchange (the fused pass's): registers hold qubits 0..3 (layout A) or
+// 4..7 (layout B); between them the tile goes through LDS, padded by one
+// 16-byte slot per 16 (element li at li + li/16).
+__global__ __launch_bounds__(256) void k_exchange_lds(v2d* __restrict__ st, uint64_t ntiles, int reps) {
+    __shared__ v2d t[4096 + 256];
+    const int tid = threadIdx.x;
+    const int offA = 17 * tid;                                       // li = 16 tid + i
+    const int liB = (tid & 15) | ((tid >> 4) << 8);                  // li = liB + 16 i
+    auto pad = [](int li) { return li + (li >> 4); };
+    for (uint64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        v2d* p = st + (ti << 12);
+        v2d a[XP_REG];
+#pragma unroll
+        for (int i = 0; i < XP_REG; ++i) t[pad(tid + 256 * i)] = __builtin_nontemporal_load(p + tid + 256 * i);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < XP_REG; ++i) a[i] = t[offA + i];
+        for (int r = 0; r < reps; ++r) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                double sn, cs;
+                sincos(0.5 * (0.3 + 0.1 * q), &sn, &cs);
+#pragma unroll
+                for (int i = 0; i < XP_REG; ++i)
+                    if (!((i >> q) & 1)) ry_pair(a[i], a[i | (1 << q)], cs, sn);
+            }
+#pragma unroll
+            for (int i = 0; i < XP_REG; ++i) t[offA + i] = a[i];
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < XP_REG; ++i) a[i] = t[pad(liB + 16 * i)];
+#pragma unroll
+            for (int q = 4; q < 6; ++q) {
+                double sn, cs;
+                sincos(0.5 * (0.3 + 0.1 * q), &sn, &cs);
+#pragma unroll
+                for (int i = 0; i < XP_REG; ++i)
+                    if (!((i >> (q - 4)) & 1)) ry_pair(a[i], a[i | (1 << (q - 4))], cs, sn);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < XP_REG; ++i) t[pad(liB + 16 * i)] = a[i];
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < XP_REG; ++i) a[i] = t[offA + i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < XP_REG; ++i) t[offA + i] = a[i];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < XP_REG; ++i) __builtin_nontemporal_store(t[pad(tid + 256 * i)], p + tid + 256 * i);
+        __syncthreads();
+    }
+}
+
+// The value of lane (lane ^ (1 << Q)) of a 32-bit word: DPP quad_perm (Q = 0, 1),
+// two DPP mirrors (Q = 2: row_half_mirror then quad_perm [3,2,1,0]; Q = 3:
+// row_mirror then row_half_mirror), ds_swizzle in bit mode (Q = 4), ds_bpermute
+// (Q = 5).
+template <int Q>
+__device__ __forceinline__ int xlane(int v) {
+    if constexpr (Q == 0) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);
+    else if constexpr (Q == 1) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);
+    else if constexpr (Q == 2) return __builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false), 0x1B, 0xF, 0xF, false);
+    else if constexpr (Q == 3) return __builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false), 0x141, 0xF, 0xF, false);
+    else if constexpr (Q == 4) return __builtin_amdgcn_ds_swizzle(v, 0x401F);
+    else return __builtin_amdgcn_ds_bpermute((int)((threadIdx.x & 63) ^ 32) << 2, v);
+}
+
+template <int Q>
+__device__ __forceinline__ double xlane_f64(double v) {
+    const uint64_t u = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)xlane<Q>((int)(uint32_t)u), hi = (uint32_t)xlane<Q>((int)(uint32_t)(u >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+template <int Q>
+__device__ __forceinline__ void ry_lanes(v2d* a, bool upper, double c, double s) {
+    // lane with bit Q clear: c a0 - s a1; set: s a0 + c a1 = c a1 + s (partner)
+    const double sg = upper ? s : -s;
+#pragma unroll
+    for (int i = 0; i < XP_REG; ++i) {
+        const double px = xlane_f64<Q>(a[i].x), py = xlane_f64<Q>(a[i].y);
+        a[i] = v2d{c * a[i].x + sg * px, c * a[i].y + sg * py};
+    }
+}
+
+// Cross-lane exchange: lanes hold qubits 0..5, registers qubits 6..9, the
+// four waves qubits 10, 11; no LDS.
+__global__ __launch_bounds__(256) void k_exchange_dpp(v2d* __restrict__ st, uint64_t ntiles, int reps) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (uint64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        v2d* p = st + (ti << 12) + (w << 10) + lane;
+        v2d a[XP_REG];
+#pragma unroll
+        for (int i = 0; i < XP_REG; ++i) a[i] = __builtin_nontemporal_load(p + (i << 6));
+        for (int r = 0; r < reps; ++r) {
+            double sn, cs;
+            sincos(0.5 * 0.3, &sn, &cs);
+            ry_lanes<0>(a, lane & 1, cs, sn);
+            sincos(0.5 * 0.4, &sn, &cs);
+            ry_lanes<1>(a, (lane >> 1) & 1, cs, sn);
+            sincos(0.5 * 0.5, &sn, &cs);
+            ry_lanes<2>(a, (lane >> 2) & 1, cs, sn);
+            sincos(0.5 * 0.6, &sn, &cs);
+            ry_lanes<3>(a, (lane >> 3) & 1, cs, sn);
+            sincos(0.5 * 0.7, &sn, &cs);
+            ry_lanes<4>(a, (lane >> 4) & 1, cs, sn);
+            sincos(0.5 * 0.8, &sn, &cs);
+            ry_lanes<5>(a, (lane >> 5) & 1, cs, sn);
+        }
+#pragma unroll
+        for (int i = 0; i < XP_REG; ++i) __builtin_nontemporal_store(a[i], p + (i << 6));
+    }
+}
+
 uint64_t bitrev(uint64_t v, int n) {
     uint64_t r = 0;
     for (int b = 0; b < n; ++b)
@@ -161,6 +292,36 @@ extern "C" int qse_qft_probe(qse_ctx* ctx, int nqubits, uint64_t seed, const uin
         const double f = 1.0 / (std::sqrt(S) * std::sqrt((double)N));
         for (int c = 0; c < 2 * K; ++c) out[2 * first + c] = acc[c] * f;
     }
+    return QSE_OK;
+}
+
+extern "C" int qse_exchange_probe(qse_ctx* ctx, qse_state* st, int method, int reps, double* ms) {
+    if (!ctx || !st || !st->d || !ms || (method != 0 && method != 1) || reps < 1 || reps > 64)
+        return set_error(QSE_ERR_INVALID_ARG, "bad argument");
+    if (st->prec != QSE_C128 || st->n < 12)
+        return set_error(QSE_ERR_UNSUPPORTED, "exchange probe: complex128 states of 12 or more qubits");
+    QSE_HIP(hipSetDevice(ctx->device));
+    const uint64_t ntiles = 1ULL << (st->n - 12);
+    const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)ctx->num_cus * 16);
+    v2d* s = (v2d*)st->d;
+    auto launch = [&] {
+        if (method == 0) k_exchange_lds<<<grid, 256, 0, ctx->stream>>>(s, ntiles, reps);
+        else k_exchange_dpp<<<grid, 256, 0, ctx->stream>>>(s, ntiles, reps);
+        return hipGetLastError();
+    };
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
+    if (e == hipSuccess) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    if (e != hipSuccess) return hip_error(e, "exchange probe");
+    *ms = t;
     return QSE_OK;
 }
 
