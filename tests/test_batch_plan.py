@@ -103,3 +103,36 @@ def test_energies_only_and_qaoa_cost_gate():
     E, Gr = run_emu([(n, ops, par), (n, ops, par[::-1].copy())], masks, w)
     assert np.max(np.abs(Gr[0] - oracle_shift(n, ops, par, masks, w))) < 1e-10
     assert np.max(np.abs(Gr[1] - oracle_shift(n, ops, par[::-1].copy(), masks, w))) < 1e-10
+
+
+def many_probe_circuit(n=12, layers=30):
+    """RZ and RY on every qubit, layer after layer, all on one tile: the reverse
+    sweep holds 2 n layers probes, far more than one joint pass's 128
+    accumulators, so it is planned in chunks."""
+    g = G.GateList(n)
+    p = 0
+    for _ in range(layers):
+        for q in range(n):
+            g.rotate_z(0.0, q, param=p, scale=1.0)
+            g.rotate_y(0.0, q, param=p + 1, scale=1.0)
+            p += 2
+    return g.ops(), np.random.default_rng(0).random(p)
+
+
+def test_reverse_sweep_in_probe_chunks():
+    n = 12
+    ops, par = many_probe_circuit(n)
+    masks = [1 << q for q in range(n)]
+    weights = [1.0 + 0.1 * q for q in range(n)]
+    E, Gr = run_emu([(n, ops, par)], masks, weights, kmax=12, L=3)
+    assert abs(E[0] - oracle_energy(n, ops, par, masks, weights)) < 1e-12
+    assert np.max(np.abs(Gr[0] - oracle_shift(n, ops, par, masks, weights))) < 1e-10
+    lib = emu()
+    nq, op_off, opsb, par_off, params = G.pack_batch([(n, ops, par)])
+    out = np.zeros(5, dtype=np.int64)
+    m = np.ascontiguousarray(masks, dtype=np.uint64)
+    lib.emu_batch_stats.argtypes = [C.c_int] + [C.c_void_p] * 6 + [C.c_int, C.c_int, C.c_int, C.c_void_p]
+    assert lib.emu_batch_stats(1, nq.ctypes.data, op_off.ctypes.data, opsb.ctypes.data, par_off.ctypes.data,
+                               params.ctypes.data, m.ctypes.data, len(m), 12, 3, out.ctypes.data) == 0
+    # 720 probes over 128-accumulator passes: at least 6 joint steps besides the forward program
+    assert out[3] >= 7

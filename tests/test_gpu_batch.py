@@ -1,4 +1,4 @@
-"""Batched small circuits in one persistent launch (qse_batch_run_small) on the
+"""Batched small circuits in persistent launches (qse_batch_run_small) on the
 GPU: energies and adjoint gradients of circuits of different sizes against the
 oracle (energies 1e-12, gradients vs parameter shift 1e-10 in complex128;
 1e-4 in complex64) and against the host-driven adjoint (qse_adjoint_expect)."""
@@ -9,7 +9,7 @@ import pytest
 
 from quantumsimevo_amd import capi
 from quantumsimevo_amd import circuits as G
-from test_batch_plan import mixed_param_circuit, oracle_energy, oracle_shift
+from test_batch_plan import many_probe_circuit, mixed_param_circuit, oracle_energy, oracle_shift
 
 pytestmark = pytest.mark.gpu
 
@@ -100,3 +100,18 @@ def test_autotuned_shape_keeps_results(ctx, prec, tol_e, tol_g):
     n, ops, par = circs[3]
     zm, zw = G.zsum_observable(n)
     assert abs(E1[3] - oracle_energy(n, ops, par, zm, zw)) < (1e-11 if prec == capi.C128 else 1e-4)
+
+
+@pytest.mark.parametrize("prec,tol_g", [(capi.C128, 1e-10), (capi.C64, 1e-3)])
+def test_reverse_sweep_in_probe_chunks(ctx, prec, tol_g):
+    # 720 generator probes on one 12-qubit tile: the reverse sweep runs as several joint steps,
+    # beside an ordinary circuit of the same batch
+    n = 12
+    ops, par = many_probe_circuit(n)
+    other_ops, other_par = mixed_param_circuit(10, 7)
+    masks = [1 << q for q in range(n)]
+    weights = [1.0 + 0.1 * q for q in range(n)]
+    E, Gr = ctx.batch_run_small(prec, [(n, ops, par), (10, other_ops, other_par)], masks, weights)
+    assert np.max(np.abs(Gr[0] - oracle_shift(n, ops, par, masks, weights))) < tol_g
+    assert np.max(np.abs(Gr[1] - oracle_shift(10, other_ops, other_par, masks, weights))) < tol_g
+    assert abs(E[0] - oracle_energy(n, ops, par, masks, weights)) < (1e-12 if prec == capi.C128 else 1e-4)
