@@ -107,7 +107,7 @@ struct FPass {
     uint32_t tq[16];       // load: tile position b -> source qubit (ascending)
     uint32_t so_q[16];     // store: store position c -> destination qubit (ascending)
     uint32_t so_l[16];     // store: store position c -> tile local bit holding it
-    uint32_t epos[64];     // external source qubits, ascending (tile index bits)
+    uint32_t epos[64];     // tile index bit e -> external source qubit (the lowest first, then descending)
     uint32_t eout[64];     // destination qubit of external source qubit epos[e]
     uint32_t stage_begin;
     uint32_t slot_begin;

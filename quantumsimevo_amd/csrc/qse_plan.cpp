@@ -451,6 +451,14 @@ int plan_items(int n, int k, int L, const std::vector<Item>& items, HostPlan& P,
         fp.next = n - k;
         std::vector<int> tq, epos;
         for (int q = 0; q < n; ++q) ((pi.T >> q) & 1 ? tq : epos).push_back(q);
+        // Tile index bit 0 is the lowest external qubit (the partner of a paired
+        // tile, adjacent runs); bits 1.. take the others from the highest down, so
+        // workgroups running at the same time (consecutive tile indices) work far
+        // apart in the state. Measured on 30-qubit complex128 passes against the
+        // ascending order: QFT 6.97 -> 6.86 ms, layered circuit 7.50 -> 7.37 ms per
+        // pass; interleaving low and high qubits or a rotation was slower (7.27 and
+        // 7.13 ms on QFT).
+        if (epos.size() > 2) std::reverse(epos.begin() + 1, epos.end());
         std::vector<int> loc(64, -1);
         for (int b = 0; b < k; ++b) {
             fp.tq[b] = (uint32_t)tq[b];
