@@ -72,7 +72,8 @@ constexpr uint32_t FUSE_NO_ALPHA = 0xffffffffu;
 // Planner variants (build_plan flags).
 enum PlanFlags : int {
     PLAN_RELABEL = 1,  // SWAPs as a qubit relabelling; last pass out of place
-    PLAN_DAG = 2       // passes by list scheduling over the gate dependency graph
+    PLAN_DAG = 2,      // passes by list scheduling over the gate dependency graph
+    PLAN_LOOKAHEAD = 4 // with PLAN_DAG: a pass's qubit set grows by the most productive ready gate
 };
 
 struct FTerm {

@@ -288,11 +288,30 @@ void build_dag(const std::vector<Item>& v, int n, Dag& D) {
 // qubits (`need`) lie in S; S grows by the requirement of the earliest ready
 // item that does not fit while |S| stays <= cap. Returns the groups in order;
 // their concatenation is a topological order of `v`.
+// With `lookahead`, S instead grows by the requirement of the ready item
+// whose qubits make the most items executable per added qubit (the items
+// that would run in S plus those qubits, given what has run so far).
 template <class Need>
 void list_schedule(const std::vector<Item>& v, int n, uint64_t start, int cap, Need need,
-                   std::vector<std::pair<uint64_t, std::vector<int>>>& groups) {
+                   std::vector<std::pair<uint64_t, std::vector<int>>>& groups, bool lookahead = false) {
     Dag D;
     build_dag(v, n, D);
+    std::vector<int> deg, stack;
+    auto closure = [&](const std::set<int>& ready, uint64_t S2) {
+        deg = D.indeg;
+        stack.clear();
+        for (int i : ready)
+            if ((need(v[i]) & ~S2) == 0) stack.push_back(i);
+        int cnt = 0;
+        while (!stack.empty()) {
+            const int i = stack.back();
+            stack.pop_back();
+            ++cnt;
+            for (int s : D.succ[i])
+                if (--deg[s] == 0 && (need(v[s]) & ~S2) == 0) stack.push_back(s);
+        }
+        return cnt;
+    };
     std::set<int> ready;
     for (int i = 0; i < (int)v.size(); ++i)
         if (D.indeg[i] == 0) ready.insert(i);
@@ -317,12 +336,32 @@ void list_schedule(const std::vector<Item>& v, int n, uint64_t start, int cap, N
             }
             if (any) continue;
             bool grown = false;
-            for (int i : ready) {
-                const uint64_t m = need(v[i]) & ~S;
-                if (popc(S | m) <= cap) {
-                    S |= m;
+            if (lookahead) {
+                uint64_t best_m = 0;
+                double best = -1.0;
+                std::vector<uint64_t> seen;
+                for (int i : ready) {
+                    const uint64_t m = need(v[i]) & ~S;
+                    if (m == 0 || popc(S | m) > cap || std::find(seen.begin(), seen.end(), m) != seen.end()) continue;
+                    seen.push_back(m);
+                    const double sc = (double)closure(ready, S | m) / (double)popc(m);
+                    if (sc > best) {
+                        best = sc;
+                        best_m = m;
+                    }
+                }
+                if (best_m) {
+                    S |= best_m;
                     grown = true;
-                    break;
+                }
+            } else {
+                for (int i : ready) {
+                    const uint64_t m = need(v[i]) & ~S;
+                    if (popc(S | m) <= cap) {
+                        S |= m;
+                        grown = true;
+                        break;
+                    }
                 }
             }
             if (!grown) break;
@@ -402,7 +441,7 @@ int plan_items(int n, int k, int L, const std::vector<Item>& items, HostPlan& P,
     std::vector<PassItems> passes;
     if (flags & PLAN_DAG) {
         std::vector<std::pair<uint64_t, std::vector<int>>> groups;
-        list_schedule(mapped, n, lowT, k, nd_mask, groups);
+        list_schedule(mapped, n, lowT, k, nd_mask, groups, (flags & PLAN_LOOKAHEAD) != 0);
         for (auto& g : groups) {
             PassItems pi;
             pi.T = g.first;
@@ -846,7 +885,8 @@ int choose_plan(int n, int k, int L, const qse_op* ops, int nops, const double* 
     bool have = false;
     long best_cost = 0;
     bool best_oop = false;
-    for (int flags : {int(PLAN_DAG), 0, int(PLAN_RELABEL | PLAN_DAG), int(PLAN_RELABEL)}) {
+    for (int flags : {int(PLAN_DAG), 0, int(PLAN_RELABEL | PLAN_DAG), int(PLAN_RELABEL), int(PLAN_DAG | PLAN_LOOKAHEAD),
+                      int(PLAN_RELABEL | PLAN_DAG | PLAN_LOOKAHEAD)}) {
         if (!allow_oop && (flags & PLAN_RELABEL)) continue;
         HostPlan C;
         const int rc = build_plan(n, k, L, ops, nops, params, C, flags);

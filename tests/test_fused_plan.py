@@ -27,9 +27,10 @@ def emu():
     return _emu
 
 
-# planner variants: bit 0 relabelling SWAPs, bit 1 dependency-graph scheduling;
+# planner variants: bit 0 relabelling SWAPs, bit 1 dependency-graph scheduling, bit 2
+# lookahead growth of the qubit set;
 # -1 is the library's choice among them
-VARIANTS = (0, 1, 2, 3, -1)
+VARIANTS = (0, 1, 2, 3, 6, 7, -1)
 
 
 def run(n, k, L, ops, psi, params=None, relabel=None):
@@ -127,8 +128,9 @@ def test_pass_counts():
     assert info(30, capi.C64, qft) == (3, 13, 4)
     capi.check(lib.qse_plan_passes(None, 30, capi.C128, qft.ctypes.data, len(qft), C.byref(p)))
     assert p.value == 3
-    # the 30-qubit random layered circuit (20 layers): dependency scheduling
-    assert info(30, capi.C128, G.random_layered(30, 20, seed=7).ops()) == (29, 12, 3)
+    # the 30-qubit random layered circuit (20 layers): dependency scheduling with
+    # lookahead growth of each pass's qubit set (29 passes with earliest-gate growth)
+    assert info(30, capi.C128, G.random_layered(30, 20, seed=7).ops()) == (14, 12, 3)
     for rl in (0, 1, 2, 3):
         assert emu().emu_check_plan(30, 12, 3, qft.ctypes.data, 480, None, rl) == 0
 
@@ -147,11 +149,26 @@ def test_dependency_scheduling_random_mixed(seed):
 def test_dependency_scheduling_cuts_passes():
     # 20-qubit random layered circuit, 20 layers, 12-qubit tiles with 3 low qubits:
     # in gate order one layer needs about two passes; list scheduling covers
-    # light cones of several layers per pass
+    # light cones of several layers per pass; growing a pass's qubit set by the
+    # ready gate that unlocks the most gates per added qubit (flag 4) covers more
     lib = emu()
     lib.emu_plan_stats.argtypes = [C.c_int] * 3 + [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
     ops = G.random_layered(20, 20, seed=7).ops()
     out = np.zeros((256, 6), np.int32)
     counts = {f: lib.emu_plan_stats(20, 12, 3, ops.ctypes.data, len(ops), None, out.ctypes.data, 256, f)
-              for f in (0, 2, -1)}
-    assert counts[0] == 39 and counts[2] <= 14 and counts[-1] == counts[2], counts
+              for f in (0, 2, 6, -1)}
+    assert counts[0] == 39 and counts[2] <= 14 and counts[6] <= 8, counts
+    assert counts[-1] == min(counts[0], counts[2], counts[6]), counts
+
+
+@pytest.mark.parametrize("n,k,L,seed", [(14, 9, 3, 1), (16, 10, 2, 2), (13, 8, 0, 3), (15, 11, 5, 4)])
+def test_lookahead_scheduling_matches_oracle(n, k, L, seed):
+    # the lookahead variant (flags 4 | 2, with and without relabelling) on
+    # layered circuits and random mixed gates of every kind
+    lib = emu()
+    for ops in (G.random_layered(n, 12, seed=seed).ops(), G.random_mixed(n, 300, seed).ops()):
+        psi = G.random_state(n, seed)
+        want = O.run_ops(psi, n, ops)
+        for flags in (6, 7):
+            got, _ = run(n, k, L, ops, psi, relabel=flags)
+            assert np.max(np.abs(got - want)) < 1e-11

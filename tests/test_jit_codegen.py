@@ -223,3 +223,27 @@ def test_paired_tiles(paired):
     check(n, 8, 2, q, params, relabels=(-1,))
     check(n, 8, 4, G.random_mixed(n, 300, 5).ops(), seed=6, relabels=(-1,), table=1)
     assert "ntiles / 2" not in _prepare(n, 8, 3, G.qft(n).ops(), None, -1, 1)
+
+
+@pytest.mark.parametrize("pairing", [0, 1])
+def test_direct_load_and_store(pairing):
+    """Passes of 4+ stages take the first stage's amplitudes straight from the
+    state and store the last stage's straight back where the lane mapping stays
+    coalesced: plain, paired (complex128) and prefetching (complex64) kernels,
+    literal and structure-keyed forms, against the oracle."""
+    import re
+    lib = emu()
+    lib.emu_jit_set_pairing.argtypes = [C.c_int]
+    lib.emu_jit_set_pairing(pairing)
+    try:
+        n, k, L = 14, 10, 3
+        ops = G.random_layered(n, 12, seed=3).ops()
+        for prec in (0, 1):
+            src = _prepare(n, k, L, ops, None, -1, prec)
+            marks = re.findall(r"direct load (\w+), direct store (\w+)", src)
+            assert any(a == "yes" for a, _ in marks) and any(b == "yes" for _, b in marks), marks
+            check(n, k, L, ops, prec=prec, relabels=(-1,))
+        check(n, k, L, ops, relabels=(-1,), table=1)
+        check(16, 12, 3, G.random_layered(16, 16, seed=5).ops(), relabels=(-1, 0))
+    finally:
+        lib.emu_jit_set_pairing(0)
