@@ -409,6 +409,9 @@ void fusion_shape(const qse_ctx* ctx, int n, int prec, int* k_out, int* L_out) {
 // place). PLAN_DAG: passes by list scheduling over the dependency graph.
 namespace {
 
+// Smallest state (qubits) whose tiles are numbered far apart (see plan_items).
+constexpr int TILE_ORDER_FAR_MIN_QUBITS = 27;
+
 int plan_items(int n, int k, int L, const std::vector<Item>& items, HostPlan& P, int flags) {
     const bool relabel = (flags & PLAN_RELABEL) != 0;
     const uint64_t lowT = (L > 0) ? ((1ULL << L) - 1) : 0;
@@ -496,8 +499,12 @@ int plan_items(int n, int k, int L, const std::vector<Item>& items, HostPlan& P,
         // apart in the state. Measured on 30-qubit complex128 passes against the
         // ascending order: QFT 6.97 -> 6.86 ms, layered circuit 7.50 -> 7.37 ms per
         // pass; interleaving low and high qubits or a rotation was slower (7.27 and
-        // 7.13 ms on QFT).
-        if (epos.size() > 2) std::reverse(epos.begin() + 1, epos.end());
+        // 7.13 ms on QFT). Only for states of 27 qubits or more: a 24-qubit
+        // complex128 state (256 MiB) largely stays in the 256 MB Infinity Cache
+        // from one pass to the next, and the far-apart order lost that (QAOA-24
+        // adjoint un-application passes 0.10 -> 0.15 ms; 64 gradients 0.66 ->
+        // 0.79 s).
+        if (epos.size() > 2 && n >= TILE_ORDER_FAR_MIN_QUBITS) std::reverse(epos.begin() + 1, epos.end());
         std::vector<int> loc(64, -1);
         for (int b = 0; b < k; ++b) {
             fp.tq[b] = (uint32_t)tq[b];
