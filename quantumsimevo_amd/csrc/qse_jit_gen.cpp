@@ -8,8 +8,10 @@
 // decided here: register pairs whose register controls fail and thread-bit
 // factors and terms are emitted only where they apply, coefficients are
 // literals, and products are formed in the generic kernel's order, so the two
-// agree to rounding (only floating-point contraction may differ). This file
-// uses no HIP runtime call: the CPU tests compile its output with a host shim.
+// agree to rounding (floating-point contraction and the deferred scale of
+// H and rotation ops, emit_stages, may differ by a few units in the last
+// place per op). This file uses no HIP runtime call: the CPU tests compile its
+// output with a host shim.
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
@@ -68,6 +70,19 @@ std::string sum2(const std::string& a, const std::string& b) {
 void emit_mul_g(std::ostringstream& o, const std::string& dr, const std::string& di) {
     o << "      { const double nr = gr * " << dr << " - gi * " << di << "; gi = gr * " << di << " + gi * " << dr
       << "; gr = nr; }\n";
+}
+
+// A positive scalar a common to both rows of an uncontrolled 2x2 op whose rows
+// have coefficient magnitudes (a0, a1) and (b0, b1): a equals one magnitude of
+// each row, so with the op divided by a every output is x + c*y, one fused
+// multiply-add per real component. The largest such a is taken (quotients at
+// most 1 in magnitude); false when the rows share no magnitude. A rotation
+// [[c, -s], [s, c]] gives a = max(|c|, |s|).
+bool common_row_scale(double a0, double a1, double b0, double b1, double& a) {
+    a = 0.0;
+    for (double v : {std::fabs(a0), std::fabs(a1)})
+        if (v > a && std::isfinite(v) && (v == std::fabs(b0) || v == std::fabs(b1))) a = v;
+    return a > 0.0;
 }
 
 // One pass as kernel `name`.
@@ -169,6 +184,25 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
         // condition is known at generation time is a renaming, not code. (Whole
         // struct copies for register exchanges made the compiler keep the
         // amplitudes in scratch memory.)
+        //
+        // Deferred scale (literal coefficients only). An uncontrolled H, real or
+        // RX-like op whose rows share a coefficient magnitude a is emitted divided
+        // by a: H as x0 +- x1, a rotation as x0 - t*x1 (one fused multiply-add per
+        // component instead of a product and a multiply-add). Every amplitude of
+        // the tile passes through every such op, so the factors a form one real
+        // scalar for the whole tile. It is applied to the thread's 16 amplitudes
+        // before the pass stores them, and earlier whenever it leaves
+        // [2^-40, 2^40], so the unscaled values stay far from the complex64 range
+        // limits. Rounding differs from the undivided form by a few units in the
+        // last place per op.
+        double dscale = 1.0;
+        auto emit_scale = [&](const int* nmv) {
+            if (dscale == 1.0) return;
+            const std::string f = "(R)" + lit(dscale);
+            for (int j = 0; j < 16; ++j)
+                o << "      ar" << nmv[j] << " *= " << f << "; ai" << nmv[j] << " *= " << f << ";\n";
+            dscale = 1.0;
+        };
         for (int sg = 0; sg < fp.nstages; ++sg) {
             const FStage& S = P.stages[fp.stage_begin + sg];
             o << "    {\n      unsigned toff = 0;\n";
@@ -248,6 +282,13 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
                     o << "      }\n";
                     continue;
                 }
+                // the op divided by a common row scale (deferred scale above), when it has one
+                double fa = 0.0;
+                const bool fac = !guard && op.rctl == 0 &&
+                                 ((op.type == FOP_H) ||
+                                  (!toff && op.type == FOP_REAL && common_row_scale(op.m[0], op.m[2], op.m[4], op.m[6], fa)) ||
+                                  (!toff && op.type == FOP_RXLIKE && common_row_scale(op.m[0], op.m[3], op.m[5], op.m[6], fa)));
+                if (op.type == FOP_H) fa = 0.7071067811865475244;
                 o << (guard ? "      if (" + cond + ") {\n" : std::string("      {\n"));
                 if (toff && op.type != FOP_H) {
                     o << "      const R";
@@ -274,22 +315,29 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
                     }
                     const std::string a = std::to_string(nm[pr.first]), b = std::to_string(nm[pr.second]);
                     o << "      { const R x0 = ar" << a << ", y0 = ai" << a << ", x1 = ar" << b << ", y1 = ai" << b << ";\n";
+                    // coefficients of the emitted form: the op's, or the op's over fa
+                    double m[8];
+                    for (int j = 0; j < 8; ++j) m[j] = fac ? op.m[j] / fa : op.m[j];
                     switch (op.type) {
                         case FOP_H:
-                            o << "        ar" << a << " = H * (x0 + x1); ai" << a << " = H * (y0 + y1); ar" << b
-                              << " = H * (x0 - x1); ai" << b << " = H * (y0 - y1); }\n";
+                            if (fac)
+                                o << "        ar" << a << " = x0 + x1; ai" << a << " = y0 + y1; ar" << b << " = x0 - x1; ai"
+                                  << b << " = y0 - y1; }\n";
+                            else
+                                o << "        ar" << a << " = H * (x0 + x1); ai" << a << " = H * (y0 + y1); ar" << b
+                                  << " = H * (x0 - x1); ai" << b << " = H * (y0 - y1); }\n";
                             break;
                         case FOP_REAL:
-                            o << "        ar" << a << " = " << lin({{op.m[0], "x0"}, {op.m[2], "x1"}}) << "; ai" << a << " = "
-                              << lin({{op.m[0], "y0"}, {op.m[2], "y1"}}) << ";\n        ar" << b << " = "
-                              << lin({{op.m[4], "x0"}, {op.m[6], "x1"}}) << "; ai" << b << " = "
-                              << lin({{op.m[4], "y0"}, {op.m[6], "y1"}}) << "; }\n";
+                            o << "        ar" << a << " = " << lin({{m[0], "x0"}, {m[2], "x1"}}) << "; ai" << a << " = "
+                              << lin({{m[0], "y0"}, {m[2], "y1"}}) << ";\n        ar" << b << " = "
+                              << lin({{m[4], "x0"}, {m[6], "x1"}}) << "; ai" << b << " = "
+                              << lin({{m[4], "y0"}, {m[6], "y1"}}) << "; }\n";
                             break;
                         case FOP_RXLIKE:
-                            o << "        ar" << a << " = " << lin({{op.m[0], "x0"}, {-op.m[3], "y1"}}) << "; ai" << a << " = "
-                              << lin({{op.m[0], "y0"}, {op.m[3], "x1"}}) << ";\n        ar" << b << " = "
-                              << lin({{op.m[6], "x1"}, {-op.m[5], "y0"}}) << "; ai" << b << " = "
-                              << lin({{op.m[6], "y1"}, {op.m[5], "x0"}}) << "; }\n";
+                            o << "        ar" << a << " = " << lin({{m[0], "x0"}, {-m[3], "y1"}}) << "; ai" << a << " = "
+                              << lin({{m[0], "y0"}, {m[3], "x1"}}) << ";\n        ar" << b << " = "
+                              << lin({{m[6], "x1"}, {-m[5], "y0"}}) << "; ai" << b << " = "
+                              << lin({{m[6], "y1"}, {m[5], "x0"}}) << "; }\n";
                             break;
                         default:  // FOP_DENSE: (m00 v0) + (m01 v1), (m10 v0) + (m11 v1)
                             o << "        ar" << a << " = " << sum2(lin({{op.m[0], "x0"}, {-op.m[1], "y0"}}), lin({{op.m[2], "x1"}, {-op.m[3], "y1"}}))
@@ -301,7 +349,12 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
                     }
                 }
                 o << "      }\n";
+                if (fac) {
+                    dscale *= fa;
+                    if (dscale < 0x1p-40 || dscale > 0x1p40) emit_scale(nm);
+                }
             }
+            if (sg == fp.nstages - 1) emit_scale(nm);
             if (sg == fp.nstages - 1 && obase) {
                 for (int j = 0; j < 16; ++j)
                     o << "      { cx t; t.re = ar" << nm[j] << "; t.im = ai" << nm[j] << "; stnt(dst + (" << obase
