@@ -410,7 +410,7 @@ static std::vector<char> plan_key(const qse_ctx* ctx, const qse_state* st, const
 // exchange after each out-of-place pass.
 int run_plan(qse_ctx* ctx, qse_state* st, const HostPlan& plan, int k, int jit_kind) {
     int rc;
-    if (!plan.passes.empty() && plan.passes.back().out_of_place && !st->shadow)
+    if (plan_out_of_place(plan) && !st->shadow)
         return set_error(QSE_ERR_UNSUPPORTED, "out-of-place plan without a second buffer");
     Serialized S;
     serialize(plan, S);
@@ -519,7 +519,7 @@ int run_fused(qse_ctx* ctx, qse_state* st, const qse_op* ops, int nops, const do
     }
     if (passes_out) *passes_out = (int)plan->passes.size();
     if (dry_run) return QSE_OK;
-    const bool needs_shadow = !plan->passes.empty() && plan->passes.back().out_of_place;
+    const bool needs_shadow = plan_out_of_place(*plan);
     if (needs_shadow && !st->shadow) {
         if (hipMalloc(&st->shadow, st->bytes) != hipSuccess) {
             // no room for a second buffer: realise SWAPs in place instead

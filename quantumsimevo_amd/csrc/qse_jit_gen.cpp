@@ -459,12 +459,6 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
 
 }  // namespace
 
-// Passes of at least this many stages take the first stage's amplitudes
-// straight from HBM and store the last stage's straight to HBM where the
-// mappings allow (measured on 30-qubit complex128 passes: 3.5% faster on the
-// 14-pass random layered circuit, whose passes have 3-11 stages; 3% slower on
-// the 3-stage QFT passes, whose lane bits then fall on other tile qubits).
-constexpr int JIT_DIRECT_MIN_STAGES = 4;
 
 // Kernel name of pass `pass`: qse_jit_pass<i> (literal coefficients) or
 // qse_jit_tpass<i> (coefficients from the program buffer), so profiles tell

@@ -28,9 +28,9 @@ def emu():
 
 
 # planner variants: bit 0 relabelling SWAPs, bit 1 dependency-graph scheduling, bit 2
-# lookahead growth of the qubit set;
-# -1 is the library's choice among them
-VARIANTS = (0, 1, 2, 3, 6, 7, -1)
+# lookahead growth of the qubit set, bit 3 out-of-place stores that make the next pass's
+# tile contiguous; -1 is the library's choice among them
+VARIANTS = (0, 1, 2, 3, 6, 7, 8, 10, 11, 14, 15, -1)
 
 
 def run(n, k, L, ops, psi, params=None, relabel=None):
