@@ -60,13 +60,16 @@ static int op_top_qubit(const qse_op& o) {
 }
 
 // Stretches of the list that go to the FP64 matrix cores: maximal runs of
-// consecutive ops confined to qubits 0..m-1 (m <= 6, at least 4) with at least
+// consecutive ops confined to qubits 0..m-1 (m <= 6, at least 5) with at least
 // dense_per_qubit * m ops, on complex128 states of m + 4 qubits or more. Such a
 // run becomes one pass with its composed 2^m x 2^m unitary (qse_apply_dense),
-// which takes the same time whatever the depth (30 qubits: 6.9-9.3 ms), while
-// a fused pass slows with the gates it carries (64 layers on 4-6 qubits:
-// 27-45 ms; 16 layers, about 20 m gates: 8.4-13 ms). Returns [begin, end, m]
-// triples in list order.
+// which takes the same time whatever the depth (30 qubits: 6.9 ms for m = 5,
+// 9.1 ms for m = 6), while a fused pass slows with the gates it carries (64
+// layers on 4-6 qubits: 18-33 ms; 16 layers on 5-6 qubits, about 22 m gates:
+// 9.2-10.3 ms). A run on qubits 0..3 is composed as a 5-qubit unitary: the
+// m = 5 kernel is faster than the m = 4 one (6.9 against 7.3 ms), and 16
+// layers on 4 qubits (88 ops) run faster fused (6.65 ms). Returns [begin, end,
+// m] triples in list order.
 static std::vector<int> dense_runs(const qse_ctx* ctx, const qse_state* st, const qse_op* ops, int nops) {
     std::vector<int> out;
     const int per = ctx->dense_per_qubit;
@@ -75,7 +78,7 @@ static std::vector<int> dense_runs(const qse_ctx* ctx, const qse_state* st, cons
     while (i < nops) {
         int j = i, top = -1;
         while (j < nops && op_top_qubit(ops[j]) < 6) top = std::max(top, op_top_qubit(ops[j++]));
-        const int m = std::max(4, top + 1);
+        const int m = std::max(5, top + 1);
         if (j - i >= per * m && st->n >= m + 4) {
             out.push_back(i);
             out.push_back(j);

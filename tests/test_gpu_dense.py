@@ -92,19 +92,21 @@ def test_rejections(ctx):
 
 
 def test_qse_run_routes_deep_low_qubit_blocks(ctx):
-    """qse_run sends a run of at least 20 m ops on qubits 0..m-1 (m = 4..6) to
-    the matrix cores (one qse_apply_dense pass) and fuses the rest; results
+    """qse_run sends a run of at least 20 m ops on qubits 0..m-1 (m = 5, 6; a run
+    on qubits 0..3 counts as m = 5) to the matrix cores (one qse_apply_dense
+    pass) and fuses the rest; results
     equal the oracle (1e-12) with the routing on and off, and the timing
     families show which path ran."""
     n = 20
     head = G.random_layered(n, 2, seed=1)
     block = G.random_layered(5, 64, seed=2)   # 64 layers on qubits 0..4: 448 ops >= 20 * 5
-    short = G.random_layered(4, 3, seed=3)    # 18 ops < 20 * 4: stays fused
+    short = G.random_layered(4, 3, seed=3)    # 18 ops < 20 * 5: stays fused
+    low4 = G.random_layered(4, 24, seed=5)    # 132 ops on qubits 0..3 >= 20 * 5: one 5-qubit block
     tail = G.random_layered(n, 2, seed=4)
-    ops = np.concatenate([head.ops(), block.ops(), tail.ops(), short.ops()])
+    ops = np.concatenate([head.ops(), block.ops(), tail.ops(), short.ops(), tail.ops(), low4.ops()])
     psi = G.random_state(n, 9)
     want = O.run_ops(psi, n, ops)
-    for per, dense_launches in ((20, 1), (0, 0)):
+    for per, dense_launches in ((20, 2), (0, 0)):
         ctx.set_dense_blocks(per)
         st = ctx.state(n)
         st.upload(psi)
