@@ -520,11 +520,13 @@ int plan_items(int n, int k, int L, const std::vector<Item>& items, HostPlan& P,
         // apart in the state. Measured on 30-qubit complex128 passes against the
         // ascending order: QFT 6.97 -> 6.86 ms, layered circuit 7.50 -> 7.37 ms per
         // pass; interleaving low and high qubits or a rotation was slower (7.27 and
-        // 7.13 ms on QFT). Only for states of 27 qubits or more: a 24-qubit
-        // complex128 state (256 MiB) largely stays in the 256 MB Infinity Cache
-        // from one pass to the next, and the far-apart order lost that (QAOA-24
-        // adjoint un-application passes 0.10 -> 0.15 ms; 64 gradients 0.66 ->
-        // 0.79 s).
+        // 7.13 ms on QFT). Only for states of 27 qubits or more: on 24-qubit
+        // complex128 states the far-apart order was slower (QAOA-24 adjoint
+        // un-application passes 0.10 -> 0.15 ms; 64 gradients 0.66 -> 0.79 s).
+        // Why is not established; it is not reuse of the Infinity Cache between
+        // passes (walking the tiles backwards on every other pass, which would
+        // find what such a cache kept of the previous pass, changed nothing at
+        // 24-25 qubits).
         if (epos.size() > 2 && n >= TILE_ORDER_FAR_MIN_QUBITS) std::reverse(epos.begin() + 1, epos.end());
         std::vector<int> loc(64, -1);
         for (int b = 0; b < k; ++b) {
@@ -934,9 +936,9 @@ bool plan_out_of_place(const HostPlan& P) {
     return false;
 }
 
-// Smallest state (qubits) for which the planner tries PLAN_CONTIG. Below it the
-// state is at most 1 GiB (complex128) and the passes find much of it in the
-// 256 MB Infinity Cache, and the second buffer is not worth its memory.
+// Smallest state (qubits) for which the planner tries PLAN_CONTIG: the states
+// where it was measured (27-33 qubits). Smaller states keep in-place plans and
+// need no second buffer.
 constexpr int CONTIG_MIN_QUBITS = 27;
 
 // The best of the planner variants: fewest passes, then fewest stages, then
