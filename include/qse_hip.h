@@ -363,10 +363,14 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision,
  * (H, or H S^dag, on their qubits, folded into the un-applications), which
  * costs passes where no un-application absorbs the rotation;
  * QSE_ADJ_TERMS_PAIRS sums amplitude pairs, Im<lambda|X_q|psi>, in read-only
- * passes over both states (one per 12 tile qubits); QSE_ADJ_TERMS_AUTO (the
- * default) takes the schedule with fewer passes by the planner's count. The
- * results agree to rounding. */
-enum { QSE_ADJ_TERMS_AUTO = 0, QSE_ADJ_TERMS_ROTATE = 1, QSE_ADJ_TERMS_PAIRS = 2 };
+ * passes over both states (one per 12 tile qubits); QSE_ADJ_TERMS_JOINT runs
+ * the reverse sweep as joint passes over psi and lambda (run-time compiled,
+ * 12-qubit tiles) that un-apply each gate
+ * from both and sum every generator term in the tile at its point (needs
+ * run-time compilation, not QSE_JIT_OFF, and no checkpoints);
+ * QSE_ADJ_TERMS_AUTO (the default) takes the schedule that moves the fewest
+ * state passes by the planner's count. The results agree to rounding. */
+enum { QSE_ADJ_TERMS_AUTO = 0, QSE_ADJ_TERMS_ROTATE = 1, QSE_ADJ_TERMS_PAIRS = 2, QSE_ADJ_TERMS_JOINT = 3 };
 int qse_ctx_set_adjoint_terms(qse_ctx* ctx, int mode);
 /* Plain forward expectation for a batch of parameter sets (used by parameter
  * shift and finite differences). */

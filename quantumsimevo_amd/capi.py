@@ -30,7 +30,7 @@ assert OP_DTYPE.itemsize == 104
 
 FAM_SWEEP, FAM_FUSED, FAM_READOUT, FAM_ADJOINT, FAM_BATCH, FAM_DENSE = range(6)
 JIT_OFF, JIT_AUTO, JIT_ALWAYS = range(3)
-ADJ_TERMS_AUTO, ADJ_TERMS_ROTATE, ADJ_TERMS_PAIRS = range(3)
+ADJ_TERMS_AUTO, ADJ_TERMS_ROTATE, ADJ_TERMS_PAIRS, ADJ_TERMS_JOINT = range(4)
 
 # Every symbol include/qse_hip.h declares (checked by the CPU tests).
 EXPORTS = [
@@ -235,8 +235,10 @@ class Context:
         return k.value, l.value
 
     def set_adjoint_terms(self, mode: int):
-        """ADJ_TERMS_AUTO (default), ADJ_TERMS_ROTATE or ADJ_TERMS_PAIRS: how the
-        adjoint evaluates RX/RY generator terms on 13+ qubit states."""
+        """ADJ_TERMS_AUTO (default), ADJ_TERMS_ROTATE, ADJ_TERMS_PAIRS or
+        ADJ_TERMS_JOINT: how the adjoint evaluates RX/RY generator terms on 13+
+        qubit states (JOINT: the reverse sweep as compiled joint passes over psi
+        and lambda with every generator term summed in the tile)."""
         check(self.lib.qse_ctx_set_adjoint_terms(self.h, int(mode)), "qse_ctx_set_adjoint_terms")
 
     def set_dense_blocks(self, ops_per_qubit: int):

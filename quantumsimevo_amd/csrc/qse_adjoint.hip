@@ -45,6 +45,12 @@ int validate_ops(int n, const qse_op* ops, int nops, const double* params);
 int reduce_zstr_async(qse_ctx* ctx, const qse_state* st, const uint64_t* d_m, const double* d_w, int nterms,
                       double* d_out);
 bool is_zz_triple(const qse_op* ops, int nops, int k);
+struct SlotVal;
+int program_on_device(qse_ctx* ctx, const Serialized& S, int prec, ProgEntry** entry);
+int jit_joint_for(qse_ctx* ctx, const HostPlan& P, const Serialized& S, int prec,
+                  const std::vector<hipFunction_t>** fns);
+int slot_table_async(qse_ctx* ctx, const char* dev, const Serialized& S, const HostPlan& P, int p, uint64_t ntiles,
+                     SlotVal* table);
 
 static constexpr int AD_THREADS = 256;
 static constexpr int DIAG_CHUNK = 16;
@@ -807,6 +813,121 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
     return QSE_OK;
 }
 
+// Tile shape of the joint reverse sweep: 12 qubits with 3 contiguous low
+// qubits; psi and lambda pass through one LDS tile in turn, so two workgroups
+// of 256 threads share a CU (with a tile per state, 11-qubit complex128 tiles
+// left one wave per SIMD and ran at a third of the bandwidth).
+void joint_shape(int n, int prec, int* k, int* L) {
+    (void)prec;
+    *k = std::min(n, 12);
+    *L = std::max(0, std::min(3, *k - FUSE_REG));
+}
+
+// The reverse sweep of one parameter set as joint passes over psi and lambda
+// (build_reverse_plan, run-time compiled by structure, jit_joint_source): the
+// forward run, the energy and lambda = O psi as in adjoint_one, then each
+// chunk's passes un-apply the gates from both states and sum every probe in
+// the tile at its point. Each pass's per-workgroup probe sums are reduced in
+// workgroup order (k_sum_columns) into d_res[*next ...]; `slots` maps each
+// value to its gradient term.
+template <class R>
+int adjoint_joint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double* params,
+                      const uint64_t* hm, const uint64_t* dm, const double* dw, int nterms, double* d_res, int* next,
+                      SetSlots& slots) {
+    const int n = ws.psi->n, prec = ws.psi->prec;
+    int rc = qse_state_set_basis(ctx, ws.psi, 0);
+    if (rc) return rc;
+    rc = run_on(ctx, {ws.psi}, std::vector<qse_op>(ops, ops + nops), params);
+    if (rc) return rc;
+    slots.energy = (*next)++;
+    rc = energy_async<R>(ctx, ws.psi, hm, dm, dw, nterms, d_res + slots.energy);
+    if (rc) return rc;
+    rc = qse_state_copy(ctx, ws.lam, ws.psi);
+    if (rc) return rc;
+    rc = launch_apply_obs<R>(ctx, ws.lam, dm, dw, nterms);
+    if (rc) return rc;
+    int k, L;
+    joint_shape(n, prec, &k, &L);
+    std::vector<HostPlan> chunks;
+    std::vector<ProbeRef> refs;
+    rc = build_reverse_plan(n, k, L, ops, nops, params, chunks, refs);
+    if (rc) return rc;
+    const uint64_t ntiles = 1ULL << (n - k);
+    const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)ctx->num_cus * 16);
+    for (const HostPlan& C : chunks) {
+        if (C.passes.empty()) continue;
+        Serialized S;
+        serialize(C, S);
+        ProgEntry* entry = nullptr;
+        rc = program_on_device(ctx, S, prec, &entry);
+        if (rc) return rc;
+        const std::vector<hipFunction_t>* fns = nullptr;
+        rc = jit_joint_for(ctx, C, S, prec, &fns);
+        if (rc) return rc;
+        int max_slots = 0, max_probes = 0;
+        for (const FPass& fp : C.passes) {
+            max_slots = std::max(max_slots, fp.nslots);
+            max_probes = std::max(max_probes, fp.nprobes);
+            if (fp.k != k) return set_error(QSE_ERR_UNSUPPORTED, "adjoint: joint pass of an unexpected tile size");
+        }
+        void* table = nullptr;
+        if (max_slots > 0) {
+            rc = scratch(ctx, 5, 32 * ntiles * (uint64_t)max_slots, &table);
+            if (rc) return rc;
+        }
+        void* part = nullptr;
+        rc = scratch(ctx, 11, sizeof(double) * (size_t)grid * (size_t)std::max(max_probes, 1), &part);
+        if (rc) return rc;
+        const char* dev = (const char*)entry->dev;
+        for (int p = 0; p < (int)C.passes.size(); ++p) {
+            const FPass& fp = C.passes[p];
+            rc = slot_table_async(ctx, dev, S, C, p, ntiles, (SlotVal*)table);
+            if (rc) return rc;
+            TimedLaunch t;
+            launch_begin(ctx, FAM_ADJOINT, &t);
+            void* pd = ws.psi->d;
+            void* ld = ws.lam->d;
+            unsigned long long nt = ntiles;
+            const void* pv = dev;
+            void* args[] = {&pd, &ld, &table, &nt, &pv, &part};
+            QSE_HIP(hipModuleLaunchKernel((*fns)[p], grid, 1, 1, 1u << (k - FUSE_REG), 1, 1, 0, ctx->stream, args,
+                                          nullptr));
+            launch_end(ctx, &t, 4.0 * (double)amp_bytes(prec) * (double)(1ULL << n));
+            if (fp.nprobes > 0) {
+                k_sum_columns<<<fp.nprobes, AD_THREADS, 0, ctx->stream>>>((const double*)part, (int)grid, fp.nprobes,
+                                                                         d_res + *next + fp.probe_base);
+                QSE_HIP(hipGetLastError());
+            }
+        }
+        for (size_t v = 0; v < C.probe_ids.size(); ++v) {
+            const ProbeRef& r = refs[C.probe_ids[v]];
+            slots.terms.push_back({*next + (int)v, GradTerm{r.param, r.coeff}});
+        }
+        *next += (int)C.probe_ids.size();
+    }
+    return QSE_OK;
+}
+
+// Values adjoint_joint_one writes per parameter set (the energy and every
+// probe), and its state passes in units of one read and one write of a state
+// (two per joint pass); false when the joint schedule does not apply.
+bool joint_schedule(int n, int prec, const qse_op* ops, int nops, const double* params, int* values, double* cost) {
+    int k, L;
+    joint_shape(n, prec, &k, &L);
+    if (k < FUSE_MIN_K) return false;
+    std::vector<HostPlan> chunks;
+    std::vector<ProbeRef> refs;
+    if (build_reverse_plan(n, k, L, ops, nops, params, chunks, refs) != QSE_OK) return false;
+    int v = 1, passes = 0;
+    for (const HostPlan& C : chunks) {
+        v += (int)C.probe_ids.size();
+        passes += (int)C.passes.size();
+    }
+    *values = v;
+    *cost = 2.0 * passes;
+    return true;
+}
+
 // Number of reduction values adjoint_one writes for a schedule.
 int values_per_set(const std::vector<AdjSegment>& segs) {
     int v = 1;
@@ -893,6 +1014,20 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
             prefix.swap(prefixP);
         }
     }
+    // the joint schedule: the reverse sweep as run-time compiled joint passes
+    // over psi and lambda, when asked for, or when it moves fewer state passes
+    bool joint = false;
+    int jvals = 0;
+    if (ctx->adjoint_terms == QSE_ADJ_TERMS_JOINT || ctx->adjoint_terms == QSE_ADJ_TERMS_AUTO) {
+        double jcost = 0.0;
+        const bool ok = ckpt_every <= 0 && ctx->jit_mode != QSE_JIT_OFF && nparams > 0 &&
+                        joint_schedule(nqubits, precision, ops, nops, params, &jvals, &jcost);
+        if (ctx->adjoint_terms == QSE_ADJ_TERMS_JOINT && !ok)
+            return set_error(QSE_ERR_UNSUPPORTED, "adjoint: the joint schedule needs parameters, run-time "
+                                                  "compilation and no checkpoints");
+        joint = ok && (ctx->adjoint_terms == QSE_ADJ_TERMS_JOINT ||
+                       (nqubits >= WHT_MIN_QUBITS && jcost < sweep_cost(nqubits, segs0)));
+    }
     Ws ws;
     ws.ctx = ctx;
     rc = qse_state_create(ctx, nqubits, precision, &ws.psi);
@@ -918,7 +1053,7 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
         QSE_HIP(hipMemcpyAsync(dw, obs_weights, 8 * nterms, hipMemcpyHostToDevice, ctx->stream));
     }
     // every reduction value of the call lands in one device array, read once at the end
-    const int vps = values_per_set(segs0);
+    const int vps = joint ? jvals : values_per_set(segs0);
     void* rbuf;
     rc = scratch(ctx, 8, sizeof(double) * (size_t)vps * batch, &rbuf);
     if (rc) return rc;
@@ -927,6 +1062,15 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
     int next = 0;
     for (int b = 0; b < batch; ++b) {
         const double* pb = params ? params + (size_t)b * nparams : nullptr;
+        if (joint) {
+            const int start = next;
+            rc = (precision == QSE_C128)
+                     ? adjoint_joint_one<double>(ctx, ws, ops, nops, pb, obs_masks, dm, dw, nterms, d_res, &next, slots[b])
+                     : adjoint_joint_one<float>(ctx, ws, ops, nops, pb, obs_masks, dm, dw, nterms, d_res, &next, slots[b]);
+            if (rc) return rc;
+            if (next - start != vps) return set_error(QSE_ERR_UNSUPPORTED, "adjoint: schedule changed between parameter sets");
+            continue;
+        }
         std::vector<AdjSegment> segs;
         rc = schedule(pb, pair_mode, segs, prefix);
         if (rc) return rc;
@@ -956,7 +1100,7 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
 }
 
 int qse_ctx_set_adjoint_terms(qse_ctx* ctx, int mode) {
-    if (!ctx || mode < QSE_ADJ_TERMS_AUTO || mode > QSE_ADJ_TERMS_PAIRS)
+    if (!ctx || mode < QSE_ADJ_TERMS_AUTO || mode > QSE_ADJ_TERMS_JOINT)
         return set_error(QSE_ERR_INVALID_ARG, "bad adjoint term mode");
     ctx->adjoint_terms = mode;
     return QSE_OK;

@@ -403,6 +403,20 @@ static std::vector<char> plan_key(const qse_ctx* ctx, const qse_state* st, const
     return key;
 }
 
+// Per-tile slot values of pass p of a program already on the device (`dev`,
+// laid out as S) into table (ntiles x the pass's slots); nothing for a pass
+// without slots.
+int slot_table_async(qse_ctx* ctx, const char* dev, const Serialized& S, const HostPlan& P, int p, uint64_t ntiles,
+                     SlotVal* table) {
+    const uint64_t work = ntiles * (uint64_t)P.passes[p].nslots;
+    if (work == 0) return QSE_OK;
+    k_slot_table<<<grid_for(ctx, work, 256, 8), 256, 0, ctx->stream>>>(
+        (const FPass*)(dev + S.off[0]), (const FSlot*)(dev + S.off[4]), (const FExtTerm*)(dev + S.off[5]), p, ntiles,
+        table);
+    QSE_HIP(hipGetLastError());
+    return QSE_OK;
+}
+
 // Launch the passes of `plan` (tile qubits k) on st: program upload (ring),
 // run-time specialised kernels when allowed and available (jit_kind: keyed by
 // the exact program, literal coefficients; or keyed by the plan structure,

@@ -186,6 +186,12 @@ int build_reverse_plan(int n, int k, int L, const qse_op* ops, int nops, const d
 std::string jit_source(const HostPlan& P, int prec, const size_t* table_off,
                        const std::vector<bool>* paired = nullptr);
 
+// HIP source of a joint reverse-sweep plan (passes over psi and lambda with
+// probes, build_reverse_plan) as kernels qse_jit_jpass<i>(psi, lam, slot_table,
+// ntiles, pv, part), coefficients read from the program buffer at table_off.
+std::string jit_joint_source(const HostPlan& P, int prec, const size_t* table_off);
+std::string jit_joint_kernel_name(int pass);
+
 // The serialized plan with every value field zeroed (gate coefficients, phase
 // factors and angles, thread-bit factors): plans that differ only in their
 // values (the same circuit at other parameters) share it.

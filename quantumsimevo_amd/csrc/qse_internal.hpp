@@ -81,6 +81,15 @@ struct JitJob {
     double ms = 0.0;
     std::thread worker;
 };
+// Joint reverse-sweep kernels (jit_joint_source) of one plan structure,
+// compiled on first sight.
+struct JointEntry {
+    std::vector<char> key;  // structure_key of the joint plan
+    int prec = 0;
+    uint64_t stamp = 0;
+    hipModule_t mod = nullptr;
+    std::vector<hipFunction_t> fns;
+};
 struct StructEntry {
     std::vector<char> key;  // structure_key of the plan
     int prec = 0;
@@ -132,13 +141,15 @@ struct qse_ctx {
     size_t lds_per_block = 65536;
     hipStream_t stream = nullptr;
     // Device scratch, grown on demand (partials, flags, cumulative arrays).
-    qse::Scratch scratch[11];
+    qse::Scratch scratch[12];
     // Recently run fused programs, their device copies in the program ring.
     std::vector<qse::ProgEntry> prog_cache;
     qse::ProgRing ring;
     uint64_t prog_clock = 0;
     // Kernels by plan structure (the adjoint's and batched expectations' runs).
     std::vector<qse::StructEntry> struct_cache;
+    // Joint reverse-sweep kernels by plan structure (qse_adjoint_expect's joint schedule).
+    std::vector<qse::JointEntry> joint_cache;
     // Tuned tile shapes by gate-list structure (internal runs of the adjoint
     // and batched expectations).
     std::vector<qse::ShapeEntry> shape_cache;

@@ -24,8 +24,9 @@ namespace qse {
 
 std::string jit_kernel_name(int pass, bool table);
 
-// Compile `src` for gfx950 and load it; one function per pass (names per jit_kernel_name).
-int jit_compile(const std::string& src, int npasses, bool table, hipModule_t* mod, std::vector<hipFunction_t>& fns) {
+// Compile `src` for gfx950 and load it; one function per pass, named by `name`.
+int jit_compile_named(const std::string& src, int npasses, std::string (*name)(int, bool), bool table,
+                      hipModule_t* mod, std::vector<hipFunction_t>& fns) {
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "qse_jit_program.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
         return set_error(QSE_ERR_HIP, "hiprtcCreateProgram failed");
@@ -48,7 +49,7 @@ int jit_compile(const std::string& src, int npasses, bool table, hipModule_t* mo
     QSE_HIP(hipModuleLoadData(mod, code.data()));
     fns.assign(npasses, nullptr);
     for (int p = 0; p < npasses; ++p) {
-        const hipError_t e = hipModuleGetFunction(&fns[p], *mod, jit_kernel_name(p, table).c_str());
+        const hipError_t e = hipModuleGetFunction(&fns[p], *mod, name(p, table).c_str());
         if (e != hipSuccess) {
             hipModuleUnload(*mod);
             *mod = nullptr;
@@ -57,6 +58,11 @@ int jit_compile(const std::string& src, int npasses, bool table, hipModule_t* mo
         }
     }
     return QSE_OK;
+}
+
+// One function per pass, named per jit_kernel_name.
+int jit_compile(const std::string& src, int npasses, bool table, hipModule_t* mod, std::vector<hipFunction_t>& fns) {
+    return jit_compile_named(src, npasses, jit_kernel_name, table, mod, fns);
 }
 
 // Tile-index bits from which complex128 passes are first generated with
@@ -201,6 +207,43 @@ int jit_for_structure(qse_ctx* ctx, const HostPlan& P, const Serialized& S, int 
     return QSE_OK;
 }
 
+// Joint reverse-sweep kernels of plan P (the device program S), compiled on
+// the first sight of its structure and kept (at most JOINT_CACHE_MAX
+// structures, least recently used out).
+constexpr size_t JOINT_CACHE_MAX = 16;
+int jit_joint_for(qse_ctx* ctx, const HostPlan& P, const Serialized& S, int prec,
+                  const std::vector<hipFunction_t>** fns) {
+    *fns = nullptr;
+    std::vector<char> key = structure_key(P);
+    for (auto& e : ctx->joint_cache)
+        if (e.prec == prec && e.key == key) {
+            e.stamp = ++ctx->prog_clock;
+            *fns = &e.fns;
+            return QSE_OK;
+        }
+    if (ctx->joint_cache.size() >= JOINT_CACHE_MAX) {
+        size_t oldest = 0;
+        for (size_t i = 1; i < ctx->joint_cache.size(); ++i)
+            if (ctx->joint_cache[i].stamp < ctx->joint_cache[oldest].stamp) oldest = i;
+        QSE_HIP(hipStreamSynchronize(ctx->stream));  // its kernels may still be queued
+        if (ctx->joint_cache[oldest].mod) QSE_HIP(hipModuleUnload(ctx->joint_cache[oldest].mod));
+        ctx->joint_cache.erase(ctx->joint_cache.begin() + oldest);
+    }
+    JointEntry e;
+    e.key = std::move(key);
+    e.prec = prec;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = jit_compile_named(jit_joint_source(P, prec, S.off), (int)P.passes.size(),
+                                     [](int p, bool) { return jit_joint_kernel_name(p); }, true, &e.mod, e.fns);
+    if (rc) return rc;
+    ++ctx->jit_compiled;
+    ctx->jit_compile_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    e.stamp = ++ctx->prog_clock;
+    ctx->joint_cache.push_back(std::move(e));
+    *fns = &ctx->joint_cache.back().fns;
+    return QSE_OK;
+}
+
 // Wait for every background compilation of the context.
 int jit_wait_all(qse_ctx* ctx) {
     for (auto& e : ctx->struct_cache)
@@ -217,6 +260,9 @@ void jit_release_all(qse_ctx* ctx) {
         e.job->mod = nullptr;
     }
     ctx->struct_cache.clear();
+    for (auto& e : ctx->joint_cache)
+        if (e.mod) hipModuleUnload(e.mod);
+    ctx->joint_cache.clear();
 }
 
 }  // namespace qse

@@ -457,6 +457,279 @@ void emit_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::stri
     o << "    __syncthreads();\n  }\n}\n\n";
 }
 
+// One joint pass of an adjoint reverse sweep as kernel `name`: the same ops on
+// the tiles of psi and lambda (both in place; the program's coefficients read
+// from pv at `toff`), and each FOP_PROBE op's Im<lambda|G|psi> summed over the
+// tile at its point: per thread over its 16 register amplitudes (in the
+// storage precision), over the wave by shuffles, into the wave's LDS
+// accumulator; at the end each workgroup writes its sums (waves in order) to
+// part[blockIdx.x * nprobes + probe]. The probes follow the batch engine's
+// definitions (qse_batch.hip, probe_regs): X/Y on a register position, Z-string
+// and projector masks over register, thread and external bits.
+void emit_joint_pass(std::ostringstream& o, const HostPlan& P, int pi, const std::string& name, const size_t* toff) {
+    const FPass& fp = P.passes[pi];
+    auto opv = [&](uint32_t oi, int j) {
+        return "pv[" + std::to_string((toff[2] + oi * sizeof(FOp) + offsetof(FOp, m)) / 8 + j) + "]";
+    };
+    auto termv = [&](uint32_t ti, int c) {
+        return "pv[" + std::to_string((toff[3] + ti * sizeof(FTerm) + offsetof(FTerm, d)) / 8 + c) + "]";
+    };
+    auto alphav = [&](uint32_t ai) { return "pv[" + std::to_string(toff[6] / 8 + ai) + "]"; };
+    const int K = fp.k, TB = K - 4, NT = 1 << TB;
+    const int nslots = fp.nslots, np = fp.nprobes;
+    const int NW = NT >= 64 ? NT / 64 : 1;
+    const int LDS = (1 << K) + (1 << (K - 4));
+    o << "extern \"C\" __global__ __launch_bounds__(" << NT << ") __attribute__((amdgpu_waves_per_eu(2, 8)))\n"
+      << "void " << name << "(cx* __restrict__ psi, cx* __restrict__ lam, const SlotVal* __restrict__ slot_table, "
+      << "unsigned long long ntiles, const double* __restrict__ pv, double* __restrict__ part) {\n"
+      << "  __shared__ cx tile[" << LDS << "];\n"
+      << "  __shared__ SlotVal slotv[" << (nslots > 0 ? nslots : 1) << "];\n"
+      << "  __shared__ double pacc[" << (np > 0 ? np * NW : 1) << "];\n"
+      << "  const unsigned tid = threadIdx.x;\n"
+      << "  for (unsigned i = tid; i < " << np * NW << "u; i += " << NT << "u) pacc[i] = 0.0;\n"
+      << "  unsigned long long gt = 0, ot = 0; unsigned st_t = 0;\n";
+    for (int b = 0; b < TB; ++b)
+        o << "  if (tid & " << (1u << b) << "u) { gt |= " << u64(1ULL << fp.tq[b]) << "; ot |= "
+          << u64(1ULL << fp.so_q[b]) << "; st_t |= " << (1u << fp.so_l[b]) << "u; }\n";
+    o << "  const unsigned swz_tid = swz(tid), swz_st = swz(st_t);\n";
+    o << "  for (unsigned long long ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {\n"
+      << "    unsigned tidv = tid;\n    QSE_OPAQUE(tidv);\n"
+      << "    unsigned long long base = 0, obase = 0;\n";
+    for (int e = 0; e < fp.next; ++e)
+        o << "    base |= ((ti >> " << e << ") & 1ull) << " << fp.epos[e] << "; obase |= ((ti >> " << e << ") & 1ull) << "
+          << fp.eout[e] << ";\n";
+    // both tiles' loads at once; psi's amplitudes go through the LDS tile first,
+    // lambda's wait in registers (one tile buffer: two workgroups per CU)
+    o << "    cx ld0, ld1, ld2, ld3, ld4, ld5, ld6, ld7, ld8, ld9, ld10, ld11, ld12, ld13, ld14, ld15;\n"
+      << "    cx lb0, lb1, lb2, lb3, lb4, lb5, lb6, lb7, lb8, lb9, lb10, lb11, lb12, lb13, lb14, lb15;\n";
+    for (int i = 0; i < 16; ++i) {
+        uint64_t off = 0;
+        for (int c = 0; c < 4; ++c)
+            if ((i >> c) & 1) off |= 1ULL << fp.tq[TB + c];
+        o << "    { const unsigned long long g = base | gt | " << u64(off) << "; ld" << i << " = psi[g]; lb" << i
+          << " = lam[g]; }\n";
+    }
+    for (int i = 0; i < 16; ++i) o << "    tile[swz_tid + " << swz_h((uint32_t)(i * NT)) << "u] = ld" << i << ";\n";
+    if (nslots > 0)
+        o << "    for (int sl = tid; sl < " << nslots << "; sl += " << NT << ") slotv[sl] = slot_table[ti * " << nslots
+          << "ull + sl];\n";
+    o << "    __syncthreads();\n";
+    o << "    R";
+    for (int v = 0; v < 16; ++v) o << (v ? ", " : " ") << "ar" << v << ", ai" << v << ", br" << v << ", bi" << v;
+    o << ";\n";
+    for (int sg = 0; sg < fp.nstages; ++sg) {
+        const FStage& S = P.stages[fp.stage_begin + sg];
+        // register j of this stage: LDS offset rj[j] from the thread's offset so
+        auto stage_offsets = [&](const FStage& St, uint32_t* r16) {
+            for (int j = 0; j < 16; ++j) {
+                uint32_t r = 0;
+                for (int q = 0; q < FUSE_REG; ++q)
+                    if ((j >> q) & 1) r |= 1u << St.rmap[q];
+                r16[j] = swz_h(r);
+            }
+        };
+        auto stage_so = [&](const FStage& St, const char* var) {
+            o << "      unsigned " << var << " = 0;\n";
+            for (int t = 0; t < TB; ++t) o << "      if (tidv & " << (1u << t) << "u) " << var << " |= " << (1u << St.tmap[t]) << "u;\n";
+        };
+        o << "    {\n";
+        stage_so(S, "toff");
+        o << "      const unsigned so = swz(toff);\n";
+        uint32_t rj[16];
+        int nm[16];
+        stage_offsets(S, rj);
+        for (int j = 0; j < 16; ++j) nm[j] = j;
+        if (sg == 0) {
+            // psi from the tile; then lambda through the same tile
+            for (int j = 0; j < 16; ++j)
+                o << "      { const cx t = tile[so + " << rj[j] << "u]; ar" << j << " = t.re; ai" << j << " = t.im; }\n";
+            o << "      __syncthreads();\n";
+            for (int i = 0; i < 16; ++i) o << "      tile[swz_tid + " << swz_h((uint32_t)(i * NT)) << "u] = lb" << i << ";\n";
+            o << "      __syncthreads();\n";
+            for (int j = 0; j < 16; ++j)
+                o << "      { const cx t = tile[so + " << rj[j] << "u]; br" << j << " = t.re; bi" << j << " = t.im; }\n";
+        }
+        for (uint32_t oi = S.op_begin; oi < S.op_end; ++oi) {
+            const FOp& op = P.ops[oi];
+            if (op.type == FOP_MONO) {
+                o << "      {\n      double gr = " << opv(oi, 0) << ", gi = " << opv(oi, 1) << ";\n";
+                for (int t = 0; t < TB; ++t)
+                    if ((op.tctl >> t) & 1) {
+                        o << "      if (tidv & " << (1u << t) << "u)\n";
+                        emit_mul_g(o, alphav(op.alpha_begin + 2 * t), alphav(op.alpha_begin + 2 * t + 1));
+                    }
+                for (uint32_t ti = op.term_begin; ti < op.term_end; ++ti) {
+                    const FTerm& T = P.terms[ti];
+                    std::string cond = "((toff & " + std::to_string(T.tmask) + "u) == " + std::to_string(T.tmask) + "u)";
+                    if (T.slot < 0 && T.emask) cond += " && ((base & " + u64(T.emask) + ") == " + u64(T.emask) + ")";
+                    o << "      if (" << cond << ")\n";
+                    if (T.slot >= 0) {
+                        const std::string sv = "slotv[" + std::to_string(T.slot) + "]";
+                        emit_mul_g(o, sv + ".c", sv + ".s");
+                    } else {
+                        emit_mul_g(o, termv(ti, 0), termv(ti, 1));
+                    }
+                }
+                o << "      const R g_r = (R)gr, g_i = (R)gi;\n";
+                for (int j = 0; j < 16; ++j)
+                    if (((uint32_t)j & op.rctl) == op.rctl) {
+                        const std::string v = std::to_string(nm[j]);
+                        for (const char* s2 : {"a", "b"})
+                            o << "      { const R x = " << s2 << "r" << v << ", y = " << s2 << "i" << v << "; " << s2 << "r"
+                              << v << " = g_r * x - g_i * y; " << s2 << "i" << v << " = g_r * y + g_i * x; }\n";
+                    }
+                o << "      }\n";
+                continue;
+            }
+            if (op.type == FOP_PROBE) {
+                o << "      { R v = (R)0;\n";
+                auto w = [&](int j) {
+                    const std::string c = std::to_string(nm[j]);
+                    return "(br" + c + " * ai" + c + " - bi" + c + " * ar" + c + ")";
+                };
+                if (op.mode == PROBE_X || op.mode == PROBE_Y) {
+                    const int pp = (int)op.p;
+                    for (int jj = 0; jj < 8; ++jj) {
+                        const int j0 = ((jj >> pp) << (pp + 1)) | (jj & ((1 << pp) - 1)), j1 = j0 | (1 << pp);
+                        const std::string a0 = std::to_string(nm[j0]), a1 = std::to_string(nm[j1]);
+                        if (op.mode == PROBE_Y)
+                            o << "        v += (br" << a1 << " * ar" << a0 << " + bi" << a1 << " * ai" << a0 << ") - (br" << a0
+                              << " * ar" << a1 << " + bi" << a0 << " * ai" << a1 << ");\n";
+                        else
+                            o << "        v += (br" << a0 << " * ai" << a1 << " - bi" << a0 << " * ar" << a1 << ") + (br" << a1
+                              << " * ai" << a0 << " - bi" << a1 << " * ar" << a0 << ");\n";
+                    }
+                } else if (op.mode == PROBE_PROJ) {
+                    o << "        if (((toff & " << op.tctl << "u) == " << op.tctl << "u) && ((base & " << u64(op.ectl)
+                      << ") == " << u64(op.ectl) << ")) {\n";
+                    for (int j = 0; j < 16; ++j)
+                        if (((uint32_t)j & op.rctl) == op.rctl) o << "          v += " << w(j) << ";\n";
+                    o << "        }\n";
+                } else {
+                    for (int j = 0; j < 16; ++j)
+                        o << "        v " << ((__builtin_popcount((uint32_t)j & op.rctl) & 1) ? "-=" : "+=") << " " << w(j)
+                          << ";\n";
+                    o << "        if ((__popc(toff & " << op.tctl << "u) + __popcll(base & " << u64(op.ectl)
+                      << ")) & 1) v = -v;\n";
+                }
+                o << "        double s = (double)v;\n";
+                if (NT >= 64) {
+                    // DPP row scans and row broadcasts: the wave's sum in lane 63
+                    o << "        s = qse_wsum64(s);\n"
+                      << "        if ((tid & 63u) == 63u) pacc[" << op.term_begin * NW << "u + (tid >> 6)] += s;\n      }\n";
+                } else {
+                    for (int h = NT / 2; h > 0; h >>= 1) o << "        s += __shfl_xor(s, " << h << ", 64);\n";
+                    o << "        if ((tid & 63u) == 0u) pacc[" << op.term_begin * NW << "u + (tid >> 6)] += s;\n      }\n";
+                }
+                continue;
+            }
+            const bool guard = op.tctl != 0 || op.ectl != 0;
+            const std::string cond = "((toff & " + std::to_string(op.tctl) + "u) == " + std::to_string(op.tctl) +
+                                     "u) && ((base & " + u64(op.ectl) + ") == " + u64(op.ectl) + ")";
+            const int pp = (int)op.p;
+            std::vector<std::pair<int, int>> prs;
+            for (int jj = 0; jj < 8; ++jj) {
+                const int j0 = ((jj >> pp) << (pp + 1)) | (jj & ((1 << pp) - 1));
+                if (((uint32_t)j0 & op.rctl) == op.rctl) prs.push_back({j0, j0 | (1 << pp)});
+            }
+            if (op.type == FOP_XSWAP) {
+                if (!guard) {
+                    for (auto& pr : prs) std::swap(nm[pr.first], nm[pr.second]);
+                    continue;
+                }
+                o << "      { const bool c = " << cond << ";\n";
+                for (auto& pr : prs)
+                    for (const char* s2 : {"a", "b"}) {
+                        const std::string a = std::to_string(nm[pr.first]), b = std::to_string(nm[pr.second]);
+                        const std::string xr = std::string(s2) + "r", xi = std::string(s2) + "i";
+                        o << "      { const R x0 = " << xr << a << ", y0 = " << xi << a << ", x1 = " << xr << b << ", y1 = " << xi
+                          << b << ";\n        " << xr << a << " = c ? x1 : x0; " << xi << a << " = c ? y1 : y0; " << xr << b
+                          << " = c ? x0 : x1; " << xi << b << " = c ? y0 : y1; }\n";
+                    }
+                o << "      }\n";
+                continue;
+            }
+            o << (guard ? "      if (" + cond + ") {\n" : std::string("      {\n"));
+            if (op.type != FOP_H) {
+                o << "      const R";
+                for (int j = 0; j < 8; ++j) o << (j ? ", " : " ") << "c" << j << " = (R)" << opv(oi, j);
+                o << ";\n";
+            }
+            for (auto& pr : prs)
+                for (const char* s2 : {"a", "b"}) {
+                    const std::string a = std::to_string(nm[pr.first]), b = std::to_string(nm[pr.second]);
+                    const std::string xr = std::string(s2) + "r", xi = std::string(s2) + "i";
+                    o << "      { const R x0 = " << xr << a << ", y0 = " << xi << a << ", x1 = " << xr << b << ", y1 = " << xi << b
+                      << ";\n";
+                    if (op.type == FOP_H)
+                        o << "        " << xr << a << " = H * (x0 + x1); " << xi << a << " = H * (y0 + y1); " << xr << b
+                          << " = H * (x0 - x1); " << xi << b << " = H * (y0 - y1); }\n";
+                    else if (op.type == FOP_REAL)
+                        o << "        " << xr << a << " = c0 * x0 + c2 * x1; " << xi << a << " = c0 * y0 + c2 * y1;\n        " << xr
+                          << b << " = c4 * x0 + c6 * x1; " << xi << b << " = c4 * y0 + c6 * y1; }\n";
+                    else if (op.type == FOP_RXLIKE)
+                        o << "        " << xr << a << " = c0 * x0 - c3 * y1; " << xi << a << " = c0 * y0 + c3 * x1;\n        " << xr
+                          << b << " = c6 * x1 - c5 * y0; " << xi << b << " = c6 * y1 + c5 * x0; }\n";
+                    else
+                        o << "        " << xr << a << " = (c0 * x0 - c1 * y0) + (c2 * x1 - c3 * y1); " << xi << a
+                          << " = (c0 * y0 + c1 * x0) + (c2 * y1 + c3 * x1);\n        " << xr << b
+                          << " = (c4 * x0 - c5 * y0) + (c6 * x1 - c7 * y1); " << xi << b
+                          << " = (c4 * y0 + c5 * x0) + (c6 * y1 + c7 * x1); }\n";
+                }
+            o << "      }\n";
+        }
+        // each state through the tile in turn: into the next stage's registers,
+        // or out to HBM after the last stage
+        auto put = [&](const char* s2) {
+            for (int j = 0; j < 16; ++j)
+                o << "      { cx t; t.re = " << s2 << "r" << nm[j] << "; t.im = " << s2 << "i" << nm[j] << "; tile[so + " << rj[j]
+                  << "u] = t; }\n";
+        };
+        if (sg + 1 < fp.nstages) {
+            const FStage& N = P.stages[fp.stage_begin + sg + 1];
+            uint32_t rn[16];
+            stage_offsets(N, rn);
+            stage_so(N, "tnx");
+            o << "      const unsigned sn = swz(tnx);\n";
+            for (const char* s2 : {"a", "b"}) {
+                o << "      __syncthreads();\n";
+                put(s2);
+                o << "      __syncthreads();\n";
+                for (int j = 0; j < 16; ++j)
+                    o << "      { const cx t = tile[sn + " << rn[j] << "u]; " << s2 << "r" << j << " = t.re; " << s2 << "i" << j
+                      << " = t.im; }\n";
+            }
+        } else {
+            for (const char* s2 : {"a", "b"}) {
+                o << "      __syncthreads();\n";
+                put(s2);
+                o << "      __syncthreads();\n";
+                const char* st = s2[0] == 'a' ? "psi" : "lam";
+                for (int i = 0; i < 16; ++i) {
+                    uint64_t g = 0;
+                    uint32_t sl = 0;
+                    for (int c = 0; c < 4; ++c)
+                        if ((i >> c) & 1) {
+                            g |= 1ULL << fp.so_q[TB + c];
+                            sl |= 1u << fp.so_l[TB + c];
+                        }
+                    o << "      " << st << "[obase | ot | " << u64(g) << "] = tile[swz_st + " << swz_h(sl) << "u];\n";
+                }
+            }
+            o << "      __syncthreads();\n";
+        }
+        o << "    }\n";
+    }
+    o << "  }\n";
+    // this workgroup's probe sums, waves in order (the tile loop ended with a barrier)
+    if (np > 0) {
+        o << "  for (int p = tid; p < " << np << "; p += " << NT << ") {\n    double v = 0.0;\n";
+        for (int w2 = 0; w2 < NW; ++w2) o << "    v += pacc[p * " << NW << " + " << w2 << "];\n";
+        o << "    part[(unsigned long long)blockIdx.x * " << np << "ull + p] = v;\n  }\n";
+    }
+    o << "}\n\n";
+}
+
 }  // namespace
 
 
@@ -499,6 +772,38 @@ std::string jit_source(const HostPlan& P, int prec, const size_t* table_off, con
         emit_pass(o, P, p, jit_kernel_name(p, table_off != nullptr), prec == QSE_C64, table_off,
                   prec == QSE_C128 && paired && p < (int)paired->size() && (*paired)[p],
                   P.passes[p].nstages >= JIT_DIRECT_MIN_STAGES);
+    return o.str();
+}
+
+// Kernel name of joint pass `pass` (adjoint reverse sweep over psi and lambda).
+std::string jit_joint_kernel_name(int pass) { return "qse_jit_jpass" + std::to_string(pass); }
+
+// HIP source of every pass of the joint reverse-sweep plan P (kernel
+// qse_jit_jpass<i> for pass i; coefficients read from the program buffer at
+// table_off).
+std::string jit_joint_source(const HostPlan& P, int prec, const size_t* table_off) {
+    std::ostringstream o;
+    o << "typedef " << (prec == QSE_C128 ? "double" : "float") << " R;\n"
+      << "struct __attribute__((aligned(2 * sizeof(R)))) cx { R re, im; };\n"
+         "struct SlotVal { double ang, c, s, pad; };\n"
+         "#define H ((R)0.7071067811865475244)\n"
+         "__device__ __forceinline__ unsigned swz(unsigned li) { return li + (li >> 4); }\n"
+         "#ifndef QSE_OPAQUE\n#define QSE_OPAQUE(v) __asm__ volatile(\"\" : \"+v\"(v))\n#endif\n"
+         // Sum of v over the 64 lanes of a wave, in lane 63: a scan over each row
+         // of 16 lanes (row_shr 1, 2, 4, 8), then the row sums chained with
+         // row_bcast:15 and row_bcast:31 (DPP, no LDS round trip). The host shim
+         // of the tests brings its own.
+         "#ifndef QSE_HOST_SHIM\n"
+         "template <int CTRL, int ROWS> __device__ __forceinline__ double qse_dpp(double v) {\n"
+         "  union { double d; unsigned long long u; } x; x.d = v;\n"
+         "  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)x.u, CTRL, ROWS, 0xf, false);\n"
+         "  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(x.u >> 32), CTRL, ROWS, 0xf, false);\n"
+         "  x.u = ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo; return x.d;\n}\n"
+         "__device__ __forceinline__ double qse_wsum64(double v) {\n"
+         "  v += qse_dpp<0x111, 0xf>(v); v += qse_dpp<0x112, 0xf>(v); v += qse_dpp<0x114, 0xf>(v);\n"
+         "  v += qse_dpp<0x118, 0xf>(v); v += qse_dpp<0x142, 0xa>(v); v += qse_dpp<0x143, 0xc>(v);\n"
+         "  return v;\n}\n#endif\n\n";
+    for (int p = 0; p < (int)P.passes.size(); ++p) emit_joint_pass(o, P, p, jit_joint_kernel_name(p), table_off);
     return o.str();
 }
 

@@ -454,6 +454,85 @@ extern "C" int emu_jit_pass_qubits(int pass, uint32_t* tq) {
     return fp.k;
 }
 
+// ---- joint reverse-sweep passes (jit_joint_source) ----
+// emu_joint_prepare builds the reverse sweep of a parameterised list
+// (build_reverse_plan), keeps its chunks and probe references, and returns the
+// generated HIP source of chunk `chunk` (its length; copied into `out` when
+// `cap` allows). The calls below read the kept chunk.
+static std::vector<HostPlan> g_joint_chunks;
+static std::vector<ProbeRef> g_joint_refs;
+static Serialized g_joint_prog;
+static int g_joint_n = 0, g_joint_cur = 0;
+
+extern "C" long emu_joint_prepare(int n, int k, int L, const qse_op* ops, int nops, const double* params, int chunk,
+                                  char* out, long cap) {
+    g_joint_chunks.clear();
+    g_joint_refs.clear();
+    const int rc = build_reverse_plan(n, k, L, ops, nops, params, g_joint_chunks, g_joint_refs);
+    if (rc) return -rc;
+    if (chunk < 0) return (long)g_joint_chunks.size();
+    if (chunk >= (int)g_joint_chunks.size()) return -1;
+    g_joint_n = n;
+    g_joint_cur = chunk;
+    serialize(g_joint_chunks[chunk], g_joint_prog);
+    const std::string src = jit_joint_source(g_joint_chunks[chunk], QSE_C128, g_joint_prog.off);
+    if (out && cap > (long)src.size()) std::memcpy(out, src.c_str(), src.size() + 1);
+    return (long)src.size();
+}
+
+extern "C" long emu_joint_program(char* out) {
+    if (out) std::memcpy(out, g_joint_prog.bytes.data(), g_joint_prog.bytes.size());
+    return (long)g_joint_prog.bytes.size();
+}
+
+// Pass count of the kept chunk (pass < 0), or pass `pass`'s tile qubits,
+// slots, probes and first probe value.
+extern "C" int emu_joint_pass_info(int pass, int* k, int* nslots, int* nprobes, int* probe_base) {
+    const HostPlan& P = g_joint_chunks[g_joint_cur];
+    if (pass < 0) return (int)P.passes.size();
+    const FPass& fp = P.passes[pass];
+    *k = fp.k;
+    *nslots = fp.nslots;
+    *nprobes = fp.nprobes;
+    *probe_base = fp.probe_base;
+    return 0;
+}
+
+// Gradient term of probe value v of the kept chunk: d params[*param] +=
+// *coeff * value.
+extern "C" int emu_joint_probe_ref(int v, int* param, double* coeff) {
+    const HostPlan& P = g_joint_chunks[g_joint_cur];
+    const ProbeRef& r = g_joint_refs[P.probe_ids[v]];
+    *param = r.param;
+    *coeff = r.coeff;
+    return 0;
+}
+
+extern "C" int emu_joint_slot_table(int pass, double* table) {
+    const HostPlan& P = g_joint_chunks[g_joint_cur];
+    const FPass& fp = P.passes[pass];
+    const uint64_t ntiles = 1ULL << (g_joint_n - fp.k);
+    for (uint64_t ti = 0; ti < ntiles; ++ti) {
+        uint64_t base = 0, t = ti;
+        for (int e = 0; e < fp.next; ++e) {
+            base |= (t & 1ULL) << fp.epos[e];
+            t >>= 1;
+        }
+        for (int sl = 0; sl < fp.nslots; ++sl) {
+            const FSlot& S = P.slots[fp.slot_begin + sl];
+            double a = 0.0;
+            for (uint32_t e = S.begin; e < S.end; ++e)
+                if ((base & P.eterms[e].emask) == P.eterms[e].emask) a += P.eterms[e].ang;
+            double* row = table + 4 * (ti * (uint64_t)fp.nslots + sl);
+            row[0] = a;
+            row[1] = std::cos(a);
+            row[2] = std::sin(a);
+            row[3] = 0.0;
+        }
+    }
+    return 0;
+}
+
 // ---- batched small-circuit engine (qse_batch_plan.cpp) ----
 // Executes a batch plan's step lists in FP64 on the host with the kernel's
 // step semantics (qse_batch.hip) and returns energies and gradients through

@@ -28,3 +28,13 @@ void shim_syncthreads();
 #define QSE_OPAQUE(v) __asm__ volatile("" : "+r"(v))
 #define __builtin_nontemporal_load(p) (*(p))
 #define __builtin_nontemporal_store(v, p) (*(p) = (v))
+#define __popc(x) __builtin_popcount(x)
+#define __popcll(x) __builtin_popcountll(x)
+// Cross-lane exchange of the joint passes' probe sums (defined with the joint
+// launch): lanes are the workgroup's threads, exchanged through a buffer
+// between two barriers.
+double __shfl_xor(double v, int mask, int width);
+// The joint passes' wave sum (DPP on the device): here the sum over the
+// thread's group of 64 threads, in every thread.
+#define QSE_HOST_SHIM 1
+double qse_wsum64(double v);

@@ -1,0 +1,80 @@
+"""The adjoint's joint schedule (QSE_ADJ_TERMS_JOINT: the reverse sweep as
+run-time compiled joint passes over psi and lambda, every generator term
+summed in the tile at its point) against the oracle's parameter shift
+(complex128, 1e-10), against the pair and rotated schedules of the same call
+(1e-10), and in complex64 against complex128 (1e-4)."""
+import math
+
+import numpy as np
+import pytest
+
+from quantumsimevo_amd import capi
+from quantumsimevo_amd import circuits as G
+from test_gpu_adjoint import mixed_circuit, oracle_energy, oracle_shift
+
+pytestmark = pytest.mark.gpu
+
+
+def run(ctx, mode, n, prec, ops, params, masks, w):
+    ctx.set_adjoint_terms(mode)
+    try:
+        return ctx.adjoint_expect(n, prec, ops, params, masks, w)
+    finally:
+        ctx.set_adjoint_terms(capi.ADJ_TERMS_AUTO)
+
+
+@pytest.mark.parametrize("n", [14, 16])
+def test_joint_qaoa_vs_oracle_and_other_schedules(ctx, n):
+    print(f"joint schedule QAOA n={n}", flush=True)
+    edges = G.random_regular_graph(n, 3, seed=n)
+    ops = G.qaoa_maxcut(n, edges, 2).ops()
+    masks, w = G.maxcut_observable(edges)
+    params = np.random.default_rng(n).random((3, 4)) * math.pi
+    E, Gj = run(ctx, capi.ADJ_TERMS_JOINT, n, capi.C128, ops, params, masks, w)
+    for b in range(2):
+        assert abs(E[b] - oracle_energy(n, ops, params[b], masks, w)) < 1e-11
+        assert np.max(np.abs(Gj[b] - oracle_shift(n, ops, params[b], masks, w))) < 1e-10
+    for mode in (capi.ADJ_TERMS_PAIRS, capi.ADJ_TERMS_ROTATE):
+        _, Go = run(ctx, mode, n, capi.C128, ops, params, masks, w)
+        assert np.max(np.abs(Gj - Go)) < 1e-10
+    _, G64 = run(ctx, capi.ADJ_TERMS_JOINT, n, capi.C64, ops, params, masks, w)
+    assert np.max(np.abs(G64 - Gj)) < 1e-4
+
+
+def test_joint_every_parameterised_kind(ctx):
+    n = 14
+    print(f"joint schedule mixed circuit n={n}", flush=True)
+    ops, npar = mixed_circuit(n, 3)
+    masks = [1 << q for q in range(n)] + [(1 << 1) | (1 << 12)]
+    w = [1.0] * n + [-0.7]
+    params = np.random.default_rng(4).random((2, npar)) * 2 * math.pi
+    _, Gj = run(ctx, capi.ADJ_TERMS_JOINT, n, capi.C128, ops, params, masks, w)
+    assert np.max(np.abs(Gj[0] - oracle_shift(n, ops, params[0], masks, w))) < 1e-10
+    _, Gp = run(ctx, capi.ADJ_TERMS_PAIRS, n, capi.C128, ops, params, masks, w)
+    assert np.max(np.abs(Gj - Gp)) < 1e-10
+
+
+def test_joint_qaoa22_vs_pairs(ctx):
+    n = 22
+    print(f"joint schedule QAOA n={n}", flush=True)
+    edges = G.random_regular_graph(n, 3, seed=22)
+    ops = G.qaoa_maxcut(n, edges, 3).ops()
+    masks, w = G.maxcut_observable(edges)
+    params = np.random.default_rng(22).random((2, 6)) * math.pi
+    Ej, Gj = run(ctx, capi.ADJ_TERMS_JOINT, n, capi.C128, ops, params, masks, w)
+    Ep, Gp = run(ctx, capi.ADJ_TERMS_PAIRS, n, capi.C128, ops, params, masks, w)
+    assert np.max(np.abs(Ej - Ep)) < 1e-10
+    assert np.max(np.abs(Gj - Gp)) < 1e-10
+
+
+def test_joint_needs_jit(ctx):
+    n = 14
+    edges = G.random_regular_graph(n, 3, seed=1)
+    ops = G.qaoa_maxcut(n, edges, 1).ops()
+    masks, w = G.maxcut_observable(edges)
+    ctx.set_jit(capi.JIT_OFF)
+    try:
+        with pytest.raises(capi.QSEError):
+            run(ctx, capi.ADJ_TERMS_JOINT, n, capi.C128, ops, np.array([0.3, 0.9]), masks, w)
+    finally:
+        ctx.set_jit(capi.JIT_AUTO)
