@@ -705,6 +705,10 @@ void emit_joint_pass(std::ostringstream& o, const HostPlan& P, int pi, const std
                 put(s2);
                 o << "      __syncthreads();\n";
                 const char* st = s2[0] == 'a' ? "psi" : "lam";
+                // store addresses from an opaque copy of the thread's base: equal
+                // to the load addresses in place, which the compiler would
+                // otherwise keep live (32 64-bit addresses) across the whole tile
+                o << "      { unsigned long long ov = obase | ot;\n      QSE_OPAQUE(ov);\n";
                 for (int i = 0; i < 16; ++i) {
                     uint64_t g = 0;
                     uint32_t sl = 0;
@@ -713,8 +717,9 @@ void emit_joint_pass(std::ostringstream& o, const HostPlan& P, int pi, const std
                             g |= 1ULL << fp.so_q[TB + c];
                             sl |= 1u << fp.so_l[TB + c];
                         }
-                    o << "      " << st << "[obase | ot | " << u64(g) << "] = tile[swz_st + " << swz_h(sl) << "u];\n";
+                    o << "      " << st << "[ov | " << u64(g) << "] = tile[swz_st + " << swz_h(sl) << "u];\n";
                 }
+                o << "      }\n";
             }
             o << "      __syncthreads();\n";
         }
