@@ -72,6 +72,62 @@ __global__ __launch_bounds__(AD_THREADS) void k_apply_obs(cx<R>* __restrict__ la
     }
 }
 
+// Observable terms grouped by the top three bits of their masks, u = m >> (n - 3):
+// group u holds entries [ubeg[u], ubeg[u + 1]) of the masks and weights.
+struct ObsGroups {
+    int ubeg[9];
+};
+
+// lambda = O psi and the energy <psi|O|psi> in one pass over psi. A thread
+// takes blocks of 8 amplitudes i = i0 + S j (S = N / 8, i0 < S, j < 8):
+// popcount(i & m) = popcount(i0 & m) + popcount(j & u), so with the group sums
+// G[u] = sum over group u of w_t (-1)^popcount(i0 & m_t), the factor
+// f(i0 + S j) = sum_u G[u] (-1)^popcount(j & u) is an 8-point Walsh-Hadamard
+// transform of G: each term is evaluated once per 8 amplitudes. part[block] =
+// the workgroup's sum of f(i) |psi_i|^2 in FP64 (threads by wave shuffles,
+// waves in order), reduced by k_sum_columns.
+template <class R>
+__global__ __launch_bounds__(AD_THREADS) void k_obs_energy(const cx<R>* __restrict__ psi, cx<R>* __restrict__ lam,
+                                                           uint64_t S, const uint64_t* __restrict__ masks,
+                                                           const double* __restrict__ w, ObsGroups g,
+                                                           double* __restrict__ part) {
+    double e = 0.0;
+    for (uint64_t i0 = gtid(); i0 < S; i0 += gstride()) {
+        double G[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            double s = 0.0;
+            for (int t = g.ubeg[u]; t < g.ubeg[u + 1]; ++t) s += (__popcll(i0 & masks[t]) & 1) ? -w[t] : w[t];
+            G[u] = s;
+        }
+#pragma unroll
+        for (int h = 1; h < 8; h <<= 1)
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (!(j & h)) {
+                    const double a = G[j], b = G[j + h];
+                    G[j] = a + b;
+                    G[j + h] = a - b;
+                }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const cx<R> a = psi[i0 + S * j];
+            const double re = (double)a.re, im = (double)a.im;
+            lam[i0 + S * j] = {(R)(G[j] * re), (R)(G[j] * im)};
+            e += G[j] * (re * re + im * im);
+        }
+    }
+    __shared__ double sh[AD_THREADS / 64];
+    const double v = wave_sum(e);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w2 = 0; w2 < AD_THREADS / 64; ++w2) t += sh[w2];
+        part[blockIdx.x] = t;
+    }
+}
+
 // partial[b][d] = sum_i (-1)^popcount(i & m_d) Im(conj(lambda_i) psi_i)
 template <class R>
 __global__ __launch_bounds__(AD_THREADS) void k_grad_diag(const cx<R>* __restrict__ psi, const cx<R>* __restrict__ lam,
@@ -413,6 +469,25 @@ template <class R>
 int launch_apply_obs(qse_ctx* ctx, qse_state* lam, const uint64_t* dm, const double* dw, int nterms) {
     const uint64_t N = 1ULL << lam->n;
     k_apply_obs<R><<<grid_for(ctx, N, AD_THREADS), AD_THREADS, 0, ctx->stream>>>((cx<R>*)lam->d, N, dm, dw, nterms);
+    QSE_HIP(hipGetLastError());
+    return QSE_OK;
+}
+
+// lambda = O psi and the energy written to *d_out, in one pass over psi
+// (k_obs_energy, then its per-workgroup sums in workgroup order). dm and dw
+// hold the terms sorted by group (ObsGroups); n >= 3.
+template <class R>
+int obs_energy_async(qse_ctx* ctx, const qse_state* psi, qse_state* lam, const uint64_t* dm, const double* dw,
+                     const ObsGroups& g, double* d_out) {
+    const uint64_t S = 1ULL << (psi->n - 3);
+    const unsigned grid = grid_for(ctx, S, AD_THREADS);
+    void* pbuf;
+    int rc = scratch(ctx, 6, sizeof(double) * grid, &pbuf);
+    if (rc) return rc;
+    k_obs_energy<R><<<grid, AD_THREADS, 0, ctx->stream>>>((const cx<R>*)psi->d, (cx<R>*)lam->d, S, dm, dw, g,
+                                                          (double*)pbuf);
+    QSE_HIP(hipGetLastError());
+    k_sum_columns<<<1, AD_THREADS, 0, ctx->stream>>>((const double*)pbuf, (int)grid, 1, d_out);
     QSE_HIP(hipGetLastError());
     return QSE_OK;
 }
@@ -832,20 +907,25 @@ void joint_shape(int n, int prec, int* k, int* L) {
 // value to its gradient term.
 template <class R>
 int adjoint_joint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double* params,
-                      const uint64_t* hm, const uint64_t* dm, const double* dw, int nterms, double* d_res, int* next,
-                      SetSlots& slots) {
+                      const uint64_t* hm, const uint64_t* dm, const double* dw, int nterms, const ObsGroups* og,
+                      double* d_res, int* next, SetSlots& slots) {
     const int n = ws.psi->n, prec = ws.psi->prec;
     int rc = qse_state_set_basis(ctx, ws.psi, 0);
     if (rc) return rc;
     rc = run_on(ctx, {ws.psi}, std::vector<qse_op>(ops, ops + nops), params);
     if (rc) return rc;
     slots.energy = (*next)++;
-    rc = energy_async<R>(ctx, ws.psi, hm, dm, dw, nterms, d_res + slots.energy);
-    if (rc) return rc;
-    rc = qse_state_copy(ctx, ws.lam, ws.psi);
-    if (rc) return rc;
-    rc = launch_apply_obs<R>(ctx, ws.lam, dm, dw, nterms);
-    if (rc) return rc;
+    if (og) {
+        rc = obs_energy_async<R>(ctx, ws.psi, ws.lam, dm, dw, *og, d_res + slots.energy);
+        if (rc) return rc;
+    } else {
+        rc = energy_async<R>(ctx, ws.psi, hm, dm, dw, nterms, d_res + slots.energy);
+        if (rc) return rc;
+        rc = qse_state_copy(ctx, ws.lam, ws.psi);
+        if (rc) return rc;
+        rc = launch_apply_obs<R>(ctx, ws.lam, dm, dw, nterms);
+        if (rc) return rc;
+    }
     int k, L;
     joint_shape(n, prec, &k, &L);
     std::vector<HostPlan> chunks;
@@ -1052,9 +1132,31 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
     if (rc) return rc;
     uint64_t* dm = (uint64_t*)obuf;
     double* dw = (double*)((char*)obuf + 8 * (((size_t)std::max(nterms, 1) + 31) / 32 * 32));
+    // the joint schedule takes the terms grouped by the top three mask bits
+    // (k_obs_energy); the host copies stay alive until the call's final wait
+    ObsGroups og{};
+    const ObsGroups* ogp = nullptr;
+    std::vector<uint64_t> sm;
+    std::vector<double> sw;
+    const uint64_t* up_m = obs_masks;
+    const double* up_w = obs_weights;
+    if (joint && nqubits >= 3) {
+        for (int u = 0; u < 8; ++u) {
+            og.ubeg[u] = (int)sm.size();
+            for (int t = 0; t < nterms; ++t)
+                if ((int)((obs_masks[t] >> (nqubits - 3)) & 7) == u) {
+                    sm.push_back(obs_masks[t]);
+                    sw.push_back(obs_weights[t]);
+                }
+        }
+        og.ubeg[8] = (int)sm.size();
+        ogp = &og;
+        up_m = sm.data();
+        up_w = sw.data();
+    }
     if (nterms > 0) {
-        QSE_HIP(hipMemcpyAsync(dm, obs_masks, 8 * nterms, hipMemcpyHostToDevice, ctx->stream));
-        QSE_HIP(hipMemcpyAsync(dw, obs_weights, 8 * nterms, hipMemcpyHostToDevice, ctx->stream));
+        QSE_HIP(hipMemcpyAsync(dm, up_m, 8 * nterms, hipMemcpyHostToDevice, ctx->stream));
+        QSE_HIP(hipMemcpyAsync(dw, up_w, 8 * nterms, hipMemcpyHostToDevice, ctx->stream));
     }
     // every reduction value of the call lands in one device array, read once at the end
     const int vps = joint ? jvals : values_per_set(segs0);
@@ -1069,8 +1171,10 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
         if (joint) {
             const int start = next;
             rc = (precision == QSE_C128)
-                     ? adjoint_joint_one<double>(ctx, ws, ops, nops, pb, obs_masks, dm, dw, nterms, d_res, &next, slots[b])
-                     : adjoint_joint_one<float>(ctx, ws, ops, nops, pb, obs_masks, dm, dw, nterms, d_res, &next, slots[b]);
+                     ? adjoint_joint_one<double>(ctx, ws, ops, nops, pb, obs_masks, dm, dw, nterms, ogp, d_res, &next,
+                                                 slots[b])
+                     : adjoint_joint_one<float>(ctx, ws, ops, nops, pb, obs_masks, dm, dw, nterms, ogp, d_res, &next,
+                                                slots[b]);
             if (rc) return rc;
             if (next - start != vps) return set_error(QSE_ERR_UNSUPPORTED, "adjoint: schedule changed between parameter sets");
             continue;
