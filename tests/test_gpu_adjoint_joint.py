@@ -78,3 +78,27 @@ def test_joint_needs_jit(ctx):
             run(ctx, capi.ADJ_TERMS_JOINT, n, capi.C128, ops, np.array([0.3, 0.9]), masks, w)
     finally:
         ctx.set_jit(capi.JIT_AUTO)
+
+
+@pytest.mark.parametrize("n", [9, 14])
+def test_joint_observable_groups(ctx, n):
+    """The joint schedule's start, lambda = O psi and the energy in one pass
+    (k_obs_energy: terms grouped by their three highest mask bits, expanded by
+    an 8-point Walsh-Hadamard transform): a constant term (mask 0), terms on the
+    top qubits alone, across groups and below them, against the oracle and the
+    rotated-frame schedule; an empty observable gives zero energy and
+    gradient."""
+    print(f"joint schedule observable groups n={n}", flush=True)
+    ops, npar = mixed_circuit(n, n + 3)
+    top = n - 1
+    masks = [0, 1 << top, (1 << top) | (1 << (top - 1)) | 1, 1 << (top - 2), (1 << 2) | (1 << (top - 1)), 1 << 3]
+    w = [0.7, -1.2, 0.4, 0.9, -0.35, 0.5]
+    params = np.random.default_rng(n + 1).random((2, npar)) * 2 * math.pi
+    Ej, Gj = run(ctx, capi.ADJ_TERMS_JOINT, n, capi.C128, ops, params, masks, w)
+    for b in range(2):
+        assert abs(Ej[b] - oracle_energy(n, ops, params[b], masks, w)) < 1e-11
+    assert np.max(np.abs(Gj[0] - oracle_shift(n, ops, params[0], masks, w))) < 1e-10
+    Er, Gr = run(ctx, capi.ADJ_TERMS_ROTATE, n, capi.C128, ops, params, masks, w)
+    assert np.max(np.abs(Ej - Er)) < 1e-11 and np.max(np.abs(Gj - Gr)) < 1e-10
+    E0, G0 = run(ctx, capi.ADJ_TERMS_JOINT, n, capi.C128, ops, params, [], [])
+    assert np.all(E0 == 0.0) and np.all(G0 == 0.0)
