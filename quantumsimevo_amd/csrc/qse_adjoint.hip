@@ -783,8 +783,8 @@ struct SetSlots {
 template <class R>
 int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double* params,
                 const std::vector<AdjSegment>& segs, const std::vector<qse_op>& prefix, const uint64_t* hm,
-                const uint64_t* dm, const double* dw, int nterms, int ckpt_every, double* d_res, int* next,
-                SetSlots& slots) {
+                const uint64_t* dm, const double* dw, int nterms, const ObsGroups* og, int ckpt_every, double* d_res,
+                int* next, SetSlots& slots) {
     int rc = qse_state_set_basis(ctx, ws.psi, 0);
     if (rc) return rc;
     const int S = (int)segs.size();
@@ -808,13 +808,17 @@ int adjoint_one(qse_ctx* ctx, Ws& ws, const qse_op* ops, int nops, const double*
         if (rc) return rc;
     }
     slots.energy = (*next)++;
-    rc = energy_async<R>(ctx, ws.psi, hm, dm, dw, nterms, d_res + slots.energy);
-    if (rc) return rc;
-    rc = qse_state_copy(ctx, ws.lam, ws.psi);
-    if (rc) return rc;
-    rc = (ws.lam->prec == QSE_C128) ? launch_apply_obs<double>(ctx, ws.lam, dm, dw, nterms)
-                                    : launch_apply_obs<float>(ctx, ws.lam, dm, dw, nterms);
-    if (rc) return rc;
+    if (og) {
+        rc = obs_energy_async<R>(ctx, ws.psi, ws.lam, dm, dw, *og, d_res + slots.energy);
+        if (rc) return rc;
+    } else {
+        rc = energy_async<R>(ctx, ws.psi, hm, dm, dw, nterms, d_res + slots.energy);
+        if (rc) return rc;
+        rc = qse_state_copy(ctx, ws.lam, ws.psi);
+        if (rc) return rc;
+        rc = launch_apply_obs<R>(ctx, ws.lam, dm, dw, nterms);
+        if (rc) return rc;
+    }
     // rotate into the first evaluation point's frame (X/Y generators become Z-strings)
     if (S > 0) {
         rc = run_on(ctx, {ws.psi, ws.lam}, segs[0].pre, nullptr);
@@ -1132,15 +1136,16 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
     if (rc) return rc;
     uint64_t* dm = (uint64_t*)obuf;
     double* dw = (double*)((char*)obuf + 8 * (((size_t)std::max(nterms, 1) + 31) / 32 * 32));
-    // the joint schedule takes the terms grouped by the top three mask bits
-    // (k_obs_energy); the host copies stay alive until the call's final wait
+    // lambda = O psi and the energy in one pass (k_obs_energy) take the terms
+    // grouped by the top three mask bits; the host copies stay alive until the
+    // call's final wait
     ObsGroups og{};
     const ObsGroups* ogp = nullptr;
     std::vector<uint64_t> sm;
     std::vector<double> sw;
     const uint64_t* up_m = obs_masks;
     const double* up_w = obs_weights;
-    if (joint && nqubits >= 3) {
+    if (nqubits >= 3) {
         for (int u = 0; u < 8; ++u) {
             og.ubeg[u] = (int)sm.size();
             for (int t = 0; t < nterms; ++t)
@@ -1185,9 +1190,9 @@ int qse_adjoint_expect(qse_ctx* ctx, int nqubits, int precision, const qse_op* o
         if (values_per_set(segs) != vps)
             return set_error(QSE_ERR_UNSUPPORTED, "adjoint: schedule changed between parameter sets");
         rc = (precision == QSE_C128)
-                 ? adjoint_one<double>(ctx, ws, ops, nops, pb, segs, prefix, obs_masks, dm, dw, nterms, ckpt_every, d_res, &next,
+                 ? adjoint_one<double>(ctx, ws, ops, nops, pb, segs, prefix, obs_masks, dm, dw, nterms, ogp, ckpt_every, d_res, &next,
                                        slots[b])
-                 : adjoint_one<float>(ctx, ws, ops, nops, pb, segs, prefix, obs_masks, dm, dw, nterms, ckpt_every, d_res, &next,
+                 : adjoint_one<float>(ctx, ws, ops, nops, pb, segs, prefix, obs_masks, dm, dw, nterms, ogp, ckpt_every, d_res, &next,
                                       slots[b]);
         if (rc) return rc;
     }
